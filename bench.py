@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""Benchmark: STEM probe.slices/s at 512^2, batched vs plain, on one MI355X.
+
+Workload (the batched HAADF STEM configuration): SrTiO3 [001] tiled 16x16x40
+(51,200 atoms), 512^2 complex64 waves, 200 kV, 20 mrad aberration-free probe,
+64x64 probe raster over one unit cell, 60-200 mrad annular detector, frozen
+phonons with u_rms 0.08 A (seeded Philox displacements), projected Kirkland
+potential, 83 slices of 1.9525 A.
+
+A step is one frozen-phonon configuration of the whole job, with the displaced
+coordinates already generated: GPU atom binning, the 83 band-limited transmission
+functions, and all 4096 probes through 83 slices in batches of B waves with the
+detector reduced in FP64 and accumulated across configurations.  value is the
+whole-job probe.slices/s over the timed steps.
+
+Also reported: the slice-loop rate for B in --sweep, the plain form (one probe per
+launch sequence with a host sync after every slice, as clTEM's doMultiSliceStep
+does), the dominant kernel's roofline from hipEvent timings, and the CPU baseline
+(the FP32 C++ restatement timed on the host cores) with the GPU-vs-FP64 detector
+error on sampled probes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import cltem_amd  # noqa: E402
+from cltem_amd import workloads as wl  # noqa: E402
+
+METRIC = "STEM probe·slices/s at 512², batched vs plain, rel-L2≤1e-4 vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup(engine, w):
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    engine.set_geometry(g)
+    p = cltem_amd.load_params(w["params"])
+    engine.set_parameterisation(p)
+    return g, p
+
+
+def build_config(engine, c, spec):
+    engine.sort_atoms(c, spec.x, spec.y, spec.z, spec.Z, want_ids=False)
+    engine.build_transmission(c)
+
+
+def plain_rate(engine, g, pos, probe, dets):
+    """One probe per launch sequence, host sync after every slice (clTEM's structure)."""
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    ns = g.n_slices
+    out = []
+    engine.synchronize()
+    t0 = time.perf_counter()
+    for p in pos:
+        engine.init_probes(p[None, :], probe)
+        for s in range(ns):
+            engine.multislice(s, 1)
+            engine.synchronize()
+        out.append(engine.stem_detect(dets)[0])
+    engine.synchronize()
+    dt = time.perf_counter() - t0
+    return len(pos) * ns / dt, np.array(out)
+
+
+def roofline(engine, g, pos, batch, probe, dets):
+    N = g.resolution * g.resolution
+    ns = g.n_slices
+    engine.profile_reset()
+    engine.profile(True)
+    engine.stem_batch(0, pos[:batch], batch, probe, dets)
+    engine.profile(False)
+    n_row, t_row = engine.profile_read(0)
+    n_col, t_col = engine.profile_read(1)
+    # algorithmic bytes per launch (8 bytes per complex64):
+    #   row pass: read + write every wave (16N each) and the shared t slice once (8N)
+    #   column pass: read + write every wave; the last slice only reads (detector fused)
+    row_bytes = ns * (batch * 16 * N + 8 * N)
+    col_bytes = (ns - 1) * batch * 16 * N + batch * 8 * N
+    kern = {"row_pass": (n_row, t_row, row_bytes), "column_pass": (n_col, t_col, col_bytes)}
+    name = max(kern, key=lambda k: kern[k][1])
+    n, tms, b = kern[name]
+    achieved = b / (tms * 1e-3) / 1e9
+    return {
+        "kernel": f"{name} (512^2, B={batch})",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "launches": n,
+        "avg_launch_us": round(tms * 1e3 / n, 2),
+        "bytes_per_launch": int(b / n),
+        "row_pass_avg_us": round(t_row * 1e3 / max(n_row, 1), 2),
+        "column_pass_avg_us": round(t_col * 1e3 / max(n_col, 1), 2),
+        "row_pass_GBs": round(row_bytes / (t_row * 1e-3) / 1e9, 1),
+        "column_pass_GBs": round(col_bytes / (t_col * 1e-3) / 1e9, 1),
+    }
+
+
+def cpu_baseline(w, g, spec0, pos, gpu_det, n_probes, n_check):
+    """FP32 C++ restatement timed on the host cores, and the FP64 oracle as checker."""
+    from oracle import oracle_py as orc
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    og = orc.geometry(orc.scenario(**w["scenario"]))
+    p = cltem_amd.load_params(w["params"])
+    t0 = time.perf_counter()
+    srt = orc.sort_atoms(og, spec0.x, spec0.y, spec0.z, spec0.Z)
+    trans = orc.transmission(og, srt, p.values, p.form, p.i_per_atom, nthreads=cores)
+    t_pot = time.perf_counter() - t0
+    P = orc.propagator(og)
+    det4 = np.array([list(d) + [0.0] * (4 - len(d)) for d in w["detectors"]])
+    sample = pos[:n_probes]
+    t0 = time.perf_counter()
+    orc.run_stem(og, trans, P, sample, det4, cond_ap_mrad=w["cond_ap_mrad"], nthreads=cores, fp32=True)
+    t_f32 = time.perf_counter() - t0
+    ref = orc.run_stem(og, trans, P, pos[:n_check], det4, cond_ap_mrad=w["cond_ap_mrad"], nthreads=cores)
+    err = float(np.max(np.abs(gpu_det[:n_check] - ref) / np.abs(ref)))
+    return {
+        "value": round(len(sample) * og.n_slices / t_f32, 2),
+        "unit": "probe·slices/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{len(sample)} probes x {og.n_slices} slices of phonon configuration 0 (FP32 C++ "
+                  f"restatement, std::thread over probes); potential build of 83 slices took "
+                  f"{t_pot:.1f} s in FP64",
+    }, err, n_check
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed frozen-phonon configurations")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--sweep", default="1,16,64,256", help="batch sizes for the slice-loop rate")
+    ap.add_argument("--sweep-probes", type=int, default=256)
+    ap.add_argument("--plain-probes", type=int, default=8)
+    ap.add_argument("--cpu-probes", type=int, default=16)
+    ap.add_argument("--check-probes", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(probes and phonon configurations shard across GPUs with one FP64 reduction, "
+              "see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    w = wl.stem_srtio3()
+    spec = w["specimen"]
+    engine = cltem_amd.Engine(0)
+    dev, cus, _ = engine.device_info()
+    g, p = setup(engine, w)
+    pos = wl.stem_positions(g)
+    probe = cltem_amd.make_probe(w["cond_ap_mrad"])
+    dets = cltem_amd.make_detectors(w["detectors"])
+    N = g.resolution * g.resolution
+    ns = g.n_slices
+    log(f"{dev}, {cus} CUs; {len(spec)} atoms, {g.resolution}^2, {ns} slices, {len(pos)} probes")
+
+    n_cfg = args.warmup + args.steps
+    displaced = [spec.displaced(w["u_rms"], w["seed"], c) for c in range(n_cfg)]
+    accum = np.zeros((len(pos), len(dets)))
+    first_cfg = None
+    for c in range(args.warmup):
+        build_config(engine, 0, displaced[c])
+        engine.stem_batch(0, pos, args.batch, probe, dets, use_graph=bool(args.graph))
+        log(f"warmup configuration {c} done")
+    engine.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        c = args.warmup + k
+        build_config(engine, 0, displaced[c])
+        engine.stem_batch(0, pos, args.batch, probe, dets, use_graph=bool(args.graph), accum=accum)
+        if k == 0:
+            engine.synchronize()
+            first_cfg = accum.copy()
+    engine.synchronize()
+    t_total = time.perf_counter() - t0
+    value = args.steps * len(pos) * ns / t_total
+    log(f"timed {args.steps} configurations in {t_total:.3f} s -> {value:.0f} probe.slices/s")
+
+    # potential build alone (configuration 0 of the timed set)
+    engine.synchronize()
+    t0 = time.perf_counter()
+    build_config(engine, 0, displaced[args.warmup])
+    engine.synchronize()
+    t_build = time.perf_counter() - t0
+
+    # slice-loop rate against B (transmission functions resident)
+    sweep = {}
+    for B in [int(b) for b in args.sweep.split(",") if b]:
+        npr = max(B, args.sweep_probes if B > 1 else min(args.sweep_probes, 64))
+        sub = pos[:npr]
+        engine.stem_batch(0, sub[:B], B, probe, dets)
+        engine.synchronize()
+        t0 = time.perf_counter()
+        engine.stem_batch(0, sub, B, probe, dets, use_graph=bool(args.graph))
+        engine.synchronize()
+        dt = time.perf_counter() - t0
+        sweep[str(B)] = round(len(sub) * ns / dt, 1)
+        log(f"slice loop B={B}: {sweep[str(B)]:.0f} probe.slices/s")
+    plain, _ = plain_rate(engine, g, pos[: args.plain_probes], probe, dets)
+    log(f"plain (B=1, sync per slice): {plain:.0f} probe.slices/s")
+    best_b = max((b for b in sweep if b != "1"), key=lambda b: sweep[b], default=str(args.batch))
+
+    roof = roofline(engine, g, pos, int(best_b), probe, dets)
+    log(f"roofline: {roof}")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        gpu_cfg0 = first_cfg
+        cpu, err, nchk = cpu_baseline(w, g, displaced[args.warmup], pos, gpu_cfg0, args.cpu_probes,
+                                      args.check_probes)
+        parity = {"detector_max_rel_err_vs_fp64": err, "probes_checked": nchk,
+                  "tolerance": 1e-3}
+        log(f"cpu baseline: {cpu}; parity {parity}")
+    engine.close()
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "probe·slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (SrTiO3 16x16x40, seeded Philox frozen phonons)",
+        "config": {
+            "workload": "batched HAADF STEM, SrTiO3 16x16x40 [001], 512^2, 83 slices, 200 kV, "
+                        "20 mrad, 64x64 raster, 60-200 mrad, projected Kirkland",
+            "resolution": g.resolution, "probes": len(pos), "slices": ns, "batch": args.batch,
+            "phonon_configs_timed": args.steps, "atoms": len(spec), "device": dev,
+        },
+        "slice_loop_rate_by_batch": sweep,
+        "plain_rate": round(plain, 1),
+        "batched_over_plain": round(max(sweep.values()) / plain, 2) if plain > 0 else None,
+        "potential_build_ms_per_config": round(1e3 * t_build, 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

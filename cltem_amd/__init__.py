@@ -1,0 +1,238 @@
+"""cltem_amd: MI355X-native multislice engine behind clTEM's simulation interfaces.
+
+The compute path is libcltem_amd.so (hand-written HIP kernels for gfx950 behind the
+C ABI in include/cltem_amd.h).  This module is a thin ctypes binding of that ABI;
+there is no Python or CPU fallback for any computation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import MseDetector, MseGeometry, MseProbe, MseScenario
+from .params import Parameterisation, load as load_params  # noqa: F401
+
+__all__ = ["Engine", "MseError", "plan_geometry", "make_scenario", "make_probe", "make_detectors",
+           "load_params", "Parameterisation"]
+
+
+class MseError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != _abi.MSE_OK:
+        msg = _abi.load().mse_last_error()
+        raise MseError(rc, msg.decode() if msg else "unknown error")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def make_scenario(**kw) -> MseScenario:
+    """Fill an mse_scenario; keyword names follow the struct fields."""
+    s = MseScenario()
+    s.mode = kw.get("mode", 1)
+    s.resolution = kw["resolution"]
+    s.voltage_kv = kw.get("voltage_kv", 200.0)
+    s.slice_thickness = kw.get("slice_thickness", 1.0)
+    s.slice_offset = kw.get("slice_offset", 0.0)
+    s.pad_xy[:] = kw.get("pad_xy", (-8.0, 8.0))
+    s.pad_z[:] = kw.get("pad_z", (-3.0, 3.0))
+    s.struct_min[:] = kw["struct_min"]
+    s.struct_max[:] = kw["struct_max"]
+    s.area[:] = kw["area"]
+    s.area_padding = kw.get("area_padding", 0.0)
+    s.stem_pixels[:] = kw.get("stem_pixels", (0, 0))
+    s.beam_tilt_mrad = kw.get("beam_tilt_mrad", 0.0)
+    s.beam_azimuth = kw.get("beam_azimuth", 0.0)
+    s.limit_factor = kw.get("limit_factor", 2.0 / 3.0)
+    s.full3d = int(kw.get("full3d", 0))
+    s.full3d_integrals = kw.get("full3d_integrals", 20)
+    return s
+
+
+def plan_geometry(s: MseScenario) -> MseGeometry:
+    g = MseGeometry()
+    _check(_abi.load().mse_plan_geometry(C.byref(s), C.byref(g)))
+    return g
+
+
+def make_probe(cond_ap_mrad=20.0, smooth_mrad=0.0, taper=False, aberrations=None) -> MseProbe:
+    """aberrations: dict like {"C10": 10.0, "C12": (mag, ang), "C30": 1e4, ...} in Angstrom/rad;
+    complex terms are given as (magnitude, angle) like ComplexAberration."""
+    p = MseProbe()
+    ab = aberrations or {}
+    for name in ("C10", "C30", "C50"):
+        setattr(p, name, float(ab.get(name, 0.0)))
+    for name in ("C12", "C21", "C23", "C32", "C34", "C41", "C43", "C45", "C52", "C54", "C56"):
+        mag, ang = ab.get(name, (0.0, 0.0))
+        z = mag * np.exp(1j * ang)
+        getattr(p, name)[:] = (z.real, z.imag)
+    p.cond_ap_mrad = cond_ap_mrad
+    p.smooth_mrad = smooth_mrad
+    p.edge_taper = int(taper)
+    return p
+
+
+def probe_aberration_vector(p: MseProbe) -> np.ndarray:
+    """The 25 doubles of an mse_probe's aberrations in oracle order."""
+    out = [p.C10, *p.C12, *p.C21, *p.C23, p.C30, *p.C32, *p.C34, *p.C41, *p.C43, *p.C45, p.C50,
+           *p.C52, *p.C54, *p.C56]
+    return np.array(out, dtype=np.float64)
+
+
+def make_detectors(rows):
+    """rows: iterable of (inner, outer, xc, yc) in mrad -> ctypes array."""
+    rows = [tuple(r) + (0.0,) * (4 - len(r)) for r in rows]
+    arr = (MseDetector * len(rows))()
+    for i, r in enumerate(rows):
+        arr[i].inner_mrad, arr[i].outer_mrad, arr[i].xc_mrad, arr[i].yc_mrad = r
+    return arr
+
+
+class Engine:
+    """One engine per GPU (C handle mse_engine*)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        _check(self.lib.mse_create(device, C.byref(h)))
+        self.h = h
+        self.g: MseGeometry | None = None
+        self.n_waves = 0
+
+    def close(self):
+        if self.h:
+            self.lib.mse_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup --------------------------------------------------------------------------
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int(0)
+        bw = C.c_double(0)
+        _check(self.lib.mse_device_info(self.h, name, 256, C.byref(cu), C.byref(bw)))
+        return name.value.decode(), cu.value, bw.value
+
+    def set_geometry(self, g: MseGeometry):
+        _check(self.lib.mse_set_geometry(self.h, C.byref(g)))
+        self.g = g
+
+    def set_parameterisation(self, p: Parameterisation):
+        v = np.ascontiguousarray(p.values, dtype=np.float64)
+        _check(self.lib.mse_set_parameterisation(self.h, p.form, p.i_per_atom, _dp(v), len(v)))
+
+    def sort_atoms(self, config: int, x, y, z, Z, want_ids: bool = True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        n = len(x)
+        g = self.g
+        bs = np.zeros(g.n_slices * g.blocks_x * g.blocks_y + 1, dtype=np.int32)
+        bid = np.zeros(max(n, 1), dtype=np.int32)
+        zid = np.zeros(max(n, 1), dtype=np.int32)
+        nr = C.c_int(0)
+        nullp = C.POINTER(C.c_int)()
+        _check(self.lib.mse_sort_atoms(self.h, config, _dp(x), _dp(y), _dp(z), _ip(Z), n,
+                                       _ip(bid) if want_ids else nullp,
+                                       _ip(zid) if want_ids else nullp, _ip(bs), C.byref(nr)))
+        return dict(block_ids=bid[:n], z_ids=zid[:n], block_start=bs, n_in_range=nr.value)
+
+    def build_transmission(self, config: int, first: int = 0, count: int | None = None):
+        count = self.g.n_slices - first if count is None else count
+        _check(self.lib.mse_build_transmission(self.h, config, first, count))
+
+    def read_transmission(self, config: int, slice_: int) -> np.ndarray:
+        res = self.g.resolution
+        out = np.zeros(2 * res * res, dtype=np.float32)
+        _check(self.lib.mse_read_transmission(self.h, config, slice_,
+                                              out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out.view(np.complex64).reshape(res, res)
+
+    # -- waves --------------------------------------------------------------------------
+    def set_wave_count(self, n: int):
+        _check(self.lib.mse_set_wave_count(self.h, n))
+        self.n_waves = n
+
+    def set_wave_configs(self, configs):
+        c = np.ascontiguousarray(configs, dtype=np.int32)
+        _check(self.lib.mse_set_wave_configs(self.h, _ip(c), len(c)))
+
+    def init_plane_wave(self, value: float = 1.0):
+        _check(self.lib.mse_init_plane_wave(self.h, value))
+
+    def init_probes(self, positions, probe: MseProbe):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
+        _check(self.lib.mse_init_probes(self.h, _dp(pos), len(pos), C.byref(probe)))
+
+    def multislice(self, first: int = 0, count: int | None = None):
+        count = self.g.n_slices - first if count is None else count
+        _check(self.lib.mse_multislice(self.h, first, count))
+
+    def stem_detect(self, detectors, sqrt2: bool = True) -> np.ndarray:
+        out = np.zeros((self.n_waves, len(detectors)))
+        _check(self.lib.mse_stem_detect(self.h, detectors, len(detectors), int(sqrt2), _dp(out)))
+        return out
+
+    def diffraction(self, wave: int = 0) -> np.ndarray:
+        res = self.g.resolution
+        out = np.zeros((res, res))
+        _check(self.lib.mse_diffraction(self.h, wave, _dp(out)))
+        return out
+
+    def exit_wave(self, wave: int = 0) -> np.ndarray:
+        res = self.g.resolution
+        out = np.zeros(2 * res * res)
+        _check(self.lib.mse_exit_wave(self.h, wave, _dp(out)))
+        return out.view(np.complex128).reshape(res, res)
+
+    def stem_batch(self, config: int, positions, batch: int, probe: MseProbe, detectors,
+                   sqrt2: bool = True, use_graph: bool = False, accum=None) -> np.ndarray:
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
+        if accum is None:
+            accum = np.zeros((len(pos), len(detectors)))
+        assert accum.shape == (len(pos), len(detectors)) and accum.dtype == np.float64
+        _check(self.lib.mse_stem_batch(self.h, config, _dp(pos), len(pos), batch, C.byref(probe),
+                                       detectors, len(detectors), int(sqrt2), int(use_graph),
+                                       _dp(accum)))
+        self.n_waves = max(self.n_waves, batch)
+        return accum
+
+    # -- profiling ----------------------------------------------------------------------
+    def profile(self, enable: bool = True):
+        _check(self.lib.mse_profile_enable(self.h, int(enable)))
+
+    def profile_read(self, kernel_id: int):
+        n = C.c_int(0)
+        ms = C.c_double(0)
+        _check(self.lib.mse_profile_read(self.h, kernel_id, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def profile_reset(self):
+        _check(self.lib.mse_profile_reset(self.h))
+
+    def synchronize(self):
+        _check(self.lib.mse_synchronize(self.h))

@@ -1,0 +1,385 @@
+// Atom binning and transmission-function kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mse {
+
+// ---------------------------------------------------------------------------------------
+// Binning.  SimulationGeneral::sortAtoms (microscope/simulationgeneral.cpp:188-328) filters
+// atoms with strict limits on the host, bins them with atom_sort_d
+// (kernels/atom_sort_d.cl:28-69) and fills the bins stably in input order.  Here the
+// filter and the bin formula run per atom in IEEE double with the reference's operation
+// order (no contraction is possible in these expressions and it is disabled anyway), and
+// the stable fill is a stable radix sort of (slice*blocks + block, index).
+// ---------------------------------------------------------------------------------------
+struct BinGeom {
+    double full_x0, full_x1, full_y0, full_y1;  // paddedFullLimitsX/Y
+    double z0, z1;                              // paddedSimLimitsZ
+    double dz;
+    int blocks_x, blocks_y, n_slices;
+};
+
+__global__ void k_bin_atoms(const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, int n, BinGeom g, unsigned* __restrict__ key,
+                            unsigned* __restrict__ idx, int* __restrict__ bid_out,
+                            int* __restrict__ zid_out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned out_key = (unsigned)(g.n_slices * g.blocks_x * g.blocks_y);
+    const double xi = x[i], yi = y[i], zi = z[i];
+    // host filter (simulationgeneral.cpp:206-210): strict against padded limits
+    const bool in_x = xi > g.full_x0 && xi < g.full_x1;
+    const bool in_y = yi > g.full_y0 && yi < g.full_y1;
+    const bool in_z = zi > g.z0 && zi < g.z1;
+    int bid = -1, zid = -1;
+    unsigned k = out_key;
+    if (in_x && in_y && in_z) {
+        const double min_x = g.full_x0, max_x = g.full_x1, min_y = g.full_y0, max_y = g.full_y1;
+        const double max_z = g.z1;
+        if (xi >= min_x && xi <= max_x && yi >= min_y && yi <= max_y) {
+            int bidx = (int)floor((xi - min_x) * (1.0 / (max_x - min_x)) * g.blocks_x);
+            int bidy = (int)floor((yi - min_y) * (1.0 / (max_y - min_y)) * g.blocks_y);
+            int z_id = (int)floor((max_z - zi) * (1.0 / (g.dz)));
+            z_id -= (z_id == g.n_slices);
+            bidx -= (bidx == g.blocks_x);
+            bidy -= (bidy == g.blocks_y);
+            bid = bidx + g.blocks_x * bidy;
+            zid = z_id;
+            if (zid >= 0 && bid >= 0)
+                k = (unsigned)(zid * g.blocks_x * g.blocks_y + bid);
+        }
+    }
+    key[i] = k;
+    idx[i] = (unsigned)i;
+    bid_out[i] = bid;
+    zid_out[i] = zid;
+}
+
+// block_start[k] = first sorted position with key >= k, for k in [0, nkeys]
+__global__ void k_block_starts(const unsigned* __restrict__ sorted_keys, int n, int nkeys,
+                               int* __restrict__ block_start) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nkeys) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (sorted_keys[mid] < (unsigned)k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    block_start[k] = lo;
+}
+
+__global__ void k_gather_atoms(const unsigned* __restrict__ order, int n, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const int* __restrict__ Z, double* __restrict__ sx,
+                               double* __restrict__ sy, double* __restrict__ sz, int* __restrict__ sZ) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned j = order[i];
+    sx[i] = x[j];
+    sy[i] = y[j];
+    sz[i] = z[j];
+    sZ[i] = Z[j];
+}
+
+// ---------------------------------------------------------------------------------------
+// Potentials.  Distances and the selection tests (r_min clamp, r <= 8, full-3D window)
+// are evaluated in double exactly as the `_d` kernels do; the parameterised potential
+// itself is evaluated in float (VALU transcendental cost) and summed in double.
+// ---------------------------------------------------------------------------------------
+
+// Bessel K0/K1, Numerical Recipes in C 2nd ed. section 6.6 (the rational forms with the
+// coefficient tables of kernels/transmission_potentials_projected_f.cl:57-69), in float.
+__device__ __forceinline__ float poly4(float x, float c0, float c1, float c2, float c3, float c4) {
+    return (((c4 * x + c3) * x + c2) * x + c1) * x + c0;
+}
+__device__ __forceinline__ float poly2(float x, float c0, float c1, float c2) {
+    return (c2 * x + c1) * x + c0;
+}
+__device__ __forceinline__ float poly7(float x, const float* c) {
+    float a = c[7];
+#pragma unroll
+    for (int i = 6; i >= 0; --i) a = a * x + c[i];
+    return a;
+}
+
+__device__ __forceinline__ float bessk0f(float x) {
+    if (x <= 1.0f) {
+        const float z = x * x;
+        const float term = poly4(z, 1.0f, 2.346487949187396e-1f, 1.187082088663404e-2f,
+                                 2.150707366040937e-4f, 1.425433617130587e-6f) *
+                           logf(x) /
+                           poly2(1.0f - z, 9.847324170755358e-1f, 1.518396076767770e-2f,
+                                 8.362215678646257e-5f);
+        return poly4(z, 1.159315156584126e-1f, 2.770731240515333e-1f, 2.066458134619875e-2f,
+                     4.574734709978264e-4f, 3.454715527986737e-6f) /
+                   poly2(1.0f - z, 9.836249671709183e-1f, 1.627693622304549e-2f,
+                         9.809660603621949e-5f) -
+               term;
+    }
+    const float pp[8] = {1.253314137315499f, 1.475731032429900e1f, 6.123767403223466e1f,
+                         1.121012633939949e2f, 9.285288485892228e1f, 3.198289277679660e1f,
+                         3.595376024148513f, 6.160228690102976e-2f};
+    const float qq[8] = {1.0f, 1.189963006673403e1f, 5.027773590829784e1f, 9.496513373427093e1f,
+                         8.318077493230258e1f, 3.181399777449301e1f, 4.443672926432041f,
+                         1.408295601966600e-1f};
+    const float z = 1.0f / x;
+    return expf(-x) * poly7(z, pp) / poly7(z, qq) * rsqrtf(x);
+}
+
+__device__ __forceinline__ float bessk1f(float x) {
+    if (x <= 1.0f) {
+        const float z = x * x;
+        const float term = poly4(z, 0.5f, 5.598072040178741e-2f, 1.818666382168295e-3f,
+                                 2.397509908859959e-5f, 1.239567816344855e-7f) *
+                           logf(x) /
+                           poly2(1.0f - z, 9.870202601341150e-1f, 1.292092053534579e-2f,
+                                 5.881933053917096e-5f);
+        return x * (poly4(z, -3.079657578292062e-1f, -8.109417631822442e-2f, -3.477550948593604e-3f,
+                          -5.385594871975406e-5f, -3.110372465429008e-7f) /
+                        poly2(1.0f - z, 9.861813171751389e-1f, 1.375094061153160e-2f,
+                              6.774221332947002e-5f) +
+                    term) +
+               1.0f / x;
+    }
+    const float pp[8] = {1.253314137315502f, 1.457171340220454e1f, 6.063161173098803e1f,
+                         1.147386690867892e2f, 1.040442011439181e2f, 4.356596656837691e1f,
+                         7.265230396353690f, 3.144418558991021e-1f};
+    const float qq[8] = {1.0f, 1.125154514806458e1f, 4.427488496597630e1f, 7.616113213117645e1f,
+                         5.863377227890893e1f, 1.850303673841586e1f, 1.857244676566022f,
+                         2.538540887654872e-2f};
+    const float z = 1.0f / x;
+    return expf(-x) * poly7(z, pp) / poly7(z, qq) * rsqrtf(x);
+}
+
+// Projected potentials of one atom at distance r (kernels/transmission_potentials_projected_f.cl:104-181)
+template <int FORM>
+__device__ __forceinline__ float vproj(const float* __restrict__ p, int ic, float r) {
+    if constexpr (FORM == 0) {  // Kirkland C.20: p = a1,b1,a2,b2,a3,b3,c1,d1,c2,d2,c3,d3
+        const float x = 2.0f * (float)M_PI * r;
+        float suml = 0.f, sumg = 0.f;
+        for (int i = 0; i < ic * 2; i += 2) suml += p[i] * bessk0f(x * sqrtf(p[i + 1]));
+        const float xg = ((float)M_PI * r) * ((float)M_PI * r);
+        for (int i = ic * 2; i < ic * 4; i += 2) {
+            const float d_inv = 1.0f / p[i + 1];
+            sumg += (p[i] * d_inv) * expf(-xg * d_inv);
+        }
+        return 300.8242834f * suml + 150.4121417f * sumg;
+    } else if constexpr (FORM == 1) {  // Peng: a1..a5, b1..b5
+        const float x = ((float)M_PI * r) * ((float)M_PI * r);
+        float sum = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float b_inv = 1.0f / p[i + 5];
+            sum += p[i] * b_inv * expf(-x * b_inv);
+        }
+        return 150.4121417f * sum;
+    } else {  // Lobato: a1..a5, b1..b5
+        const float x = 2.0f * (float)M_PI * r;
+        float sum = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float bir = rsqrtf(p[i + 5]);
+            sum += p[i] * (bir * bir * bir) * (bessk0f(x * bir) + r * bessk1f(x * bir));
+        }
+        return 945.090144399935f * sum;
+    }
+}
+
+// Full-3D potentials (kernels/transmission_potentials_full_3d_f.cl:57-134)
+template <int FORM>
+__device__ __forceinline__ float v3d(const float* __restrict__ p, int ic, float r) {
+    if constexpr (FORM == 0) {
+        const float x = 2.0f * (float)M_PI * r;
+        float suml = 0.f, sumg = 0.f;
+        for (int i = 0; i < ic * 2; i += 2) suml += p[i] * expf(-x * sqrtf(p[i + 1]));
+        const float xg = ((float)M_PI * r) * ((float)M_PI * r);
+        for (int i = ic * 2; i < ic * 4; i += 2) {
+            const float dir = rsqrtf(p[i + 1]);
+            sumg += p[i] * (dir * dir * dir) * expf(-xg / p[i + 1]);
+        }
+        return 150.4121417f / r * suml + 266.5157269f * sumg;
+    } else if constexpr (FORM == 1) {
+        const float x = ((float)M_PI * r) * ((float)M_PI * r);
+        float sum = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float bir = rsqrtf(p[i + 5]);
+            sum += p[i] * (bir * bir * bir) * expf(-x / p[i + 5]);
+        }
+        return 266.5157269f * sum;
+    } else {
+        const float x = (float)M_PI * r;
+        float sum = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float bir = rsqrtf(p[i + 5]);
+            sum += p[i] * (bir * bir * bir) * expf(-2.0f * x * bir) * (sqrtf(p[i + 5]) / x + 1.0f);
+        }
+        return 472.545072199968f * sum;
+    }
+}
+
+struct PotArgs {
+    int res;
+    double pixelscale, startx, starty;
+    double min_x, max_x, min_y, max_y;  // paddedFullLimits
+    int blocks_x, blocks_y, load_x, load_y, load_z, n_slices;
+    double cos_phi, sin_phi, sin_2theta, tan_theta;  // beam tilt geometry (projected)
+    double sigma;        // sigma_eff
+    double dz, z_min;    // full 3D
+    double shift_x, shift_y;
+    int integrals;
+    int per_atom;        // parameters per Z (12 or 10)
+    int i_count;
+};
+
+__device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int gy, int& starti,
+                                                 int& endi, int& startj, int& endj) {
+    // kernels/transmission_potentials_projected_d.cl:227-246 with a 16x16 group
+    const double gsx = 16 * a.pixelscale, gsy = 16 * a.pixelscale;
+    const double group_start_x = a.startx + gx * gsx;
+    const double group_end_x = group_start_x + gsx;
+    const double group_start_y = a.starty + gy * gsy;
+    const double group_end_y = group_start_y + gsy;
+    const double rrx = 1.0 / (a.max_x - a.min_x);
+    const double rry = 1.0 / (a.max_y - a.min_y);
+    starti = (int)fmax(floor(a.blocks_x * (group_start_x - a.min_x) * rrx) - a.load_x, 0.0);
+    endi = (int)fmin(ceil(a.blocks_x * (group_end_x - a.min_x) * rrx) + a.load_x, (double)(a.blocks_x - 1));
+    startj = (int)fmax(floor(a.blocks_y * (group_start_y - a.min_y) * rry) - a.load_y, 0.0);
+    endj = (int)fmin(ceil(a.blocks_y * (group_end_y - a.min_y) * rry) + a.load_y, (double)(a.blocks_y - 1));
+}
+
+__device__ __forceinline__ float2 phase_to_t(double ph) {
+    ph = remainder(ph, 2.0 * M_PI);
+    float s, c;
+    sincosf((float)ph, &s, &c);
+    return make_float2(c, s);
+}
+
+// Transmission function of slices [first, first + gridDim.z): one 16x16 pixel tile per
+// block; the atoms of each block row are staged through LDS in chunks of 256, with no
+// limit on how many a row holds.  out has one res*res plane per slice (relative to first).
+template <int FORM>
+__global__ __launch_bounds__(256) void k_potential_projected(
+    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
+    PotArgs a, int first) {
+#pragma clang fp contract(off)
+    __shared__ double sx[256], sy[256];
+    __shared__ int sZ[256];
+    const int lid = threadIdx.x;
+    const int xid = blockIdx.x * 16 + (lid & 15);
+    const int yid = blockIdx.y * 16 + (lid >> 4);
+    const int slice = first + blockIdx.z;
+    int starti, endi, startj, endj;
+    tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+    int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
+    const double im_pos_x = a.startx + xid * a.pixelscale;
+    const double im_pos_y = a.starty + yid * a.pixelscale;
+    const double r_min = 0.25 * a.pixelscale;
+    double sumz = 0.0;
+    const int nb = a.blocks_x * a.blocks_y;
+    for (int j = startj; j <= endj; ++j) {
+        const int start = bsp[k * nb + a.blocks_x * j + starti];
+        const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
+        for (int c0 = start; c0 < end; c0 += 256) {
+            const int cnt = min(256, end - c0);
+            __syncthreads();
+            if (lid < cnt) {
+                sx[lid] = ax[c0 + lid];
+                sy[lid] = ay[c0 + lid];
+                sZ[lid] = aZ[c0 + lid];
+            }
+            __syncthreads();
+            for (int l = 0; l < cnt; ++l) {
+                const double rad_x = im_pos_x - sx[l];
+                const double rad_y = im_pos_y - sy[l];
+                const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
+                const double zt = z_prime / a.tan_theta;
+                const double x_prime = rad_x + zt * a.cos_phi;
+                const double y_prime = rad_y + zt * a.sin_phi;
+                const double r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
+                if (r2 <= 64.0000000001) {
+                    double rad = sqrt(r2);
+                    if (rad < r_min) rad = r_min;
+                    if (rad <= 8.0)
+                        sumz += (double)vproj<FORM>(params + (size_t)(sZ[l] - 1) * a.per_atom,
+                                                    a.i_count, (float)rad);
+                }
+            }
+        }
+    }
+    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
+}
+
+// transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the
+// slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
+// kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).
+template <int FORM>
+__global__ __launch_bounds__(256) void k_potential_full3d(
+    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
+    const float* __restrict__ params, PotArgs a, int first) {
+#pragma clang fp contract(off)
+    __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int sZ[256];
+    const int lid = threadIdx.x;
+    const int xid = blockIdx.x * 16 + (lid & 15);
+    const int yid = blockIdx.y * 16 + (lid >> 4);
+    const int slice = first + blockIdx.z;
+    int starti, endi, startj, endj;
+    tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+    int topz = slice - a.load_z, bottomz = slice + a.load_z;
+    if (topz < 0) topz = 0;
+    if (bottomz >= a.n_slices) bottomz = a.n_slices - 1;
+    const double int_r = 1.0 / a.integrals;
+    const double current_z = a.z_min + (a.n_slices - slice) * a.dz;
+    const double im_pos_x = a.startx + xid * a.pixelscale;
+    const double im_pos_y = a.starty + yid * a.pixelscale;
+    const double r_min = 0.25 * a.pixelscale;
+    const bool no_shift = (a.shift_x == 0.0 && a.shift_y == 0.0);
+    double sumz = 0.0;
+    const int nb = a.blocks_x * a.blocks_y;
+    for (int k = topz; k <= bottomz; ++k) {
+        for (int j = startj; j <= endj; ++j) {
+            const int start = bsp[k * nb + a.blocks_x * j + starti];
+            const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
+            double p2 = 0.0;
+            for (int c0 = start; c0 < end; c0 += 256) {
+                const int cnt = min(256, end - c0);
+                __syncthreads();
+                if (lid < cnt) {
+                    sx[lid] = ax[c0 + lid];
+                    sy[lid] = ay[c0 + lid];
+                    sz[lid] = az[c0 + lid];
+                    sZ[lid] = aZ[c0 + lid];
+                }
+                __syncthreads();
+                for (int l = 0; l < cnt; ++l) {
+                    double rad_x = im_pos_x - sx[l];
+                    double rad_y = im_pos_y - sy[l];
+                    if (no_shift && rad_x * rad_x + rad_y * rad_y > 64.0) continue;
+                    const float* pz = params + (size_t)(sZ[l] - 1) * a.per_atom;
+                    for (int h = 0; h <= a.integrals; ++h) {
+                        rad_x -= a.shift_x;
+                        rad_y -= a.shift_y;
+                        const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
+                        const double im_pos_z = current_z - h * a.dz * int_r;
+                        const double rad_z = im_pos_z - sz[l];
+                        if (xyrad2 <= 64.0 && rad_z <= 3.0) {
+                            double rad = sqrt(xyrad2 + rad_z * rad_z);
+                            if (rad < r_min) rad = r_min;
+                            const double p1 = (double)v3d<FORM>(pz, a.i_count, (float)rad);
+                            sumz += (h != 0) * (p1 + p2) * 0.5;
+                            p2 = p1;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const double sst = a.dz * int_r;
+    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(sst * a.sigma * sumz);
+}
+
+}  // namespace mse

@@ -1,0 +1,976 @@
+// cltem_amd engine: HIP runtime state and the C ABI of include/cltem_amd.h.
+//
+// One engine = one GPU, one HIP stream (default priority), device buffers owned by
+// the engine and reused across calls, as clTEM's SimulationGeneral owns its clMemory
+// members (microscope/simulationgeneral.h:74-95).  Errors are C++ exceptions inside
+// and become an int status plus a thread-local message at the ABI.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cltem_amd.h"
+#include "atom_kernels.hpp"
+#include "fft_wave.hpp"
+#include "slice_kernels.hpp"
+
+namespace mse_geom {
+void kgrid(int res, double real_scale, double* kx, double* ky, double* bandwidthkmax);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw Error(MSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+#define REQUIRE(cond, code, msg)              \
+    do {                                      \
+        if (!(cond)) throw Error(code, msg);  \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        HIPX(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+};
+
+struct ConfigData {
+    DevBuf<double> x, y, z;
+    DevBuf<int> Z;
+    DevBuf<int> block_start;
+    int n_in_range = 0;
+    bool sorted = false;
+    DevBuf<float2> trans;  // n_slices * N
+};
+
+enum ProfKind { PK_ROW = 0, PK_COL = 1, PK_POT = 2, PK_PROBE = 3, PK_OTHER = 4, PK_COUNT = 5 };
+
+constexpr int kTC = 16;  // columns per column-pass block
+
+}  // namespace
+
+struct mse_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool have_geom = false;
+    mse_geometry g{};
+    int res = 0;
+    size_t N = 0;
+    DevBuf<float2> tw, px, py;
+    DevBuf<int> prop_lim, band_lim;
+    DevBuf<double> kx, ky;
+    std::vector<double> h_kx, h_ky;
+    DevBuf<float> params;
+    int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
+    std::vector<ConfigData> configs;
+    DevBuf<const float2*> cfg_ptrs;
+    int n_waves = 0;
+    DevBuf<float2> waves;
+    DevBuf<int> wave_cfg;
+    std::vector<int> h_wave_cfg;
+    DevBuf<double> probe_pos;
+    DevBuf<unsigned> det_bits;
+    std::vector<double> det_key;  // detectors the bitmasks were built for
+    DevBuf<double> partials;
+    DevBuf<double> det_out;
+    DevBuf<double> diff_img;
+    DevBuf<float2> scratch;
+    // sort scratch
+    DevBuf<double> tmp_xyz;
+    DevBuf<int> tmp_Z, tmp_bid, tmp_zid;
+    DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
+    DevBuf<unsigned char> cub_tmp;
+    // profiling
+    bool prof = false;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[PK_COUNT] = {0};
+    int prof_n[PK_COUNT] = {0};
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        HIPX(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+hipEvent_t take_event(mse_engine* e) {
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev;
+    HIPX(hipEventCreate(&ev));
+    return ev;
+}
+
+template <class F>
+void launch(mse_engine* e, int kind, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e->prof) {
+        a = take_event(e);
+        b = take_event(e);
+        HIPX(hipEventRecord(a, e->stream));
+    }
+    f();
+    HIPX(hipGetLastError());
+    if (e->prof) {
+        HIPX(hipEventRecord(b, e->stream));
+        e->pending.emplace_back(kind, a, b);
+    }
+}
+
+void drain_profile(mse_engine* e) {
+    if (e->pending.empty()) return;
+    HIPX(hipStreamSynchronize(e->stream));
+    for (auto& t : e->pending) {
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, std::get<1>(t), std::get<2>(t)));
+        e->prof_ms[std::get<0>(t)] += ms;
+        e->prof_n[std::get<0>(t)] += 1;
+        e->ev_pool.push_back(std::get<1>(t));
+        e->ev_pool.push_back(std::get<2>(t));
+    }
+    e->pending.clear();
+}
+
+template <class F>
+void dispatch_res(int res, F&& f) {
+    switch (res) {
+        case 256: f(std::integral_constant<int, 256>{}); break;
+        case 384: f(std::integral_constant<int, 384>{}); break;
+        case 512: f(std::integral_constant<int, 512>{}); break;
+        case 768: f(std::integral_constant<int, 768>{}); break;
+        case 1024: f(std::integral_constant<int, 1024>{}); break;
+        default:
+            throw Error(MSE_ERR_UNSUPPORTED,
+                        "resolution " + std::to_string(res) +
+                            " has no FFT plan (supported: 256, 384, 512, 768, 1024)");
+    }
+}
+
+// ---- sweeps ------------------------------------------------------------------------------
+void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets,
+              size_t wave_stride, const mse::RowArgs& args) {
+    dispatch_res(e->res, [&](auto Nc) {
+        constexpr int N = decltype(Nc)::value;
+        using PL = typename mse::PlanFor<N>::P;
+        constexpr int LINES = 256 / PL::L;
+        dim3 grid(N / LINES, nlines_sets);
+        launch(e, PK_ROW, [&]() {
+            switch (mode) {
+                case mse::ROW_IFFT_T_FFT:
+                    mse::k_row<PL, mse::ROW_IFFT_T_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                    break;
+                case mse::ROW_PLANE_T_FFT:
+                    mse::k_row<PL, mse::ROW_PLANE_T_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                    break;
+                case mse::ROW_FFT:
+                    mse::k_row<PL, mse::ROW_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                    break;
+                case mse::ROW_IFFT:
+                    mse::k_row<PL, mse::ROW_IFFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                    break;
+            }
+        });
+    });
+}
+
+void col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
+              const mse::ColArgs& args, int kind = PK_COL) {
+    dispatch_res(e->res, [&](auto Nc) {
+        constexpr int N = decltype(Nc)::value;
+        using PL = typename mse::PlanFor<N>::P;
+        constexpr int NT = kTC * PL::L;
+        dim3 grid(N / kTC, nsets);
+        launch(e, kind, [&]() {
+            switch (mode) {
+#define MSE_COL_CASE(M) \
+    case mse::M: mse::k_col<PL, kTC, mse::M><<<grid, NT, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args); break;
+                MSE_COL_CASE(COL_PROP)
+                MSE_COL_CASE(COL_PROP_DETECT)
+                MSE_COL_CASE(COL_BAND)
+                MSE_COL_CASE(COL_PROBE)
+                MSE_COL_CASE(COL_FFT_DETECT)
+                MSE_COL_CASE(COL_FFT_DIFF)
+                MSE_COL_CASE(COL_PROP_STORE_DETECT)
+#undef MSE_COL_CASE
+            }
+        });
+    });
+}
+
+mse::ColArgs base_col_args(mse_engine* e) {
+    mse::ColArgs a{};
+    a.px = e->px.p;
+    a.py = e->py.p;
+    a.lim = e->prop_lim.p;
+    a.kx = e->kx.p;
+    a.ky = e->ky.p;
+    a.scale = 1.0f;
+    a.det_scale = 1.0;
+    a.ndet = 0;
+    return a;
+}
+
+mse::RowArgs base_row_args(mse_engine* e) {
+    mse::RowArgs a{};
+    a.cfg_trans = e->cfg_ptrs.p;
+    a.wave_cfg = e->wave_cfg.p;
+    a.slice = 0;
+    a.plane_value = 0.f;
+    a.scale = 1.0f;
+    return a;
+}
+
+void require_geom(mse_engine* e) { REQUIRE(e->have_geom, MSE_ERR_STATE, "mse_set_geometry has not been called"); }
+
+void upload_cfg_ptrs(mse_engine* e) {
+    std::vector<const float2*> h(e->configs.size());
+    for (size_t i = 0; i < h.size(); ++i) h[i] = e->configs[i].trans.p;
+    e->cfg_ptrs.ensure(std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+        HIPX(hipMemcpyAsync(e->cfg_ptrs.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, e->stream));
+}
+
+ConfigData& config(mse_engine* e, int c) {
+    REQUIRE(c >= 0 && c < 4096, MSE_ERR_ARG, "configuration index out of range");
+    if ((size_t)c >= e->configs.size()) e->configs.resize(c + 1);
+    return e->configs[c];
+}
+
+// Detector bitmasks over the fft-shifted image, decided in double with band_pass's test
+// (kernels/band_pass_d.cl:98-121) on the radii of getStemPixel (simulationstem.cpp:105-110).
+void ensure_detectors(mse_engine* e, const mse_detector* dets, int n_det) {
+    REQUIRE(n_det >= 0 && n_det <= mse::kMaxDet, MSE_ERR_ARG, "at most 8 detectors");
+    std::vector<double> key;
+    for (int d = 0; d < n_det; ++d) {
+        key.push_back(dets[d].inner_mrad);
+        key.push_back(dets[d].outer_mrad);
+        key.push_back(dets[d].xc_mrad);
+        key.push_back(dets[d].yc_mrad);
+    }
+    key.push_back((double)e->res);
+    key.push_back(e->g.inverse_scale_angle);
+    if (key == e->det_key && e->det_bits.p) return;
+    const int W = e->res, H = e->res;
+    const size_t words = (size_t)W * H / 32;
+    std::vector<unsigned> bits(words * std::max(n_det, 1), 0u);
+    const double angle_scale = e->g.inverse_scale_angle;
+    for (int d = 0; d < n_det; ++d) {
+        const double inner = dets[d].inner_mrad / angle_scale;
+        const double outer = dets[d].outer_mrad / angle_scale;
+        const double centX = W / 2.0 + dets[d].xc_mrad / angle_scale;
+        const double centY = H / 2.0 + dets[d].yc_mrad / angle_scale;
+        for (int yid = 0; yid < H; ++yid)
+            for (int xid = 0; xid < W; ++xid) {
+                const double radius = std::sqrt((xid - centX) * (xid - centX) + (yid - centY) * (yid - centY));
+                if (radius <= outer && radius >= inner) {
+                    const size_t b = (size_t)yid * W + xid;
+                    bits[d * words + (b >> 5)] |= 1u << (b & 31);
+                }
+            }
+    }
+    e->det_bits.ensure(bits.size());
+    HIPX(hipMemcpyAsync(e->det_bits.p, bits.data(), bits.size() * sizeof(unsigned), hipMemcpyHostToDevice, e->stream));
+    HIPX(hipStreamSynchronize(e->stream));
+    e->det_key = key;
+}
+
+int run_guarded(const char* fn, mse_engine* e, const std::function<void()>& body) {
+    try {
+        if (e) {
+            DeviceGuard dg(e->device);
+            body();
+        } else {
+            body();
+        }
+        return MSE_OK;
+    } catch (const Error& err) {
+        g_err = std::string(fn) + ": " + err.what();
+        return err.code;
+    } catch (const std::exception& ex) {
+        g_err = std::string(fn) + ": " + ex.what();
+        return MSE_ERR_STATE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mse_last_error(void) { return g_err.c_str(); }
+
+int mse_create(int device_ordinal, mse_engine** out) {
+    return run_guarded("mse_create", nullptr, [&]() {
+        REQUIRE(out != nullptr, MSE_ERR_ARG, "out is NULL");
+        int count = 0;
+        HIPX(hipGetDeviceCount(&count));
+        REQUIRE(device_ordinal >= 0 && device_ordinal < count, MSE_ERR_ARG,
+                "device ordinal " + std::to_string(device_ordinal) + " not present (" + std::to_string(count) + " devices)");
+        DeviceGuard dg(device_ordinal);
+        auto* e = new mse_engine();
+        e->device = device_ordinal;
+        hipError_t st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+        if (st != hipSuccess) {
+            delete e;
+            throw Error(MSE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(st));
+        }
+        *out = e;
+    });
+}
+
+int mse_destroy(mse_engine* e) {
+    if (!e) return MSE_OK;
+    int rc = run_guarded("mse_destroy", e, [&]() {
+        (void)hipStreamSynchronize(e->stream);
+        for (auto& t : e->pending) {
+            (void)hipEventDestroy(std::get<1>(t));
+            (void)hipEventDestroy(std::get<2>(t));
+        }
+        for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+        e->configs.clear();
+        hipStream_t s = e->stream;
+        delete e;
+        (void)hipStreamDestroy(s);
+    });
+    return rc;
+}
+
+int mse_synchronize(mse_engine* e) {
+    return run_guarded("mse_synchronize", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_device_info(mse_engine* e, char* name, int name_len, int* cu_count, double* hbm_gbs) {
+    return run_guarded("mse_device_info", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        hipDeviceProp_t p;
+        HIPX(hipGetDeviceProperties(&p, e->device));
+        if (name && name_len > 0) {
+            std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (cu_count) *cu_count = p.multiProcessorCount;
+        if (hbm_gbs) *hbm_gbs = 2.0 * p.memoryClockRate * 1e3 * (p.memoryBusWidth / 8) / 1e9;
+    });
+}
+
+int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
+    return run_guarded("mse_set_geometry", e, [&]() {
+        REQUIRE(e && g, MSE_ERR_ARG, "NULL argument");
+        const int res = g->resolution;
+        dispatch_res(res, [](auto) {});
+        REQUIRE(g->n_slices > 0 && g->blocks_x > 0 && g->blocks_y > 0, MSE_ERR_ARG, "geometry not planned");
+        HIPX(hipStreamSynchronize(e->stream));
+        const bool res_changed = res != e->res;
+        e->g = *g;
+        e->res = res;
+        e->N = (size_t)res * res;
+        if (res_changed) {
+            for (auto& c : e->configs) c.trans.release();
+            e->waves.release();
+            e->n_waves = 0;
+            e->det_key.clear();
+        }
+        for (auto& c : e->configs) c.sorted = false;  // bins depend on the geometry
+        // k-grid exactly as the reference builds it (double)
+        e->h_kx.assign(res, 0.0);
+        e->h_ky.assign(res, 0.0);
+        double bw = 0.0;
+        mse_geom::kgrid(res, g->real_scale, e->h_kx.data(), e->h_ky.data(), &bw);
+        // twiddles tw[q] = exp(-2 pi i q / N), evaluated in double
+        std::vector<float2> tw(res);
+        for (int q = 0; q < res; ++q) {
+            const double ang = -2.0 * M_PI * (double)q / (double)res;
+            tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        // propagator (kernels/propagator_d.cl:49-83) split as
+        // f e^{i pi dz (k^2-kz^2)/kz} e^{-i pi dz (bx+kx)^2/kz} e^{-i pi dz (by+ky)^2/kz},
+        // with the two unitary FFT scales 1/res^2 folded into px
+        const double dz = g->slice_thickness;
+        const double k = g->wavenumber, bkx = g->k_vec[0], bky = g->k_vec[1], bkz = g->k_vec[2];
+        const double f = bkz / k;
+        const double c0 = M_PI * dz * (k * k - bkz * bkz) / bkz;
+        const double inv_n = 1.0 / ((double)res * (double)res);
+        std::vector<float2> hpx(res), hpy(res);
+        for (int i = 0; i < res; ++i) {
+            double ax = std::remainder(c0 - M_PI * dz * (bkx + e->h_kx[i]) * (bkx + e->h_kx[i]) / bkz, 2.0 * M_PI);
+            hpx[i] = make_float2((float)(f * inv_n * std::cos(ax)), (float)(f * inv_n * std::sin(ax)));
+            double ay = std::remainder(-M_PI * dz * (bky + e->h_ky[i]) * (bky + e->h_ky[i]) / bkz, 2.0 * M_PI);
+            hpy[i] = make_float2((float)std::cos(ay), (float)std::sin(ay));
+        }
+        // masks, decided in double with the reference formulas, stored as per-row |kx index| limits
+        const double k_max = bw * g->limit_factor;           // simulationgeneral.cpp:633
+        const double k_max_2 = k_max * k_max;                 // propagator_d.cl:69-71 (strict <)
+        const double band_lim = bw * g->limit_factor;         // band_limit_d.cl:24-30 (<=)
+        std::vector<int> plim(res), blim(res);
+        for (int y = 0; y < res; ++y) {
+            int pl = -1, bl = -1;
+            for (int x = 0; x < res; ++x) {
+                const int xs = (x < res / 2) ? x : res - x;
+                const double k0x = e->h_kx[x] * e->h_kx[x];
+                const double k0y = e->h_ky[y] * e->h_ky[y];
+                if (k0x + k0y < k_max_2) pl = std::max(pl, xs);
+                if (std::sqrt(e->h_kx[x] * e->h_kx[x] + e->h_ky[y] * e->h_ky[y]) <= band_lim) bl = std::max(bl, xs);
+            }
+            // the kept sets are {x : |x| <= limit}; verify (monotone in |kx|)
+            for (int x = 0; x < res; ++x) {
+                const int xs = (x < res / 2) ? x : res - x;
+                const double k0x = e->h_kx[x] * e->h_kx[x];
+                const double k0y = e->h_ky[y] * e->h_ky[y];
+                const bool pk = k0x + k0y < k_max_2;
+                const bool bk = std::sqrt(e->h_kx[x] * e->h_kx[x] + e->h_ky[y] * e->h_ky[y]) <= band_lim;
+                REQUIRE(pk == (xs <= pl) && bk == (xs <= bl), MSE_ERR_STATE, "non-radial aperture mask");
+            }
+            plim[y] = pl;
+            blim[y] = bl;
+        }
+        e->tw.ensure(res);
+        e->px.ensure(res);
+        e->py.ensure(res);
+        e->prop_lim.ensure(res);
+        e->band_lim.ensure(res);
+        e->kx.ensure(res);
+        e->ky.ensure(res);
+        HIPX(hipMemcpyAsync(e->tw.p, tw.data(), res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->px.p, hpx.data(), res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->py.p, hpy.data(), res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->prop_lim.p, plim.data(), res * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->band_lim.p, blim.data(), res * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->kx.p, e->h_kx.data(), res * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->ky.p, e->h_ky.data(), res * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+        e->have_geom = true;
+        upload_cfg_ptrs(e);
+    });
+}
+
+int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const double* params, int n_params) {
+    return run_guarded("mse_set_parameterisation", e, [&]() {
+        REQUIRE(e && params, MSE_ERR_ARG, "NULL argument");
+        REQUIRE(form >= 0 && form <= 2, MSE_ERR_ARG, "form must be 0 (Kirkland), 1 (Peng) or 2 (Lobato)");
+        const int per = i_per_atom * (form == 0 ? 4 : 2);
+        REQUIRE(i_per_atom > 0 && per > 0 && n_params % per == 0, MSE_ERR_ARG,
+                "parameterisation does not contain correct parameters per atom");
+        REQUIRE(form == 0 ? i_per_atom == 3 : i_per_atom == 5, MSE_ERR_UNSUPPORTED,
+                "parameter layout expects 3 Kirkland or 5 Peng/Lobato terms");
+        std::vector<float> f(params, params + n_params);
+        e->params.ensure(n_params);
+        HIPX(hipMemcpyAsync(e->params.p, f.data(), n_params * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+        e->form = form;
+        e->i_count = i_per_atom;
+        e->per_atom = per;
+        e->max_Z = n_params / per;
+    });
+}
+
+int mse_sort_atoms(mse_engine* e, int cfg, const double* x, const double* y, const double* z, const int* Z,
+                   int n, int* block_ids, int* z_ids, int* block_start, int* n_in_range) {
+    return run_guarded("mse_sort_atoms", e, [&]() {
+        REQUIRE(e && x && y && z && Z, MSE_ERR_ARG, "NULL argument");
+        REQUIRE(n >= 0, MSE_ERR_ARG, "negative atom count");
+        require_geom(e);
+        if (e->max_Z > 0)
+            for (int i = 0; i < n; ++i)
+                REQUIRE(Z[i] >= 1 && Z[i] <= e->max_Z, MSE_ERR_ARG,
+                        "atomic number " + std::to_string(Z[i]) + " outside the parameterisation");
+        ConfigData& c = config(e, cfg);
+        const mse_geometry& g = e->g;
+        const int nblk = g.blocks_x * g.blocks_y;
+        const long long nkeys = (long long)g.n_slices * nblk;
+        REQUIRE(nkeys < (1LL << 30), MSE_ERR_UNSUPPORTED, "too many bins");
+        const size_t na = std::max(n, 1);
+        e->tmp_xyz.ensure(3 * na);
+        e->tmp_Z.ensure(na);
+        e->tmp_bid.ensure(na);
+        e->tmp_zid.ensure(na);
+        e->keys.ensure(na);
+        e->keys_sorted.ensure(na);
+        e->idx.ensure(na);
+        e->idx_sorted.ensure(na);
+        c.block_start.ensure(nkeys + 1);
+        c.x.ensure(na);
+        c.y.ensure(na);
+        c.z.ensure(na);
+        c.Z.ensure(na);
+        double* dx = e->tmp_xyz.p;
+        double* dy = dx + na;
+        double* dzp = dy + na;
+        if (n > 0) {
+            HIPX(hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(dy, y, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(dzp, z, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(e->tmp_Z.p, Z, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+            mse::BinGeom bg{g.full_x[0], g.full_x[1], g.full_y[0], g.full_y[1], g.z_lims[0], g.z_lims[1],
+                            g.slice_thickness, g.blocks_x, g.blocks_y, g.n_slices};
+            launch(e, PK_OTHER, [&]() {
+                mse::k_bin_atoms<<<(n + 255) / 256, 256, 0, e->stream>>>(dx, dy, dzp, n, bg, e->keys.p, e->idx.p,
+                                                                         e->tmp_bid.p, e->tmp_zid.p);
+            });
+            int end_bit = 1;
+            while ((1LL << end_bit) <= nkeys) ++end_bit;
+            size_t tmp_bytes = 0;
+            HIPX(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, e->keys.p, e->keys_sorted.p, e->idx.p,
+                                                    e->idx_sorted.p, n, 0, end_bit, e->stream));
+            e->cub_tmp.ensure(tmp_bytes + 1);
+            HIPX(hipcub::DeviceRadixSort::SortPairs(e->cub_tmp.p, tmp_bytes, e->keys.p, e->keys_sorted.p, e->idx.p,
+                                                    e->idx_sorted.p, n, 0, end_bit, e->stream));
+            launch(e, PK_OTHER, [&]() {
+                mse::k_block_starts<<<(int)((nkeys + 1 + 255) / 256), 256, 0, e->stream>>>(e->keys_sorted.p, n,
+                                                                                       (int)nkeys, c.block_start.p);
+            });
+            int nr = 0;
+            HIPX(hipMemcpyAsync(&nr, c.block_start.p + nkeys, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+            HIPX(hipStreamSynchronize(e->stream));
+            c.n_in_range = nr;
+            if (nr > 0) {
+                launch(e, PK_OTHER, [&]() {
+                    mse::k_gather_atoms<<<(nr + 255) / 256, 256, 0, e->stream>>>(e->idx_sorted.p, nr, dx, dy, dzp,
+                                                                               e->tmp_Z.p, c.x.p, c.y.p, c.z.p, c.Z.p);
+                });
+            }
+        } else {
+            std::vector<int> zeros(nkeys + 1, 0);
+            HIPX(hipMemcpyAsync(c.block_start.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice,
+                                e->stream));
+            c.n_in_range = 0;
+        }
+        if (block_ids && n > 0)
+            HIPX(hipMemcpyAsync(block_ids, e->tmp_bid.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        if (z_ids && n > 0)
+            HIPX(hipMemcpyAsync(z_ids, e->tmp_zid.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        if (block_start)
+            HIPX(hipMemcpyAsync(block_start, c.block_start.p, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                                e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+        if (n_in_range) *n_in_range = c.n_in_range;
+        c.sorted = true;
+    });
+}
+
+int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
+    return run_guarded("mse_build_transmission", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        require_geom(e);
+        REQUIRE(e->form >= 0, MSE_ERR_STATE, "mse_set_parameterisation has not been called");
+        ConfigData& c = config(e, cfg);
+        REQUIRE(c.sorted, MSE_ERR_STATE, "atoms of this configuration are not sorted for this geometry");
+        const mse_geometry& g = e->g;
+        REQUIRE(first >= 0 && count >= 0 && first + count <= g.n_slices, MSE_ERR_ARG, "slice range out of bounds");
+        if (count == 0) return;
+        const bool fresh = c.trans.p == nullptr;
+        c.trans.ensure((size_t)g.n_slices * e->N);
+        if (fresh) upload_cfg_ptrs(e);
+        mse::PotArgs a{};
+        a.res = e->res;
+        a.pixelscale = g.real_scale;
+        a.startx = g.sim_x[0];
+        a.starty = g.sim_y[0];
+        a.min_x = g.full_x[0];
+        a.max_x = g.full_x[1];
+        a.min_y = g.full_y[0];
+        a.max_y = g.full_y[1];
+        a.blocks_x = g.blocks_x;
+        a.blocks_y = g.blocks_y;
+        a.load_x = g.load_x;
+        a.load_y = g.load_y;
+        a.load_z = g.load_z;
+        a.n_slices = g.n_slices;
+        const double beam_theta = M_PI_2 - g.beam_tilt_mrad * 0.001;  // ..._projected_d.cl:221
+        a.cos_phi = std::cos(g.beam_azimuth);
+        a.sin_phi = std::sin(g.beam_azimuth);
+        a.sin_2theta = std::sin(2.0 * beam_theta);
+        a.tan_theta = std::tan(beam_theta);
+        a.sigma = g.sigma_eff;
+        a.dz = g.slice_thickness;
+        a.z_min = g.z_lims[0];
+        a.shift_x = (g.k_vec[0] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;  // simulationgeneral.cpp:562
+        a.shift_y = (g.k_vec[1] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;
+        a.integrals = g.full3d_integrals;
+        a.per_atom = e->per_atom;
+        a.i_count = e->i_count;
+        float2* outp = c.trans.p + (size_t)first * e->N;
+        // potential kernels chunk the slice range so one launch stays short
+        const int chunk = g.full3d ? 1 : 16;
+        for (int s0 = 0; s0 < count; s0 += chunk) {
+            const int cs = std::min(chunk, count - s0);
+            dim3 grid(e->res / 16, e->res / 16, cs);
+            float2* o = outp + (size_t)s0 * e->N;
+            launch(e, PK_POT, [&]() {
+                if (g.full3d) {
+                    if (e->form == 0)
+                        mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                    else if (e->form == 1)
+                        mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                    else
+                        mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                } else {
+                    if (e->form == 0)
+                        mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                    else if (e->form == 1)
+                        mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                    else
+                        mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                }
+            });
+        }
+        // band limit: FFT -> keep |k| <= kmax*2/3 -> IFFT (simulationgeneral.cpp:596-602)
+        mse::RowArgs ra = base_row_args(e);
+        ra.scale = 1.0f;
+        row_pass(e, mse::ROW_FFT, outp, outp, count, e->N, ra);
+        mse::ColArgs ca = base_col_args(e);
+        ca.lim = e->band_lim.p;
+        ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
+        col_pass(e, mse::COL_BAND, outp, outp, count, e->N, ca);
+        row_pass(e, mse::ROW_IFFT, outp, outp, count, e->N, ra);
+    });
+}
+
+int mse_read_transmission(mse_engine* e, int cfg, int slice, float* out) {
+    return run_guarded("mse_read_transmission", e, [&]() {
+        REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
+        require_geom(e);
+        ConfigData& c = config(e, cfg);
+        REQUIRE(c.trans.p, MSE_ERR_STATE, "no transmission functions for this configuration");
+        REQUIRE(slice >= 0 && slice < e->g.n_slices, MSE_ERR_ARG, "slice out of range");
+        HIPX(hipMemcpyAsync(out, c.trans.p + (size_t)slice * e->N, e->N * sizeof(float2), hipMemcpyDeviceToHost,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_set_wave_count(mse_engine* e, int n_waves) {
+    return run_guarded("mse_set_wave_count", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        require_geom(e);
+        REQUIRE(n_waves >= 1 && n_waves <= 65535, MSE_ERR_ARG, "wave count out of range");
+        HIPX(hipStreamSynchronize(e->stream));
+        e->waves.ensure((size_t)n_waves * e->N);
+        e->wave_cfg.ensure(n_waves);
+        e->probe_pos.ensure(2 * (size_t)n_waves);
+        if ((int)e->h_wave_cfg.size() != n_waves) {
+            e->h_wave_cfg.assign(n_waves, 0);
+            HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
+                                e->stream));
+        }
+        e->n_waves = n_waves;
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_set_wave_configs(mse_engine* e, const int* configs, int n_waves) {
+    return run_guarded("mse_set_wave_configs", e, [&]() {
+        REQUIRE(e && configs, MSE_ERR_ARG, "NULL argument");
+        REQUIRE(n_waves == e->n_waves, MSE_ERR_ARG, "n_waves must equal the wave count");
+        for (int i = 0; i < n_waves; ++i)
+            REQUIRE(configs[i] >= 0 && configs[i] < (int)e->configs.size() && e->configs[configs[i]].trans.p,
+                    MSE_ERR_STATE, "wave refers to a configuration without transmission functions");
+        e->h_wave_cfg.assign(configs, configs + n_waves);
+        HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_fill_plane(float2* __restrict__ w, int res, size_t N, int n_waves, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * (size_t)n_waves) return;
+    const size_t p = i % N;
+    // S = F_x psi of a constant wave: value * res at kx = 0, zero elsewhere
+    w[i] = make_float2((p % res) == 0 ? v : 0.f, 0.f);
+}
+
+void check_waves(mse_engine* e) {
+    require_geom(e);
+    REQUIRE(e->n_waves > 0 && e->waves.p, MSE_ERR_STATE, "mse_set_wave_count has not been called");
+}
+
+void check_trans(mse_engine* e) {
+    for (int i = 0; i < e->n_waves; ++i) {
+        const int c = e->h_wave_cfg[i];
+        REQUIRE(c >= 0 && c < (int)e->configs.size() && e->configs[c].trans.p, MSE_ERR_STATE,
+                "transmission functions of configuration " + std::to_string(c) + " are not built");
+    }
+}
+
+mse::ProbeArgs probe_args(mse_engine* e, const mse_probe* p) {
+    mse::ProbeArgs a{};
+    a.pos = e->probe_pos.p;
+    a.wavelength = e->g.wavelength;
+    a.cond_ap2 = (p->cond_ap_mrad * 0.001) / e->g.wavelength;
+    a.smooth_r = (p->smooth_mrad * 0.001) / e->g.wavelength;
+    a.C10 = p->C10;
+    a.C30 = p->C30;
+    a.C50 = p->C50;
+    a.C12 = make_double2(p->C12[0], p->C12[1]);
+    a.C21 = make_double2(p->C21[0], p->C21[1]);
+    a.C23 = make_double2(p->C23[0], p->C23[1]);
+    a.C32 = make_double2(p->C32[0], p->C32[1]);
+    a.C34 = make_double2(p->C34[0], p->C34[1]);
+    a.C41 = make_double2(p->C41[0], p->C41[1]);
+    a.C43 = make_double2(p->C43[0], p->C43[1]);
+    a.C45 = make_double2(p->C45[0], p->C45[1]);
+    a.C52 = make_double2(p->C52[0], p->C52[1]);
+    a.C54 = make_double2(p->C54[0], p->C54[1]);
+    a.C56 = make_double2(p->C56[0], p->C56[1]);
+    a.taper = p->edge_taper;
+    return a;
+}
+
+void upload_positions(mse_engine* e, const double* pos_xy, int n) {
+    std::vector<double> rel(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        rel[2 * i] = pos_xy[2 * i] - e->g.sim_x[0];  // simulationcbed.cpp:54-59
+        rel[2 * i + 1] = pos_xy[2 * i + 1] - e->g.sim_y[0];
+    }
+    HIPX(hipMemcpyAsync(e->probe_pos.p, rel.data(), rel.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+}
+
+void probes_into_waves(mse_engine* e, int n, const mse_probe* p) {
+    mse::ColArgs ca = base_col_args(e);
+    ca.probe = probe_args(e, p);
+    col_pass(e, mse::COL_PROBE, e->waves.p, e->waves.p, n, e->N, ca, PK_PROBE);
+}
+
+void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, const mse::ColArgs* det_args) {
+    mse::RowArgs ra = base_row_args(e);
+    mse::ColArgs ca = base_col_args(e);
+    for (int s = first; s < first + count; ++s) {
+        ra.slice = s;
+        row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nw, e->N, ra);
+        if (last_detect && s == first + count - 1)
+            col_pass(e, mse::COL_PROP_DETECT, e->waves.p, e->waves.p, nw, e->N, *det_args);
+        else
+            col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nw, e->N, ca);
+    }
+}
+
+void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_out, int accumulate) {
+    const int nwd = nw * n_det;
+    const int ntiles = e->res / kTC;
+    launch(e, PK_OTHER, [&]() {
+        mse::k_reduce_partials<<<(nwd + 255) / 256, 256, 0, e->stream>>>(e->partials.p, ntiles, nwd, dev_out,
+                                                                          accumulate, sqrt2 ? M_SQRT2 : 1.0);
+    });
+}
+}  // namespace
+
+extern "C" {
+
+int mse_init_plane_wave(mse_engine* e, double value) {
+    return run_guarded("mse_init_plane_wave", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        check_waves(e);
+        const size_t total = e->N * e->n_waves;
+        const float v = (float)(value * e->res);
+        launch(e, PK_OTHER, [&]() {
+            k_fill_plane<<<(unsigned)((total + 255) / 256), 256, 0, e->stream>>>(e->waves.p, e->res, e->N, e->n_waves, v);
+        });
+    });
+}
+
+int mse_init_probes(mse_engine* e, const double* pos_xy, int n, const mse_probe* probe) {
+    return run_guarded("mse_init_probes", e, [&]() {
+        REQUIRE(e && pos_xy && probe, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(n >= 1 && n <= e->n_waves, MSE_ERR_ARG, "probe count must be 1..wave count");
+        upload_positions(e, pos_xy, n);
+        probes_into_waves(e, n, probe);
+    });
+}
+
+int mse_multislice(mse_engine* e, int first, int count) {
+    return run_guarded("mse_multislice", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        check_waves(e);
+        check_trans(e);
+        REQUIRE(first >= 0 && count >= 0 && first + count <= e->g.n_slices, MSE_ERR_ARG, "slice range out of bounds");
+        multislice(e, e->n_waves, first, count, false, nullptr);
+    });
+}
+
+int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out) {
+    return run_guarded("mse_stem_detect", e, [&]() {
+        REQUIRE(e && dets && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        ensure_detectors(e, dets, n_det);
+        const int nw = e->n_waves;
+        e->partials.ensure((size_t)nw * n_det * (e->res / kTC));
+        e->det_out.ensure((size_t)nw * n_det);
+        mse::ColArgs ca = base_col_args(e);
+        ca.scale = (float)(1.0 / e->res);
+        ca.det_scale = 1.0;
+        ca.det_bits = e->det_bits.p;
+        ca.ndet = n_det;
+        ca.partials = e->partials.p;
+        col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
+        reduce_detectors(e, nw, n_det, sqrt2, e->det_out.p, 0);
+        HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)nw * n_det * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_diffraction(mse_engine* e, int wave, double* out) {
+    return run_guarded("mse_diffraction", e, [&]() {
+        REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(wave >= 0 && wave < e->n_waves, MSE_ERR_ARG, "wave index out of range");
+        e->diff_img.ensure(e->N);
+        mse::ColArgs ca = base_col_args(e);
+        ca.scale = (float)(1.0 / e->res);
+        ca.diff = e->diff_img.p;
+        ca.diff_accumulate = 0;
+        const float2* w = e->waves.p + (size_t)wave * e->N;
+        col_pass(e, mse::COL_FFT_DIFF, w, nullptr, 1, e->N, ca);
+        HIPX(hipMemcpyAsync(out, e->diff_img.p, e->N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_exit_wave(mse_engine* e, int wave, double* out) {
+    return run_guarded("mse_exit_wave", e, [&]() {
+        REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(wave >= 0 && wave < e->n_waves, MSE_ERR_ARG, "wave index out of range");
+        e->scratch.ensure(e->N);
+        mse::RowArgs ra = base_row_args(e);
+        ra.scale = (float)(1.0 / e->res);
+        row_pass(e, mse::ROW_IFFT, e->waves.p + (size_t)wave * e->N, e->scratch.p, 1, e->N, ra);
+        std::vector<float2> h(e->N);
+        HIPX(hipMemcpyAsync(h.data(), e->scratch.p, e->N * sizeof(float2), hipMemcpyDeviceToHost, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+        for (size_t i = 0; i < e->N; ++i) {
+            out[2 * i] = h[i].x;
+            out[2 * i + 1] = h[i].y;
+        }
+    });
+}
+
+int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, int batch, const mse_probe* probe,
+                   const mse_detector* dets, int n_det, int sqrt2, int use_graph, double* accum) {
+    return run_guarded("mse_stem_batch", e, [&]() {
+        REQUIRE(e && pos_xy && probe && dets && accum, MSE_ERR_ARG, "NULL argument");
+        require_geom(e);
+        REQUIRE(n_probes >= 1 && batch >= 1 && n_det >= 1, MSE_ERR_ARG, "counts must be positive");
+        (void)use_graph;
+        ConfigData& c = config(e, cfg);
+        REQUIRE(c.trans.p, MSE_ERR_STATE, "transmission functions of this configuration are not built");
+        if (e->n_waves < batch || !e->waves.p) {
+            HIPX(hipStreamSynchronize(e->stream));
+            e->waves.ensure((size_t)batch * e->N);
+            e->wave_cfg.ensure(batch);
+            e->probe_pos.ensure(2 * (size_t)batch);
+            e->n_waves = batch;
+        }
+        std::vector<int> wc(batch, cfg);
+        if (e->h_wave_cfg != wc) {
+            e->h_wave_cfg = wc;
+            HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        }
+        ensure_detectors(e, dets, n_det);
+        e->partials.ensure((size_t)batch * n_det * (e->res / kTC));
+        e->det_out.ensure((size_t)n_probes * n_det);
+        HIPX(hipMemcpyAsync(e->det_out.p, accum, (size_t)n_probes * n_det * sizeof(double), hipMemcpyHostToDevice,
+                            e->stream));
+        mse::ColArgs da = base_col_args(e);
+        da.det_scale = 1.0;  // V = P' F_y A is Psi(k) itself (unitary scales folded into P')
+        da.det_bits = e->det_bits.p;
+        da.ndet = n_det;
+        da.partials = e->partials.p;
+        for (int p0 = 0; p0 < n_probes; p0 += batch) {
+            const int nb = std::min(batch, n_probes - p0);
+            upload_positions(e, pos_xy + 2 * (size_t)p0, nb);
+            probes_into_waves(e, nb, probe);
+            multislice(e, nb, 0, e->g.n_slices, true, &da);
+            reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
+        }
+        HIPX(hipMemcpyAsync(accum, e->det_out.p, (size_t)n_probes * n_det * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_profile_enable(mse_engine* e, int enable) {
+    return run_guarded("mse_profile_enable", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        drain_profile(e);
+        e->prof = enable != 0;
+    });
+}
+
+int mse_profile_read(mse_engine* e, int kernel_id, int* launches, double* total_ms) {
+    return run_guarded("mse_profile_read", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(kernel_id >= 0 && kernel_id < PK_COUNT, MSE_ERR_ARG, "kernel id out of range");
+        drain_profile(e);
+        if (launches) *launches = e->prof_n[kernel_id];
+        if (total_ms) *total_ms = e->prof_ms[kernel_id];
+    });
+}
+
+int mse_profile_reset(mse_engine* e) {
+    return run_guarded("mse_profile_reset", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        drain_profile(e);
+        for (int i = 0; i < PK_COUNT; ++i) {
+            e->prof_ms[i] = 0.0;
+            e->prof_n[i] = 0;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+// Multislice sweep kernels (gfx950, wave64).
+//
+// The wave of every probe lives in HBM in the mixed (k_x, y) domain: S = F_x psi,
+// the unnormalised 1D DFT along x of the real-space wave.  One slice of
+// doMultiSliceStep (microscope/simulationgeneral.cpp:744-764),
+//     psi' = IFFT2( P . FFT2( t . psi ) )     (unitary 2D FFTs),
+// is two sweeps over the wave:
+//   row pass    (one line per row):    A = F_x (t . F_x^-1 S)   (= res F_x(t psi))
+//   column pass (16 columns per group): V = P' . F_y A ; S' = F_y^-1 V
+// with P' = P / res^2 folding both unitary scales, so that V = P . FFT2_unitary(t psi); each sweep reads and writes the
+// wave once (2 x 8N bytes) and the row pass reads t once (8N bytes).  The propagator
+// is separable, P(kx,ky) = px[kx] py[ky] inside the aperture mask, so it is applied
+// from two res-length tables and a per-row mask limit instead of an N-sized array.
+// V = Psi(k) is the propagated k-space wave (clWaveFunctionRecip), so detector
+// sums are taken from it in the last slice's column pass.
+#pragma once
+#include "fft_wave.hpp"
+
+namespace mse {
+
+enum RowMode : int {
+    ROW_IFFT_T_FFT = 0,   // slice step, row half
+    ROW_PLANE_T_FFT = 1,  // first slice of a plane wave: psi = value (no load)
+    ROW_FFT = 2,          // forward x-DFT (transmission band limit, step 1)
+    ROW_IFFT = 3,         // inverse x-DFT (band limit step 3, exit-wave readback)
+};
+
+enum ColMode : int {
+    COL_PROP = 0,         // slice step, column half
+    COL_PROP_DETECT = 1,  // last slice: propagate and reduce detectors, no store
+    COL_BAND = 2,         // transmission band limit (mask <= , scale 1/res^2)
+    COL_PROBE = 3,        // generate the k-space probe and inverse y-DFT
+    COL_FFT_DETECT = 4,   // detectors from a stored wave (F_y S / res)
+    COL_FFT_DIFF = 5,     // |fftshift Psi|^2 of a stored wave into a double image
+    COL_PROP_STORE_DETECT = 6,  // last slice: propagate, reduce detectors and store
+};
+
+constexpr int kMaxDet = 8;
+
+struct RowArgs {
+    const float2* const* cfg_trans;  // per configuration: base of its slices
+    const int* wave_cfg;             // per wave: configuration
+    int slice;
+    float plane_value;               // ROW_PLANE_T_FFT: psi amplitude times res
+    float scale;                     // ROW_FFT / ROW_IFFT output scale
+};
+
+struct ProbeArgs {
+    const double* pos;   // per wave: (x, y) relative to the padded start, Angstrom
+    double wavelength;
+    double cond_ap2, smooth_r;  // k-space radii (1/Angstrom)
+    double C10, C30, C50;
+    double2 C12, C21, C23, C32, C34, C41, C43, C45, C52, C54, C56;
+    int taper;
+};
+
+struct ColArgs {
+    const float2* px;        // propagator factors (COL_PROP*)
+    const float2* py;
+    const int* lim;          // per ky row: largest |kx index| kept (mask), -1 for none
+    const double* kx;        // k-grid (COL_PROBE)
+    const double* ky;
+    float scale;             // COL_BAND / COL_FFT_*: multiply after the forward DFT
+    double det_scale;        // intensity factor applied to |V|^2 in detector sums
+    const unsigned* det_bits;  // ndet bitmasks over the fft-shifted image, res*res bits each
+    int ndet;
+    double* partials;        // [wave][det][tile] partial sums
+    double* diff;            // COL_FFT_DIFF: res*res doubles per wave, fft-shifted
+    int diff_accumulate;
+    ProbeArgs probe;
+};
+
+// --------------------------------------------------------------------------------------
+// Row pass: 256 threads, 256/L lines per block; grid (res / lines, n_waves)
+// --------------------------------------------------------------------------------------
+template <class PL, int MODE>
+__global__ __launch_bounds__(256) void k_row(const float2* __restrict__ in, float2* __restrict__ out,
+                                             size_t wave_stride, const float2* __restrict__ tw,
+                                             RowArgs a) {
+    constexpr int N = PL::N, L = PL::L, E = PL::E;
+    constexpr int LINES = 256 / L;
+    __shared__ float2 lds[LINES][LdsLine<N>::size];
+    const int tid = threadIdx.x;
+    const int line = tid / L;
+    const int t = tid % L;
+    const int row = blockIdx.x * LINES + line;
+    const int wave = blockIdx.y;
+    float2* ln = lds[line];
+
+    PlanTw<PL> tws;
+    tws.load(tw, t);
+    auto sync = []() { __syncthreads(); };
+
+    const size_t base = (size_t)wave * wave_stride + (size_t)row * N;
+    float2 v[E];
+    if constexpr (MODE == ROW_PLANE_T_FFT) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = make_float2(a.plane_value, 0.f);
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = in[base + t + L * m];
+    }
+    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_IFFT) {
+        fft_line<PL, true>(v, ln, t, tws, sync);
+        sync();
+    }
+    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_PLANE_T_FFT) {
+        const float2* tr = a.cfg_trans[a.wave_cfg[wave]] + (size_t)a.slice * N * N + (size_t)row * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[t + L * m]);
+    }
+    if constexpr (MODE != ROW_IFFT) {
+        fft_line<PL, false>(v, ln, t, tws, sync);
+    }
+    if constexpr (MODE == ROW_FFT || MODE == ROW_IFFT) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) out[base + t + L * m] = v[m];
+}
+
+// complex helpers in double for the probe
+__device__ __forceinline__ double2 dmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 dpow(double2 a, int n) {
+    double2 r = a;
+    for (int j = 1; j < n; ++j) r = dmul(r, a);
+    return r;
+}
+
+// init_probe_wave (kernels/init_probe_wave_f.cl:49-118) for one k-space pixel, in double.
+__device__ __forceinline__ float2 probe_value(const ProbeArgs& p, double kx, double ky, double px,
+                                              double py) {
+    const double k = sqrt(kx * kx + ky * ky);
+    if (!(k < p.cond_ap2 + p.smooth_r)) return make_float2(0.f, 0.f);
+    const double lam = p.wavelength;
+    const double posTerm = 2.0 * M_PI * (kx * px + ky * py);
+    const double2 w = make_double2(lam * kx, lam * ky);
+    const double2 wc = make_double2(w.x, -w.y);
+    const double w2 = w.x * w.x + w.y * w.y;
+    double cchi = 0.5 * p.C10 * w2;
+    cchi += 0.5 * dmul(p.C12, dpow(wc, 2)).x;
+    cchi += dmul(p.C21, dmul(dpow(wc, 2), w)).x / 3.0;
+    cchi += dmul(p.C23, dpow(wc, 3)).x / 3.0;
+    cchi += 0.25 * p.C30 * w2 * w2;
+    cchi += 0.25 * dmul(p.C32, dmul(dpow(wc, 3), w)).x;
+    cchi += 0.25 * dmul(p.C34, dpow(wc, 4)).x;
+    cchi += 0.2 * dmul(p.C41, dmul(dpow(wc, 3), dpow(w, 2))).x;
+    cchi += 0.2 * dmul(p.C43, dmul(dpow(wc, 4), w)).x;
+    cchi += 0.2 * dmul(p.C45, dpow(wc, 5)).x;
+    cchi += p.C50 * w2 * w2 * w2 / 6.0;
+    cchi += dmul(p.C52, dmul(dpow(wc, 4), dpow(w, 2))).x / 6.0;
+    cchi += dmul(p.C54, dmul(dpow(wc, 5), w)).x / 6.0;
+    cchi += dmul(p.C56, dpow(wc, 6)).x / 6.0;
+    const double chi = 2.0 * M_PI * cchi / lam;
+    double edge = 1.0;
+    if (p.taper && fabs(k - p.cond_ap2) < p.smooth_r) {
+        const double e0 = p.cond_ap2 - p.smooth_r, e1 = p.cond_ap2 + p.smooth_r;
+        double tt = (k - e0) / (e1 - e0);
+        tt = tt < 0.0 ? 0.0 : (tt > 1.0 ? 1.0 : tt);
+        edge = 1.0 - tt * tt * (3.0 - 2.0 * tt);
+    }
+    double ph = -posTerm - chi;
+    ph = remainder(ph, 2.0 * M_PI);
+    float s, c;
+    sincosf((float)ph, &s, &c);
+    return make_float2((float)(c * edge), (float)(s * edge));
+}
+
+// wave64 double reduction
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// --------------------------------------------------------------------------------------
+// Column pass: TC columns per block, L lanes per column (TC*L threads); grid
+// (res / TC, n_waves).  The TC x res tile is staged through LDS column-major with a
+// column stride of 1 mod 16 complex, so the coalesced 128-byte row loads and the
+// per-column FFT exchanges are both free of bank conflicts; each column's LDS line
+// doubles as its FFT exchange buffer.
+// --------------------------------------------------------------------------------------
+template <class PL, int TC, int MODE>
+__global__ __launch_bounds__(TC * PL::L) void k_col(const float2* __restrict__ in, float2* __restrict__ out,
+                                                    size_t wave_stride, const float2* __restrict__ tw,
+                                                    ColArgs a) {
+    constexpr int N = PL::N, L = PL::L, E = PL::E;
+    constexpr int NT = TC * L;
+    constexpr int CS = LdsLine<N>::col_stride;
+    __shared__ float2 tile[TC * CS];
+    __shared__ double red[NT / 64][kMaxDet];
+    const int tid = threadIdx.x;
+    const int c = tid / L;
+    const int t = tid % L;
+    const int x0 = blockIdx.x * TC;
+    const int x = x0 + c;
+    const int wave = blockIdx.y;
+    float2* ln = tile + c * CS;
+    const size_t wbase = (size_t)wave * wave_stride;
+
+    PlanTw<PL> tws;
+    tws.load(tw, t);
+    auto sync = []() { __syncthreads(); };
+
+    float2 v[E];
+    if constexpr (MODE == COL_PROBE) {
+        const double kxv = a.kx[x];
+        const double ppx = a.probe.pos[2 * wave], ppy = a.probe.pos[2 * wave + 1];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = probe_value(a.probe, kxv, a.ky[t + L * m], ppx, ppy);
+    } else {
+        // coalesced load of the TC x N tile: TC consecutive complex per row
+        for (int idx = tid; idx < TC * N; idx += NT) {
+            const int r = idx / TC, cc = idx % TC;
+            tile[cc * CS + lpad(r)] = in[wbase + (size_t)r * N + x0 + cc];
+        }
+        sync();
+        line_gather<N, L>(v, ln, t);
+        sync();
+        fft_line<PL, false>(v, ln, t, tws, sync);
+    }
+
+    if constexpr (MODE == COL_PROP || MODE == COL_PROP_DETECT || MODE == COL_PROP_STORE_DETECT ||
+                  MODE == COL_BAND) {
+        const int xs = (x < N / 2) ? x : N - x;  // |signed kx index|
+        float2 pxv = make_float2(a.scale, 0.f);
+        if constexpr (MODE != COL_BAND) pxv = a.px[x];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int y = t + L * m;
+            const bool keep = xs <= a.lim[y];
+            float2 f = pxv;
+            if constexpr (MODE != COL_BAND) f = cmul(pxv, a.py[y]);
+            v[m] = keep ? cmul(v[m], f) : make_float2(0.f, 0.f);
+        }
+    } else if constexpr (MODE == COL_FFT_DETECT || MODE == COL_FFT_DIFF) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    }
+
+    if constexpr (MODE == COL_PROP_DETECT || MODE == COL_PROP_STORE_DETECT ||
+                  MODE == COL_FFT_DETECT) {
+        const int xs = (x + N / 2) % N;  // fft-shifted column
+        const int nw = NT / 64;
+        const int wid = tid / 64, lane = tid % 64;
+        for (int d = 0; d < a.ndet; ++d) {
+            const unsigned* bits = a.det_bits + (size_t)d * ((size_t)N * N / 32);
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int ys = (t + L * m + N / 2) % N;
+                const size_t bit = (size_t)ys * N + xs;
+                if ((bits[bit >> 5] >> (bit & 31)) & 1u) {
+                    const float i2 = v[m].x * v[m].x + v[m].y * v[m].y;
+                    acc += (double)i2;
+                }
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) red[wid][d] = acc;
+        }
+        __syncthreads();
+        if (tid < a.ndet) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += red[w][tid];
+            a.partials[((size_t)wave * a.ndet + tid) * gridDim.x + blockIdx.x] = s * a.det_scale;
+        }
+    }
+    if constexpr (MODE == COL_FFT_DIFF) {
+        double* img = a.diff + (size_t)wave * N * N;
+        const int xs = (x + N / 2) % N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int ys = (t + L * m + N / 2) % N;
+            const double i2 = (double)v[m].x * v[m].x + (double)v[m].y * v[m].y;
+            const size_t o = (size_t)ys * N + xs;
+            img[o] = a.diff_accumulate ? img[o] + i2 : i2;
+        }
+    }
+
+    if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
+                  MODE == COL_PROBE) {
+        if constexpr (MODE == COL_PROBE) sync();  // tile unused so far; order with exchanges
+        fft_line<PL, true>(v, ln, t, tws, sync);
+        sync();
+#pragma unroll
+        for (int m = 0; m < E; ++m) ln[lpad(t + L * m)] = v[m];
+        sync();
+        for (int idx = tid; idx < TC * N; idx += NT) {
+            const int r = idx / TC, cc = idx % TC;
+            out[wbase + (size_t)r * N + x0 + cc] = tile[cc * CS + lpad(r)];
+        }
+    }
+}
+
+// sum per-tile partials in fixed order: out[w*ndet + d] (+)= sum_tiles
+__global__ void k_reduce_partials(const double* __restrict__ partials, int ntiles, int nwd,
+                                  double* __restrict__ out, int accumulate, double factor) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nwd) return;
+    double s = 0.0;
+    for (int k = 0; k < ntiles; ++k) s += partials[(size_t)i * ntiles + k];
+    s *= factor;
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+}  // namespace mse

@@ -1,0 +1,69 @@
+"""Named workloads: specimen + scenario fields for the engine (and, identically, the oracle).
+
+Each function returns a dict with
+  scenario: keyword fields of mse_scenario (see cltem_amd.make_scenario)
+  specimen: cltem_amd.specimens.Specimen (undisplaced)
+  params:   parameterisation name
+plus mode-specific entries (probe positions, detectors, phonon settings).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import specimens as sp
+
+SRTIO3_A = 3.905
+PHONON_SEED = 20261202
+
+
+def _limits(spec):
+    lo, hi = spec.limits
+    return dict(struct_min=tuple(lo), struct_max=tuple(hi))
+
+
+def ctem_srtio3(nx=8, ny=8, nz=20, resolution=256, slice_thickness=1.9525, voltage_kv=200.0,
+                params="kirkland"):
+    """Plain CTEM exit wave: SrTiO3 [001] tiled nx*ny*nz, plane wave, no phonons
+    (8x8x20, 256^2, 1.9525 A slices, 200 kV is the CPU-runnable reference case)."""
+    spec = sp.srtio3(nx, ny, nz, SRTIO3_A)
+    lim = _limits(spec)
+    scen = dict(mode=1, resolution=resolution, voltage_kv=voltage_kv,
+                slice_thickness=slice_thickness, slice_offset=0.0,
+                area=(lim["struct_min"][0], lim["struct_max"][0], lim["struct_min"][1],
+                      lim["struct_max"][1]), **lim)
+    return dict(name=f"ctem_srtio3_{nx}x{ny}x{nz}_{resolution}", scenario=scen, specimen=spec,
+                params=params)
+
+
+def stem_srtio3(nx=16, ny=16, nz=40, resolution=512, pixels=64, window_cells=(8, 9),
+                slice_thickness=SRTIO3_A / 2, voltage_kv=200.0, cond_ap_mrad=20.0,
+                detectors=((60.0, 200.0, 0.0, 0.0),), u_rms=0.08, params="kirkland"):
+    """Batched HAADF STEM on SrTiO3 [001] tiled nx*ny*nz.
+
+    The scan window is one unit cell (cells window_cells[0]..[1] in x and y).  With
+    clTEM's xy padding of 8 A each side the field is 3.905 + 16 = 19.905 A, so at 512^2
+    the 2/3 band limit is 215 mrad and the 60-200 mrad annulus lies inside it.  (A
+    scan over the whole 16x16-cell specimen would give a 76.5 A field whose band limit,
+    55.9 mrad, is below the detector's inner angle.)
+    """
+    spec = sp.srtio3(nx, ny, nz, SRTIO3_A)
+    lim = _limits(spec)
+    x0, x1 = window_cells[0] * SRTIO3_A, window_cells[1] * SRTIO3_A
+    scen = dict(mode=2, resolution=resolution, voltage_kv=voltage_kv,
+                slice_thickness=slice_thickness, slice_offset=0.0, area=(x0, x1, x0, x1),
+                stem_pixels=(pixels, pixels), **lim)
+    return dict(name=f"stem_srtio3_{nx}x{ny}x{nz}_{resolution}_{pixels}px", scenario=scen,
+                specimen=spec, params=params, cond_ap_mrad=cond_ap_mrad,
+                detectors=[tuple(d) for d in detectors], u_rms=u_rms, seed=PHONON_SEED)
+
+
+def stem_positions(geom, pixels=None):
+    """Probe positions of SimulationStem::simulate (microscope/simulationstem.cpp:156-187):
+    pixel p -> (start_x + (p % nx) * step_x, start_y + (p // nx) * step_y)."""
+    nx, ny = (geom.stem_pixels[0], geom.stem_pixels[1]) if pixels is None else pixels
+    p = np.arange(nx * ny)
+    xi = p % nx
+    yi = p // nx
+    x = geom.stem_start[0] + xi * geom.stem_scale[0]
+    y = geom.stem_start[1] + yi * geom.stem_scale[1]
+    return np.stack([x, y], axis=1)

@@ -1,0 +1,174 @@
+/*
+ * cltem_amd -- MI355X-native multislice engine behind clTEM's simulation interfaces.
+ *
+ * Plain C ABI: opaque engine handle, int status returns (0 = success), a
+ * thread-local error string, caller-owned host buffers passed as pointer + size,
+ * inputs in double.  One engine owns one HIP stream on one GPU; engines are not
+ * shared between threads (clTEM runs one worker thread per device,
+ * threading/threadpool.cpp:9-18, and each worker owns its own context).
+ *
+ * Every entry point names the reference method or kernel whose contract it
+ * replaces (paths relative to the clTEM source tree).
+ */
+#ifndef CLTEM_AMD_H
+#define CLTEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSE_OK 0
+#define MSE_ERR_ARG -1
+#define MSE_ERR_STATE -2
+#define MSE_ERR_HIP -3
+#define MSE_ERR_UNSUPPORTED -4
+
+typedef struct mse_engine mse_engine;
+
+/* ---- scenario geometry: SimulationManager / SimulationCell / MicroscopeParameters ---- */
+
+/* Inputs held by SimulationManager and SimulationCell (simulationmanager.h,
+ * structure/simulationcell.h, utilities/commonstructs.h:210-401). */
+typedef struct mse_scenario {
+    int mode;                /* 1 CTEM, 2 STEM, 3 CBED (utilities/enums.h:8-14) */
+    int resolution;          /* 256, 384, 512, 768, 1024, 1536, 2048 */
+    double voltage_kv;
+    double slice_thickness;  /* SimulationCell::slice_thickness */
+    double slice_offset;     /* SimulationCell::slice_offset */
+    double pad_xy[2];        /* SimulationCell default_xy_padding, {-8, 8} */
+    double pad_z[2];         /* SimulationCell default_z_padding, {-3, 3} */
+    double struct_min[3];    /* CrystalStructure::limitsX/Y/Z lower */
+    double struct_max[3];    /* CrystalStructure::limitsX/Y/Z upper */
+    double area[4];          /* xStart, xFinish, yStart, yFinish of the mode's area */
+    double area_padding;     /* SimulationArea::padding */
+    int stem_pixels[2];      /* StemArea xPixels, yPixels (STEM only) */
+    double beam_tilt_mrad;   /* MicroscopeParameters::BeamTilt */
+    double beam_azimuth;     /* MicroscopeParameters::BeamAzimuth (rad) */
+    double limit_factor;     /* SimulationManager::max_inverse_factor, 2/3 */
+    int full3d;              /* SimulationManager::use_full_3d */
+    int full3d_integrals;    /* SimulationManager::full_3d_integrals, 20 */
+} mse_scenario;
+
+/* Derived quantities the reference computes on demand from the manager. */
+typedef struct mse_geometry {
+    int resolution;
+    double full_x[2], full_y[2]; /* paddedFullLimitsX/Y (simulationmanager.cpp:446-452) */
+    double sim_x[2], sim_y[2];   /* paddedSimLimitsX/Y(0) (simulationmanager.cpp:434-440) */
+    double z_lims[2];            /* paddedSimLimitsZ (simulationmanager.cpp:442-444) */
+    double pad_z[2];             /* SimulationCell::roundPaddingZ (simulationcell.cpp:43-61) */
+    double real_scale;           /* realScale (simulationmanager.cpp:211-222) */
+    double inverse_scale;        /* inverseScale (simulationmanager.cpp:224-230) */
+    double inverse_scale_angle;  /* inverseScaleAngle, mrad/px (simulationmanager.cpp:241-247) */
+    int n_slices;                /* SimulationCell::sliceCount (simulationcell.cpp:75-83) */
+    int pre_slices;              /* SimulationCell::preSliceCount (simulationcell.cpp:85-87) */
+    int blocks_x, blocks_y;      /* calculateBlocks (simulationmanager.cpp:393-408) */
+    double block_scale_x, block_scale_y;
+    int load_x, load_y, load_z;  /* simulationgeneral.cpp:517-521 */
+    double slice_thickness;
+    double wavelength, wavenumber, sigma, sigma_eff; /* commonstructs.h:269-289, simulationgeneral.cpp:418 */
+    double k_vec[3];             /* MicroscopeParameters::Wavevector */
+    double kmax_band;            /* bandwidthkmax (simulationgeneral.cpp:451-455) */
+    double limit_factor;
+    double beam_tilt_mrad, beam_azimuth;
+    double stem_scale[2];        /* StemArea::getScaleX/Y (commonstructs.cpp:99-107) */
+    double stem_start[2];        /* StemArea raw start */
+    int stem_pixels[2];
+    int full3d, full3d_integrals;
+} mse_geometry;
+
+/* Probe (condenser) parameters: SimulationCbed::initialiseProbeWave
+ * (microscope/simulationcbed.cpp:35-100) -> init_probe_wave (kernels/init_probe_wave_f.cl:49-118).
+ * Complex aberrations are ComplexAberration::getComplex() = polar(mag, ang) as (re, im). */
+typedef struct mse_probe {
+    double C10;
+    double C12[2], C21[2], C23[2];
+    double C30;
+    double C32[2], C34[2];
+    double C41[2], C43[2], C45[2];
+    double C50;
+    double C52[2], C54[2], C56[2];
+    double cond_ap_mrad;     /* CondenserAperture */
+    double smooth_mrad;      /* CondenserApertureSmoothing */
+    int edge_taper;          /* 1: smoothstep edge of init_probe_wave_f; 0: init_probe_wave_d */
+} mse_probe;
+
+/* StemDetector (utilities/commonstructs.h:292-300): radii and centre in mrad. */
+typedef struct mse_detector {
+    double inner_mrad, outer_mrad, xc_mrad, yc_mrad;
+} mse_detector;
+
+/* ---- lifecycle (replaces clContext creation, SimulationGeneral ctor/dtor,
+ *      microscope/simulationgeneral.h:20-34) ---- */
+int mse_create(int device_ordinal, mse_engine** out);
+int mse_destroy(mse_engine* e);
+/* thread-local message for the last failing call on this thread (clError::Throw text,
+ * clwrapper/clerror.h:17-24) */
+const char* mse_last_error(void);
+/* ctx->WaitForQueueFinish() (clwrapper/clcontext.h) */
+int mse_synchronize(mse_engine* e);
+/* device name and compute-unit count, for reports */
+int mse_device_info(mse_engine* e, char* name, int name_len, int* cu_count, double* hbm_gbs);
+
+/* ---- geometry (host only; SimulationManager geometry methods, see mse_geometry) ---- */
+int mse_plan_geometry(const mse_scenario* s, mse_geometry* g);
+
+/* ---- SimulationGeneral::initialiseSimulation (microscope/simulationgeneral.cpp:345-646):
+ *      k-grid, band-limit and propagator tables (propagator kernel, kernels/propagator_f.cl:18-51) ---- */
+int mse_set_geometry(mse_engine* e, const mse_geometry* g);
+/* ClParameterisation upload (simulationgeneral.cpp:375-378); form 0 Kirkland, 1 Peng, 2 Lobato */
+int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const double* params, int n_params);
+
+/* ---- SimulationGeneral::sortAtoms (microscope/simulationgeneral.cpp:147-342) with
+ *      atom_sort_d (kernels/atom_sort_d.cl:28-69), binned in IEEE double on the GPU.
+ *      x/y/z are already-displaced coordinates of one frozen-phonon configuration.
+ *      Optional outputs (may be NULL): block_ids/z_ids per input atom (-1 when the host
+ *      filter drops it), block_start (n_slices*bx*by + 1 entries), n_in_range. ---- */
+int mse_sort_atoms(mse_engine* e, int config, const double* x, const double* y, const double* z,
+                   const int* Z, int n, int* block_ids, int* z_ids, int* block_start, int* n_in_range);
+
+/* ---- transmission functions for slices [first, first+count) of a configuration:
+ *      transmission_potentials_projected / _full_3d (kernels/transmission_potentials_*_f.cl)
+ *      plus the FFT -> band_limit -> IFFT of simulationgeneral.cpp:576-613 ---- */
+int mse_build_transmission(mse_engine* e, int config, int first, int count);
+/* read one transmission function back as interleaved float (2N) */
+int mse_read_transmission(mse_engine* e, int config, int slice, float* out);
+
+/* ---- waves: clWaveFunctionReal/Recip[parallel_pixels] (simulationgeneral.h:85-86) ---- */
+int mse_set_wave_count(mse_engine* e, int n_waves);
+/* wave -> frozen-phonon configuration whose transmission functions it uses (default 0) */
+int mse_set_wave_configs(mse_engine* e, const int* configs, int n_waves);
+/* init_plane_wave (kernels/init_plane_wave_f.cl:12-25; simulationctem.cpp:77-82) */
+int mse_init_plane_wave(mse_engine* e, double value);
+/* SimulationCbed::initialiseProbeWave for waves 0..n-1 at absolute positions pos_xy[2*i]
+ * (Angstrom), made relative to the padded simulation start (simulationcbed.cpp:54-59) */
+int mse_init_probes(mse_engine* e, const double* pos_xy, int n, const mse_probe* probe);
+/* SimulationGeneral::doMultiSliceStep for slices [first, first+count) on every wave
+ * (microscope/simulationgeneral.cpp:682-769), batched over the waves */
+int mse_multislice(mse_engine* e, int first, int count);
+/* SimulationStem::getStemPixel for every wave and detector (microscope/simulationstem.cpp:42-131):
+ * out[wave*n_det + d] = (sqrt2 ? sqrt(2) : 1) * sum over the annulus of |Psi_shifted|^2, FP64 */
+int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out);
+/* getDiffractionImage (simulationgeneral.cpp:772-805): |fftshift Psi(k)|^2, N doubles */
+int mse_diffraction(mse_engine* e, int wave, double* out);
+/* getExitWaveImage (simulationgeneral.cpp:808-829) without cropping: interleaved re/im, 2N doubles */
+int mse_exit_wave(mse_engine* e, int wave, double* out);
+
+/* ---- batched STEM hot path: SimulationStem::simulate (microscope/simulationstem.cpp:143-278)
+ *      for n_probes positions of one configuration, in batches of `batch` waves, with the
+ *      detector reduction fused into the last slice.  accum[p*n_det + d] += intensity
+ *      (FP64, so repeated calls over configurations accumulate like updateImages,
+ *      simulationmanager.cpp:274-350).  use_graph=1 replays a captured hipGraph per batch. ---- */
+int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes, int batch,
+                   const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
+                   int use_graph, double* accum);
+
+/* ---- profiling: per-kernel device time measured with hipEvents on the engine stream ---- */
+int mse_profile_enable(mse_engine* e, int enable);
+/* kernel ids: 0 row pass, 1 column pass, 2 potential, 3 probe init, 4 other */
+int mse_profile_read(mse_engine* e, int kernel_id, int* launches, double* total_ms);
+int mse_profile_reset(mse_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+/*
+ * CPU restatement of clTEM's multislice path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the oracle that the HIP product is checked against.  It is
+ * linked by nothing in the product: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, and only as the checker / CPU baseline.
+ *
+ * Every function restates one piece of the reference in double precision
+ * (the `_d` kernels and the host code that drives them), op for op where the
+ * result is decided by rounding (atom binning, masks), built with
+ * -ffp-contract=off.  Each function cites the reference file:line it follows.
+ *
+ * Parity pins (what checks each function, see tests/test_oracle_*.py):
+ *   orc_wavelength/orc_sigma ........ published closed forms (lambda(200kV), sigma)
+ *   orc_geometry .................... hand-derived SrTiO3 numbers (L, px, slices, blocks)
+ *   orc_sort_atoms .................. brute-force tiny cases, stability, edge clamps
+ *   orc_bessk0/orc_bessk1 ........... scipy.special.k0/k1
+ *   orc_potential_* ................. scipy radial tables, superposition, cutoffs
+ *   orc_fft2 ........................ numpy.fft.fft2(norm="ortho") golden vectors
+ *   orc_propagator / band limit ..... exact masks and exp(-i pi lambda k^2 dz)
+ *   orc_probe ....................... numpy chi(), aperture-disc pixel count
+ *   orc_run_* ....................... empty specimen identities, numpy multislice
+ */
+#ifndef MSE_ORACLE_H
+#define MSE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Inputs that clTEM's SimulationManager / SimulationCell hold
+ * (simulationmanager.h, structure/simulationcell.h, utilities/commonstructs.h). */
+typedef struct OrcScenario {
+    int mode;                 /* 1 CTEM, 2 STEM, 3 CBED (utilities/enums.h:8-14) */
+    int resolution;
+    double voltage_kv;
+    double slice_thickness;
+    double slice_offset;
+    double pad_xy[2];         /* SimulationCell default_xy_padding {-8, 8} */
+    double pad_z[2];          /* SimulationCell default_z_padding {-3, 3} */
+    double struct_min[3];     /* CrystalStructure limits */
+    double struct_max[3];
+    double area[4];           /* xStart, xFinish, yStart, yFinish of the mode's area */
+    double area_padding;      /* SimulationArea::padding */
+    int stem_pixels[2];
+    double beam_tilt_mrad;
+    double beam_azimuth;
+    double limit_factor;      /* max_inverse_factor = 2/3 */
+    int full3d;
+    int full3d_integrals;
+} OrcScenario;
+
+typedef struct OrcGeometry {
+    int resolution;
+    double full_x[2], full_y[2];   /* paddedFullLimitsX/Y */
+    double sim_x[2], sim_y[2];     /* paddedSimLimitsX/Y(0) */
+    double z_lims[2];              /* paddedSimLimitsZ */
+    double pad_z[2];               /* rounded z padding */
+    double real_scale;
+    double inverse_scale;
+    double inverse_scale_angle;
+    int n_slices;
+    int pre_slices;
+    int blocks_x, blocks_y;
+    double block_scale_x, block_scale_y;
+    int load_x, load_y, load_z;
+    double slice_thickness;
+    double wavelength, wavenumber, sigma, sigma_eff;
+    double k_vec[3];
+    double kmax_band;              /* bandwidthkmax */
+    double limit_factor;
+    double beam_tilt_mrad, beam_azimuth;
+    double stem_scale[2];
+    double stem_start[2];
+    int stem_pixels[2];
+    int full3d, full3d_integrals;
+} OrcGeometry;
+
+double orc_wavelength(double kv);
+double orc_sigma(double kv);
+int orc_geometry(const OrcScenario* s, OrcGeometry* g);
+
+/* sortAtoms: host filter, atom_sort_d, stable bucket fill.
+ * out arrays need capacity n; block_start needs n_slices*bx*by+1. */
+int orc_sort_atoms(const OrcGeometry* g, const double* x, const double* y, const double* z,
+                   const int* Z, int n, double* sx, double* sy, double* sz, int* sZ,
+                   int* block_ids, int* z_ids, int* block_start, int* n_filtered, int* n_in_range);
+
+double orc_bessk0(double x);
+double orc_bessk1(double x);
+double orc_vproj(int form, const double* params, int i_count, int Z, double r);
+double orc_v3d(int form, const double* params, int i_count, int Z, double r);
+
+int orc_kgrid(const OrcGeometry* g, double* kx, double* ky);
+
+/* raw potential sum (sumz) for one slice; out has N doubles */
+int orc_potential_slice(const OrcGeometry* g, const double* sx, const double* sy, const double* sz,
+                        const int* sZ, const int* block_start, const double* params, int form,
+                        int i_count, int slice, double* out, int nthreads);
+/* transmission for slices [first, first+count): out is count * 2N doubles (interleaved) */
+int orc_transmission(const OrcGeometry* g, const double* sx, const double* sy, const double* sz,
+                     const int* sZ, const int* block_start, const double* params, int form,
+                     int i_count, int first, int count, int band_limit, double* out, int nthreads);
+
+int orc_fft2(double* data, int w, int h, int inverse);
+int orc_band_limit(const OrcGeometry* g, double* t);
+int orc_propagator(const OrcGeometry* g, double* P);
+/* k-space probe (before the IFFT) for position (px, py) in Angstrom relative to
+ * the padded simulation start; aberr = 25 doubles:
+ *   C10, C12(re,im), C21(re,im), C23(re,im), C30, C32(re,im), C34(re,im),
+ *   C41(re,im), C43(re,im), C45(re,im), C50, C52(re,im), C54(re,im), C56(re,im) */
+int orc_probe_kspace(const OrcGeometry* g, double px, double py, const double* aberr,
+                     double cond_ap_mrad, double smooth_mrad, int taper, double* out);
+
+/* One doMultiSliceStep for one wave: recip = FFT(t psi) P ; psi = IFFT(recip) */
+int orc_slice_step(int res, double* psi, double* recip, const double* t, const double* P);
+double orc_stem_pixel(int res, const double* recip, double inner_mrad, double outer_mrad,
+                      double xc_mrad, double yc_mrad, double angle_scale, int sqrt2);
+int orc_diffraction(int res, const double* recip, double* out);
+
+/* Whole-path drivers.  trans = n_slices * 2N (already band limited). */
+int orc_run_plane(int res, int n_slices, const double* trans, const double* P,
+                  double* psi_out, double* recip_out);
+/* STEM: probes at pos[2*i] (Angstrom, absolute), detectors det[4*d] = inner, outer, xc, yc (mrad),
+ * out[i*ndet + d] */
+int orc_run_stem(const OrcGeometry* g, const double* trans, const double* P, const double* pos,
+                 int n_probes, const double* aberr, double cond_ap_mrad, double smooth_mrad,
+                 int taper, const double* det, int ndet, int sqrt2, double* out, int nthreads);
+/* FP32 restatement of the same loop (the `_f` path), used as the CPU baseline. */
+int orc_run_stem_f32(const OrcGeometry* g, const double* trans, const double* P, const double* pos,
+                     int n_probes, const double* aberr, double cond_ap_mrad, double smooth_mrad,
+                     int taper, const double* det, int ndet, int sqrt2, double* out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+"""GPU parity: the HIP path against the FP64 oracle on the same inputs."""
+import numpy as np
+import pytest
+
+import cltem_amd
+from cltem_amd import specimens as sp
+from cltem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def setup_case(engine, oracle, w, config=0, spec=None):
+    scen = w["scenario"]
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
+    og = oracle.geometry(oracle.scenario(**scen))
+    p = cltem_amd.load_params(w["params"])
+    engine.set_geometry(g)
+    engine.set_parameterisation(p)
+    spec = w["specimen"] if spec is None else spec
+    return g, og, p, spec
+
+
+def test_binning_bit_exact_config1(engine, oracle):
+    w = wl.ctem_srtio3()
+    g, og, p, spec = setup_case(engine, oracle, w)
+    r = engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+    assert r["n_in_range"] == o["n_in_range"]
+    np.testing.assert_array_equal(r["block_start"], o["block_start"])
+    keep = r["block_ids"] >= 0
+    np.testing.assert_array_equal(r["block_ids"][keep], o["block_ids"])
+    np.testing.assert_array_equal(r["z_ids"][keep], o["z_ids"])
+
+
+def test_transmission_config1_slices(engine, oracle):
+    w = wl.ctem_srtio3()
+    g, og, p, spec = setup_case(engine, oracle, w)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+    for s in (0, 2, 3, 21, 40):
+        ref = oracle.transmission(og, o, p.values, p.form, p.i_per_atom, first=s, count=1)[0]
+        got = engine.read_transmission(0, s)
+        err = rel_l2(got, ref)
+        print(f"slice {s}: rel-L2 {err:.3e}")
+        assert err <= 1e-5
+
+
+def test_ctem_exit_wave_config1(engine, oracle):
+    w = wl.ctem_srtio3()
+    g, og, p, spec = setup_case(engine, oracle, w)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    ew = engine.exit_wave(0)
+    dif = engine.diffraction(0)
+    o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    P = oracle.propagator(og)
+    psi, recip = oracle.run_plane(trans, P)
+    e1 = rel_l2(ew, psi)
+    e2 = rel_l2(dif, oracle.diffraction(recip))
+    print(f"exit wave rel-L2 {e1:.3e}, diffraction rel-L2 {e2:.3e}")
+    assert e1 <= 1e-4
+    assert e2 <= 1e-3
+
+
+def test_stem_small_vs_oracle(engine, oracle):
+    w = wl.stem_srtio3(nx=6, ny=6, nz=6, resolution=256, pixels=8, window_cells=(2, 3),
+                       detectors=((60.0, 200.0), (0.0, 20.0), (20.0, 60.0, 3.0, -2.0)))
+    g, og, p, spec = setup_case(engine, oracle, w)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    pos = wl.stem_positions(g)[::7]
+    probe = cltem_amd.make_probe(w["cond_ap_mrad"])
+    dets = cltem_amd.make_detectors(w["detectors"])
+    got = engine.stem_batch(0, pos, 4, probe, dets)
+    o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    P = oracle.propagator(og)
+    det4 = np.array([list(d) + [0.0] * (4 - len(d)) for d in w["detectors"]])
+    ref = oracle.run_stem(og, trans, P, pos, det4, cond_ap_mrad=w["cond_ap_mrad"])
+    err = np.abs(got - ref) / np.abs(ref)
+    print("detector rel err max", err.max(), got[:2], ref[:2])
+    assert err.max() <= 1e-3
+    # split path (init_probes / multislice / stem_detect) agrees with the fused batch
+    engine.set_wave_count(len(pos))
+    engine.set_wave_configs([0] * len(pos))
+    engine.init_probes(pos, probe)
+    engine.multislice()
+    split = engine.stem_detect(dets)
+    assert np.abs(split - got).max() / np.abs(got).max() <= 1e-5
