@@ -176,8 +176,12 @@ def main():
     displaced = [spec.displaced(w["u_rms"], w["seed"], c) for c in range(n_cfg)]
     accum = np.zeros((len(pos), len(dets)))
     first_cfg = None
+    tune = None
     for c in range(args.warmup):
         build_config(engine, 0, displaced[c])
+        if tune is None:
+            tune = engine.autotune(0, args.batch, reps=3)
+            log(f"autotune at B={args.batch}: {tune}")
         engine.stem_batch(0, pos, args.batch, probe, dets, use_graph=bool(args.graph))
         log(f"warmup configuration {c} done")
     engine.synchronize()
@@ -251,6 +255,7 @@ def main():
             "resolution": g.resolution, "probes": len(pos), "slices": ns, "batch": args.batch,
             "phonon_configs_timed": args.steps, "atoms": len(spec), "device": dev,
         },
+        "kernel_tuning": tune,
         "slice_loop_rate_by_batch": sweep,
         "plain_rate": round(plain, 1),
         "batched_over_plain": round(max(sweep.values()) / plain, 2) if plain > 0 else None,

@@ -221,6 +221,28 @@ class Engine:
         self.n_waves = max(self.n_waves, batch)
         return accum
 
+    # -- tuning -------------------------------------------------------------------------
+    def set_kernel_variant(self, kernel: int, variant: int):
+        _check(self.lib.mse_set_kernel_variant(self.h, kernel, variant))
+
+    def autotune(self, config: int = 0, n_waves: int = 64, reps: int = 5):
+        row = np.zeros(8)
+        col = np.zeros(8)
+        br = C.c_int(0)
+        bc = C.c_int(0)
+        _check(self.lib.mse_autotune(self.h, config, n_waves, reps, _dp(row), _dp(col), C.byref(br),
+                                     C.byref(bc)))
+        self.n_waves = max(self.n_waves, n_waves)
+        nr, nc = 5, 8
+        return dict(row_us=row[:nr].tolist(), col_us=col[:nc].tolist(), best_row=br.value,
+                    best_col=bc.value)
+
+    def bandwidth_probe(self, pattern: int, n_waves: int, reps: int = 5, variant: int = 0) -> float:
+        us = C.c_double(0)
+        _check(self.lib.mse_bandwidth_probe(self.h, pattern, variant, n_waves, reps, C.byref(us)))
+        self.n_waves = max(self.n_waves, n_waves)
+        return us.value
+
     # -- profiling ----------------------------------------------------------------------
     def profile(self, enable: bool = True):
         _check(self.lib.mse_profile_enable(self.h, int(enable)))

@@ -130,6 +130,9 @@ SIGNATURES = [
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_stem_batch", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
                                  C.POINTER(MseDetector), C.c_int, C.c_int, C.c_int, _dp]),
+    ("mse_set_kernel_variant", C.c_int, [_eng, C.c_int, C.c_int]),
+    ("mse_autotune", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _ip]),
+    ("mse_bandwidth_probe", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     ("mse_profile_enable", C.c_int, [_eng, C.c_int]),
     ("mse_profile_read", C.c_int, [_eng, C.c_int, _ip, _dp]),
     ("mse_profile_reset", C.c_int, [_eng]),

@@ -85,7 +85,6 @@ struct ConfigData {
 
 enum ProfKind { PK_ROW = 0, PK_COL = 1, PK_POT = 2, PK_PROBE = 3, PK_OTHER = 4, PK_COUNT = 5 };
 
-constexpr int kTC = 16;  // columns per column-pass block
 
 }  // namespace
 
@@ -120,6 +119,9 @@ struct mse_engine {
     DevBuf<int> tmp_Z, tmp_bid, tmp_zid;
     DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
     DevBuf<unsigned char> cub_tmp;
+    int row_var = 0, col_var = 0;  // kernel variants (see row_pass / col_pass)
+    int num_cus = 256;
+    int det_tiles = 0;             // column tiles of the last detector launch
     // profiling
     bool prof = false;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
@@ -198,54 +200,147 @@ void dispatch_res(int res, F&& f) {
 }
 
 // ---- sweeps ------------------------------------------------------------------------------
+// Kernel variants (chosen by mse_autotune or mse_set_kernel_variant):
+//   row    0: default plan, wave-level LDS sync   1: half lanes per line   2: block barriers
+//          3: twiddles read at use, >= 8 waves per SIMD
+//   column 0: default plan, 16 columns per block  1: half lanes per column
+//          2: 8 columns per block                 3: block barriers
+//          4: 16 columns, twiddles read at use, >= 8 waves per SIMD (2 blocks per CU)
+//          5: 8 columns, twiddles read at use, >= 8 waves per SIMD (4 blocks per CU)
+//          6: 32 columns (256-byte row runs), half lanes per column
+//   row    4: persistent lines with the next row prefetched into registers
+//   column 7: persistent blocks with the next tile prefetched into registers
+constexpr int kRowVariants = 5;
+constexpr int kColVariants = 8;
+
 void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets,
-              size_t wave_stride, const mse::RowArgs& args) {
+              size_t wave_stride, const mse::RowArgs& args, int variant = -1) {
+    if (variant < 0) variant = e->row_var;
     dispatch_res(e->res, [&](auto Nc) {
         constexpr int N = decltype(Nc)::value;
-        using PL = typename mse::PlanFor<N>::P;
-        constexpr int LINES = 256 / PL::L;
-        dim3 grid(N / LINES, nlines_sets);
-        launch(e, PK_ROW, [&]() {
-            switch (mode) {
-                case mse::ROW_IFFT_T_FFT:
-                    mse::k_row<PL, mse::ROW_IFFT_T_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                    break;
-                case mse::ROW_PLANE_T_FFT:
-                    mse::k_row<PL, mse::ROW_PLANE_T_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                    break;
-                case mse::ROW_FFT:
-                    mse::k_row<PL, mse::ROW_FFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                    break;
-                case mse::ROW_IFFT:
-                    mse::k_row<PL, mse::ROW_IFFT><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                    break;
-            }
-        });
+        auto go = [&](auto plan, auto sync, auto twreg) {
+            using PL = decltype(plan);
+            using SY = decltype(sync);
+            constexpr bool TR = decltype(twreg)::value;
+            constexpr int WPE = TR ? 1 : 8;
+            constexpr int LINES = 256 / PL::L;
+            dim3 grid(N / LINES, nlines_sets);
+            launch(e, PK_ROW, [&]() {
+                switch (mode) {
+                    case mse::ROW_IFFT_T_FFT:
+                        mse::k_row<PL, mse::ROW_IFFT_T_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                        break;
+                    case mse::ROW_PLANE_T_FFT:
+                        mse::k_row<PL, mse::ROW_PLANE_T_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                        break;
+                    case mse::ROW_FFT:
+                        mse::k_row<PL, mse::ROW_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                        break;
+                    case mse::ROW_IFFT:
+                        mse::k_row<PL, mse::ROW_IFFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                        break;
+                    case mse::ROW_COPY:
+                        mse::k_row<PL, mse::ROW_COPY, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
+                        break;
+                }
+            });
+        };
+        using P0 = typename mse::PlanFor<N>::P;
+        using P1 = typename mse::PlanHalfLanes<N>::P;
+        using REG = std::true_type;
+        using MEM = std::false_type;
+        if (mode == mse::ROW_IFFT_T_FFT && variant == 4 && in == out) {
+            constexpr int LINES = 256 / P0::L;
+            const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+            launch(e, PK_ROW, [&]() {
+                mse::k_row_step_pipe<P0, mse::WaveSync><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
+                                                                                     nlines_sets);
+            });
+            return;
+        }
+        if (mode == mse::ROW_IFFT_T_FFT && variant == 1)
+            go(P1{}, mse::WaveSync{}, REG{});
+        else if (mode == mse::ROW_IFFT_T_FFT && variant == 2)
+            go(P0{}, mse::BlockSync{}, REG{});
+        else if (mode == mse::ROW_IFFT_T_FFT && variant == 3)
+            go(P0{}, mse::WaveSync{}, MEM{});
+        else
+            go(P0{}, mse::WaveSync{}, REG{});
     });
 }
 
-void col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
-              const mse::ColArgs& args, int kind = PK_COL) {
+// returns the number of column tiles of the launch (detector partials per wave)
+int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
+             const mse::ColArgs& args, int kind = PK_COL, int variant = -1) {
+    if (variant < 0) variant = e->col_var;
+    const bool hot = mode == mse::COL_PROP || mode == mse::COL_PROP_DETECT || mode == mse::COL_COPY;
+    if (!hot) variant = 0;
+    int tiles = 0;
     dispatch_res(e->res, [&](auto Nc) {
         constexpr int N = decltype(Nc)::value;
-        using PL = typename mse::PlanFor<N>::P;
-        constexpr int NT = kTC * PL::L;
-        dim3 grid(N / kTC, nsets);
-        launch(e, kind, [&]() {
-            switch (mode) {
+        auto go = [&](auto plan, auto tc, auto sync, auto twreg) {
+            using PL = decltype(plan);
+            using SY = decltype(sync);
+            constexpr bool TR = decltype(twreg)::value;
+            constexpr int WPE = TR ? 1 : 8;
+            constexpr int TC = decltype(tc)::value;
+            constexpr int NT = TC * PL::L;
+            dim3 grid(N / TC, nsets);
+            tiles = N / TC;
+            launch(e, kind, [&]() {
+                switch (mode) {
 #define MSE_COL_CASE(M) \
-    case mse::M: mse::k_col<PL, kTC, mse::M><<<grid, NT, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args); break;
-                MSE_COL_CASE(COL_PROP)
-                MSE_COL_CASE(COL_PROP_DETECT)
-                MSE_COL_CASE(COL_BAND)
-                MSE_COL_CASE(COL_PROBE)
-                MSE_COL_CASE(COL_FFT_DETECT)
-                MSE_COL_CASE(COL_FFT_DIFF)
-                MSE_COL_CASE(COL_PROP_STORE_DETECT)
+    case mse::M: mse::k_col<PL, TC, mse::M, SY, TR, WPE><<<grid, NT, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args); break;
+                    MSE_COL_CASE(COL_PROP)
+                    MSE_COL_CASE(COL_PROP_DETECT)
+                    MSE_COL_CASE(COL_BAND)
+                    MSE_COL_CASE(COL_PROBE)
+                    MSE_COL_CASE(COL_FFT_DETECT)
+                    MSE_COL_CASE(COL_FFT_DIFF)
+                    MSE_COL_CASE(COL_PROP_STORE_DETECT)
+                    MSE_COL_CASE(COL_COPY)
 #undef MSE_COL_CASE
-            }
-        });
+                }
+            });
+        };
+        using P0 = typename mse::PlanFor<N>::P;
+        using P1 = typename mse::PlanHalfLanes<N>::P;
+        using C16 = std::integral_constant<int, 16>;
+        using C8 = std::integral_constant<int, 8>;
+        using C32 = std::integral_constant<int, 32>;
+        using REG = std::true_type;
+        using MEM = std::false_type;
+        if (variant == 7 && mode == mse::COL_PROP && in == out) {
+            constexpr int NT = 16 * P0::L;
+            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
+            tiles = N / 16;
+            launch(e, kind, [&]() {
+                mse::k_col_prop_pipe<P0, 16, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
+                                                                                          nsets);
+            });
+            return;
+        }
+        if (variant == 7) variant = 0;
+        if (variant == 1)
+            go(P1{}, C16{}, mse::WaveSync{}, REG{});
+        else if (variant == 2)
+            go(P0{}, C8{}, mse::WaveSync{}, REG{});
+        else if (variant == 3)
+            go(P0{}, C16{}, mse::BlockSync{}, REG{});
+        else if (variant == 4)
+            go(P0{}, C16{}, mse::WaveSync{}, MEM{});
+        else if (variant == 5)
+            go(P0{}, C8{}, mse::WaveSync{}, MEM{});
+        else if (variant == 6) {
+            if constexpr (N <= 512)  // 32 columns exceed the 160 KiB LDS above 512 points
+                go(P1{}, C32{}, mse::WaveSync{}, REG{});
+            else
+                go(P0{}, C16{}, mse::WaveSync{}, REG{});
+        }
+        else
+            go(P0{}, C16{}, mse::WaveSync{}, REG{});
     });
+    return tiles;
 }
 
 mse::ColArgs base_col_args(mse_engine* e) {
@@ -359,6 +454,11 @@ int mse_create(int device_ordinal, mse_engine** out) {
         DeviceGuard dg(device_ordinal);
         auto* e = new mse_engine();
         e->device = device_ordinal;
+        {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess && prop.multiProcessorCount > 0)
+                e->num_cus = prop.multiProcessorCount;
+        }
         hipError_t st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
         if (st != hipSuccess) {
             delete e;
@@ -793,7 +893,7 @@ void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, c
         ra.slice = s;
         row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nw, e->N, ra);
         if (last_detect && s == first + count - 1)
-            col_pass(e, mse::COL_PROP_DETECT, e->waves.p, e->waves.p, nw, e->N, *det_args);
+            e->det_tiles = col_pass(e, mse::COL_PROP_DETECT, e->waves.p, e->waves.p, nw, e->N, *det_args);
         else
             col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nw, e->N, ca);
     }
@@ -801,7 +901,7 @@ void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, c
 
 void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_out, int accumulate) {
     const int nwd = nw * n_det;
-    const int ntiles = e->res / kTC;
+    const int ntiles = e->det_tiles;
     launch(e, PK_OTHER, [&]() {
         mse::k_reduce_partials<<<(nwd + 255) / 256, 256, 0, e->stream>>>(e->partials.p, ntiles, nwd, dev_out,
                                                                           accumulate, sqrt2 ? M_SQRT2 : 1.0);
@@ -849,7 +949,7 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
         check_waves(e);
         ensure_detectors(e, dets, n_det);
         const int nw = e->n_waves;
-        e->partials.ensure((size_t)nw * n_det * (e->res / kTC));
+        e->partials.ensure((size_t)nw * n_det * (e->res / 8));
         e->det_out.ensure((size_t)nw * n_det);
         mse::ColArgs ca = base_col_args(e);
         ca.scale = (float)(1.0 / e->res);
@@ -857,7 +957,7 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
         ca.det_bits = e->det_bits.p;
         ca.ndet = n_det;
         ca.partials = e->partials.p;
-        col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
+        e->det_tiles = col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
         reduce_detectors(e, nw, n_det, sqrt2, e->det_out.p, 0);
         HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)nw * n_det * sizeof(double), hipMemcpyDeviceToHost, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
@@ -922,7 +1022,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
         }
         ensure_detectors(e, dets, n_det);
-        e->partials.ensure((size_t)batch * n_det * (e->res / kTC));
+        e->partials.ensure((size_t)batch * n_det * (e->res / 8));
         e->det_out.ensure((size_t)n_probes * n_det);
         HIPX(hipMemcpyAsync(e->det_out.p, accum, (size_t)n_probes * n_det * sizeof(double), hipMemcpyHostToDevice,
                             e->stream));
@@ -941,6 +1041,129 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         HIPX(hipMemcpyAsync(accum, e->det_out.p, (size_t)n_probes * n_det * sizeof(double), hipMemcpyDeviceToHost,
                             e->stream));
         HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_set_kernel_variant(mse_engine* e, int kernel, int variant) {
+    return run_guarded("mse_set_kernel_variant", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(kernel == 0 || kernel == 1, MSE_ERR_ARG, "kernel must be 0 (row) or 1 (column)");
+        REQUIRE(variant >= 0 && variant < (kernel == 0 ? kRowVariants : kColVariants), MSE_ERR_ARG,
+                "variant out of range");
+        if (kernel == 0)
+            e->row_var = variant;
+        else
+            e->col_var = variant;
+    });
+}
+
+int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, double* col_us,
+                 int* best_row, int* best_col) {
+    return run_guarded("mse_autotune", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        require_geom(e);
+        REQUIRE(n_waves >= 1 && reps >= 1, MSE_ERR_ARG, "n_waves and reps must be positive");
+        ConfigData& c = config(e, cfg);
+        REQUIRE(c.trans.p, MSE_ERR_STATE, "transmission functions of this configuration are not built");
+        if (e->n_waves < n_waves || !e->waves.p) {
+            HIPX(hipStreamSynchronize(e->stream));
+            e->waves.ensure((size_t)n_waves * e->N);
+            e->wave_cfg.ensure(n_waves);
+            e->probe_pos.ensure(2 * (size_t)n_waves);
+            e->n_waves = n_waves;
+        }
+        e->h_wave_cfg.assign(e->n_waves, cfg);
+        HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), e->n_waves * sizeof(int), hipMemcpyHostToDevice,
+                            e->stream));
+        const size_t total = e->N * n_waves;
+        k_fill_plane<<<(unsigned)((total + 255) / 256), 256, 0, e->stream>>>(e->waves.p, e->res, e->N, n_waves, 1.0f);
+        HIPX(hipGetLastError());
+        hipEvent_t a, b;
+        HIPX(hipEventCreate(&a));
+        HIPX(hipEventCreate(&b));
+        const bool prof = e->prof;
+        e->prof = false;
+        mse::RowArgs ra = base_row_args(e);
+        mse::ColArgs ca = base_col_args(e);
+        double best = 1e30;
+        int bi = 0;
+        for (int v = 0; v < kRowVariants; ++v) {
+            row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, n_waves, e->N, ra, v);
+            HIPX(hipEventRecord(a, e->stream));
+            for (int r = 0; r < reps; ++r) row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, n_waves, e->N, ra, v);
+            HIPX(hipEventRecord(b, e->stream));
+            HIPX(hipEventSynchronize(b));
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, a, b));
+            const double us = 1e3 * ms / reps;
+            if (row_us) row_us[v] = us;
+            if (us < best) { best = us; bi = v; }
+        }
+        e->row_var = bi;
+        best = 1e30;
+        bi = 0;
+        for (int v = 0; v < kColVariants; ++v) {
+            col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, n_waves, e->N, ca, PK_COL, v);
+            HIPX(hipEventRecord(a, e->stream));
+            for (int r = 0; r < reps; ++r) col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, n_waves, e->N, ca, PK_COL, v);
+            HIPX(hipEventRecord(b, e->stream));
+            HIPX(hipEventSynchronize(b));
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, a, b));
+            const double us = 1e3 * ms / reps;
+            if (col_us) col_us[v] = us;
+            if (us < best) { best = us; bi = v; }
+        }
+        e->col_var = bi;
+        e->prof = prof;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (best_row) *best_row = e->row_var;
+        if (best_col) *best_col = e->col_var;
+    });
+}
+
+int mse_bandwidth_probe(mse_engine* e, int pattern, int variant, int n_waves, int reps, double* us_per_launch) {
+    return run_guarded("mse_bandwidth_probe", e, [&]() {
+        REQUIRE(e && us_per_launch, MSE_ERR_ARG, "NULL argument");
+        require_geom(e);
+        REQUIRE(pattern == 0 || pattern == 1, MSE_ERR_ARG, "pattern must be 0 (rows) or 1 (column tiles)");
+        REQUIRE(n_waves >= 1 && reps >= 1, MSE_ERR_ARG, "n_waves and reps must be positive");
+        REQUIRE(variant >= 0 && variant < kColVariants, MSE_ERR_ARG, "variant out of range");
+        if (e->n_waves < n_waves || !e->waves.p) {
+            HIPX(hipStreamSynchronize(e->stream));
+            e->waves.ensure((size_t)n_waves * e->N);
+            e->wave_cfg.ensure(n_waves);
+            e->probe_pos.ensure(2 * (size_t)n_waves);
+            e->h_wave_cfg.assign(n_waves, 0);
+            HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
+                                e->stream));
+            e->n_waves = n_waves;
+        }
+        hipEvent_t a, b;
+        HIPX(hipEventCreate(&a));
+        HIPX(hipEventCreate(&b));
+        const bool prof = e->prof;
+        e->prof = false;
+        mse::RowArgs ra = base_row_args(e);
+        mse::ColArgs ca = base_col_args(e);
+        auto one = [&]() {
+            if (pattern == 0)
+                row_pass(e, mse::ROW_COPY, e->waves.p, e->waves.p, n_waves, e->N, ra, 0);
+            else
+                col_pass(e, mse::COL_COPY, e->waves.p, e->waves.p, n_waves, e->N, ca, PK_COL, variant);
+        };
+        one();
+        HIPX(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; ++r) one();
+        HIPX(hipEventRecord(b, e->stream));
+        HIPX(hipEventSynchronize(b));
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, a, b));
+        *us_per_launch = 1e3 * ms / reps;
+        e->prof = prof;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
     });
 }
 

@@ -16,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace mse {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -195,8 +197,27 @@ template <> struct PlanFor<384> { using P = Plan3<384, 16, 8, 8, 6>; };
 template <> struct PlanFor<512> { using P = Plan3<512, 64, 8, 8, 8>; };
 template <> struct PlanFor<768> { using P = Plan3<768, 32, 8, 8, 12>; };
 template <> struct PlanFor<1024> { using P = Plan3<1024, 64, 16, 8, 8>; };
-template <> struct PlanFor<1536> { using P = Plan3<1536, 32, 16, 8, 12>; };
-template <> struct PlanFor<2048> { using P = Plan3<2048, 64, 16, 16, 8>; };
+
+// Alternative plans with half the lanes per line (twice the elements per lane): more
+// independent work per lane and fewer threads per column tile.
+template <int N>
+struct PlanHalfLanes { using P = typename PlanFor<N>::P; };
+template <> struct PlanHalfLanes<256> { using P = Plan3<256, 16, 8, 8, 4>; };
+template <> struct PlanHalfLanes<512> { using P = Plan3<512, 32, 8, 8, 8>; };
+template <> struct PlanHalfLanes<1024> { using P = Plan3<1024, 32, 16, 8, 8>; };
+
+// Orders LDS traffic among the lanes of one wavefront: waits for this wave's LDS
+// operations and stops the compiler moving memory accesses across the point.  Valid
+// when every lane sharing an LDS line belongs to the same wavefront (L <= 64).
+struct WaveSync {
+    __device__ __forceinline__ void operator()() const {
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+struct BlockSync {
+    __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
 
 // padded LDS index: one extra complex every 16 entries
 __device__ __forceinline__ int lpad(int q) { return q + (q >> 4); }
@@ -224,12 +245,28 @@ struct PassTw {
             for (int r = 1; r < R; ++r) w[s][r - 1] = tw[r * k * step];
         }
     }
+    // twiddle W_{pR}^{r k} of butterfly b
+    __device__ __forceinline__ float2 get(int b, int r, int) const { return w[shared ? 0 : b][r - 1]; }
 };
 
-template <class PL>
+// Same twiddles read from the (cache-resident) table at the point of use: fewer
+// registers held across the transform, one extra load per twiddle.
+template <int N, int L, int R, int P_>
+struct PassTwMem {
+    static constexpr int step = N / (P_ * R);
+    const float2* __restrict__ tw;
+    __device__ __forceinline__ void load(const float2* __restrict__ table, int) { tw = table; }
+    __device__ __forceinline__ float2 get(int, int r, int k) const { return tw[r * k * step]; }
+};
+
+template <class PL, bool IN_REGS = true>
 struct PlanTw {
-    PassTw<PL::N, PL::L, PL::R1, PL::R0> p1;
-    PassTw<PL::N, PL::L, PL::R2, PL::R0 * PL::R1> p2;
+    using T1 = typename std::conditional<IN_REGS, PassTw<PL::N, PL::L, PL::R1, PL::R0>,
+                                         PassTwMem<PL::N, PL::L, PL::R1, PL::R0>>::type;
+    using T2 = typename std::conditional<IN_REGS, PassTw<PL::N, PL::L, PL::R2, PL::R0 * PL::R1>,
+                                         PassTwMem<PL::N, PL::L, PL::R2, PL::R0 * PL::R1>>::type;
+    T1 p1;
+    T2 p2;
     __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
         p1.load(tw, t);
         p2.load(tw, t);
@@ -251,10 +288,9 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[N / L], float2* line, 
         const int i = t + L * b;
         const int k = (P_ == 1) ? 0 : (i % P_);
         if constexpr (P_ > 1) {
-            constexpr int s = TW::shared ? 0 : 1;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const float2 w = tw->w[s * b][r - 1];
+                const float2 w = tw->get(b, r, k);
                 a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
             }
         }
@@ -278,15 +314,14 @@ __device__ __forceinline__ void line_gather(float2 (&v)[N / L], const float2* li
 }
 
 struct NoTw {
-    static constexpr bool shared = true;
-    float2 w[1][1];
+    __device__ __forceinline__ float2 get(int, int, int) const { return make_float2(1.f, 0.f); }
 };
 
 // Full transform of one line in register layout.  sync() must order LDS writes before
 // reads and reads before the next pass's writes for every lane sharing the LDS line.
-template <class PL, bool INV, class Sync>
-__device__ __forceinline__ void fft_line(float2 (&v)[PL::E], float2* line, int t,
-                                         const PlanTw<PL>& tw, Sync sync) {
+template <class PL, bool INV, class TWS, class Sync>
+__device__ __forceinline__ void fft_line(float2 (&v)[PL::E], float2* line, int t, const TWS& tw,
+                                         Sync sync) {
     constexpr int N = PL::N, L = PL::L;
     stockham_pass<N, L, PL::R0, 1, INV, false, NoTw>(v, line, t, nullptr);
     sync();

@@ -23,6 +23,7 @@ enum RowMode : int {
     ROW_PLANE_T_FFT = 1,  // first slice of a plane wave: psi = value (no load)
     ROW_FFT = 2,          // forward x-DFT (transmission band limit, step 1)
     ROW_IFFT = 3,         // inverse x-DFT (band limit step 3, exit-wave readback)
+    ROW_COPY = 4,         // read and write the rows only (bandwidth reference)
 };
 
 enum ColMode : int {
@@ -33,6 +34,7 @@ enum ColMode : int {
     COL_FFT_DETECT = 4,   // detectors from a stored wave (F_y S / res)
     COL_FFT_DIFF = 5,     // |fftshift Psi|^2 of a stored wave into a double image
     COL_PROP_STORE_DETECT = 6,  // last slice: propagate, reduce detectors and store
+    COL_COPY = 7,         // move the tile through LDS only (bandwidth reference)
 };
 
 constexpr int kMaxDet = 8;
@@ -73,10 +75,10 @@ struct ColArgs {
 // --------------------------------------------------------------------------------------
 // Row pass: 256 threads, 256/L lines per block; grid (res / lines, n_waves)
 // --------------------------------------------------------------------------------------
-template <class PL, int MODE>
-__global__ __launch_bounds__(256) void k_row(const float2* __restrict__ in, float2* __restrict__ out,
-                                             size_t wave_stride, const float2* __restrict__ tw,
-                                             RowArgs a) {
+template <class PL, int MODE, class SYNC = WaveSync, bool TWREG = true, int WPE = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_row(
+    const float2* __restrict__ in, float2* __restrict__ out, size_t wave_stride,
+    const float2* __restrict__ tw, RowArgs a) {
     constexpr int N = PL::N, L = PL::L, E = PL::E;
     constexpr int LINES = 256 / L;
     __shared__ float2 lds[LINES][LdsLine<N>::size];
@@ -87,9 +89,9 @@ __global__ __launch_bounds__(256) void k_row(const float2* __restrict__ in, floa
     const int wave = blockIdx.y;
     float2* ln = lds[line];
 
-    PlanTw<PL> tws;
+    PlanTw<PL, TWREG> tws;
     tws.load(tw, t);
-    auto sync = []() { __syncthreads(); };
+    const SYNC sync{};
 
     const size_t base = (size_t)wave * wave_stride + (size_t)row * N;
     float2 v[E];
@@ -99,6 +101,11 @@ __global__ __launch_bounds__(256) void k_row(const float2* __restrict__ in, floa
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = in[base + t + L * m];
+    }
+    if constexpr (MODE == ROW_COPY) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) out[base + t + L * m] = v[m];
+        return;
     }
     if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_IFFT) {
         fft_line<PL, true>(v, ln, t, tws, sync);
@@ -183,15 +190,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // per-column FFT exchanges are both free of bank conflicts; each column's LDS line
 // doubles as its FFT exchange buffer.
 // --------------------------------------------------------------------------------------
-template <class PL, int TC, int MODE>
-__global__ __launch_bounds__(TC * PL::L) void k_col(const float2* __restrict__ in, float2* __restrict__ out,
-                                                    size_t wave_stride, const float2* __restrict__ tw,
-                                                    ColArgs a) {
+template <class PL, int TC, int MODE, class SYNC = WaveSync, bool TWREG = true, int WPE = 1>
+__global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE))) void k_col(
+    const float2* __restrict__ in, float2* __restrict__ out, size_t wave_stride,
+    const float2* __restrict__ tw, ColArgs a) {
     constexpr int N = PL::N, L = PL::L, E = PL::E;
     constexpr int NT = TC * L;
     constexpr int CS = LdsLine<N>::col_stride;
+    static_assert(TC % 2 == 0, "tiles are moved two complex at a time");
     __shared__ float2 tile[TC * CS];
-    __shared__ double red[NT / 64][kMaxDet];
+    __shared__ double red[(NT + 63) / 64][kMaxDet];
     const int tid = threadIdx.x;
     const int c = tid / L;
     const int t = tid % L;
@@ -201,9 +209,9 @@ __global__ __launch_bounds__(TC * PL::L) void k_col(const float2* __restrict__ i
     float2* ln = tile + c * CS;
     const size_t wbase = (size_t)wave * wave_stride;
 
-    PlanTw<PL> tws;
+    PlanTw<PL, TWREG> tws;
     tws.load(tw, t);
-    auto sync = []() { __syncthreads(); };
+    const SYNC sync{};
 
     float2 v[E];
     if constexpr (MODE == COL_PROBE) {
@@ -212,12 +220,26 @@ __global__ __launch_bounds__(TC * PL::L) void k_col(const float2* __restrict__ i
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = probe_value(a.probe, kxv, a.ky[t + L * m], ppx, ppy);
     } else {
-        // coalesced load of the TC x N tile: TC consecutive complex per row
-        for (int idx = tid; idx < TC * N; idx += NT) {
-            const int r = idx / TC, cc = idx % TC;
-            tile[cc * CS + lpad(r)] = in[wbase + (size_t)r * N + x0 + cc];
+        // coalesced load of the TC x N tile: TC consecutive complex per row, 16 bytes a lane
+        const float4* in4 = reinterpret_cast<const float4*>(in + wbase + x0);
+#pragma unroll 4
+        for (int idx = tid; idx < TC * N / 2; idx += NT) {
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            const float4 q = in4[((size_t)r * N) / 2 + cc / 2];
+            tile[cc * CS + lpad(r)] = make_float2(q.x, q.y);
+            tile[(cc + 1) * CS + lpad(r)] = make_float2(q.z, q.w);
         }
-        sync();
+        __syncthreads();
+        if constexpr (MODE == COL_COPY) {
+            float4* out4 = reinterpret_cast<float4*>(out + wbase + x0);
+#pragma unroll 4
+            for (int idx = tid; idx < TC * N / 2; idx += NT) {
+                const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+                const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
+                out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
+            }
+            return;
+        }
         line_gather<N, L>(v, ln, t);
         sync();
         fft_line<PL, false>(v, ln, t, tws, sync);
@@ -282,16 +304,162 @@ __global__ __launch_bounds__(TC * PL::L) void k_col(const float2* __restrict__ i
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
                   MODE == COL_PROBE) {
-        if constexpr (MODE == COL_PROBE) sync();  // tile unused so far; order with exchanges
+        if constexpr (MODE != COL_PROBE) sync();  // the forward pass's last reads precede new writes
         fft_line<PL, true>(v, ln, t, tws, sync);
         sync();
 #pragma unroll
         for (int m = 0; m < E; ++m) ln[lpad(t + L * m)] = v[m];
-        sync();
-        for (int idx = tid; idx < TC * N; idx += NT) {
-            const int r = idx / TC, cc = idx % TC;
-            out[wbase + (size_t)r * N + x0 + cc] = tile[cc * CS + lpad(r)];
+        __syncthreads();
+        float4* out4 = reinterpret_cast<float4*>(out + wbase + x0);
+#pragma unroll 4
+        for (int idx = tid; idx < TC * N / 2; idx += NT) {
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
+            out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
         }
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// Persistent, software-pipelined column propagation (COL_PROP).  One block per CU walks
+// the (wave, column tile) items; while the columns of item i are transformed, the global
+// loads of item i+1 are already in flight into registers, and the stores of item i drain
+// during the next item's transforms.  Same arithmetic as k_col<..., COL_PROP>.
+// --------------------------------------------------------------------------------------
+template <class PL, int TC, class SYNC = WaveSync>
+__global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict__ data, size_t wave_stride,
+                                                              const float2* __restrict__ tw, ColArgs a,
+                                                              int nsets) {
+    constexpr int N = PL::N, L = PL::L, E = PL::E;
+    constexpr int NT = TC * L;
+    constexpr int CS = LdsLine<N>::col_stride;
+    constexpr int NQ = (TC * N / 2) / NT;  // float4 per thread per tile
+    static_assert((TC * N / 2) % NT == 0, "tile must split evenly over the block");
+    constexpr int ntiles = N / TC;
+    __shared__ float2 tile[TC * CS];
+    const int tid = threadIdx.x;
+    const int c = tid / L;
+    const int t = tid % L;
+    const int nitems = nsets * ntiles;
+
+    PlanTw<PL, true> tws;
+    tws.load(tw, t);
+    const SYNC sync{};
+
+    float4 q[NQ];
+    auto fetch = [&](int item) {
+        const int wv = item / ntiles;
+        const int xx = (item % ntiles) * TC;
+        const float4* in4 = reinterpret_cast<const float4*>(data + (size_t)wv * wave_stride + xx);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int idx = tid + k * NT;
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            q[k] = in4[((size_t)r * N) / 2 + cc / 2];
+        }
+    };
+    int item = blockIdx.x;
+    if (item < nitems) fetch(item);
+    for (; item < nitems; item += gridDim.x) {
+        const int wv = item / ntiles;
+        const int x0 = (item % ntiles) * TC;
+        const int x = x0 + c;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int idx = tid + k * NT;
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            tile[cc * CS + lpad(r)] = make_float2(q[k].x, q[k].y);
+            tile[(cc + 1) * CS + lpad(r)] = make_float2(q[k].z, q[k].w);
+        }
+        __syncthreads();
+        if (item + (int)gridDim.x < nitems) fetch(item + gridDim.x);
+        float2* ln = tile + c * CS;
+        float2 v[E];
+        line_gather<N, L>(v, ln, t);
+        sync();
+        fft_line<PL, false>(v, ln, t, tws, sync);
+        {
+            const int xs = (x < N / 2) ? x : N - x;
+            const float2 pxv = a.px[x];
+            const int lim = a.lim[xs];  // the aperture mask is symmetric in kx and ky
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int y = t + L * m;
+                const int ys = (y < N / 2) ? y : N - y;
+                v[m] = ys <= lim ? cmul(v[m], cmul(pxv, a.py[y])) : make_float2(0.f, 0.f);
+            }
+        }
+        sync();
+        fft_line<PL, true>(v, ln, t, tws, sync);
+        sync();
+#pragma unroll
+        for (int m = 0; m < E; ++m) ln[lpad(t + L * m)] = v[m];
+        __syncthreads();
+        float4* out4 = reinterpret_cast<float4*>(data + (size_t)wv * wave_stride + x0);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int idx = tid + k * NT;
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
+            out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// Persistent, software-pipelined row slice step (ROW_IFFT_T_FFT).  Every line walks the
+// (wave, row) items independently; the next item's wave row and transmission row are
+// loaded into registers while the current row is transformed.
+// --------------------------------------------------------------------------------------
+template <class PL, class SYNC = WaveSync>
+__global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data, size_t wave_stride,
+                                                       const float2* __restrict__ tw, RowArgs a, int nsets) {
+    constexpr int N = PL::N, L = PL::L, E = PL::E;
+    constexpr int LINES = 256 / L;
+    __shared__ float2 lds[LINES][LdsLine<N>::size];
+    const int tid = threadIdx.x;
+    const int line = tid / L;
+    const int t = tid % L;
+    float2* ln = lds[line];
+    const int nitems = nsets * N;
+    const int stride = gridDim.x * LINES;
+
+    PlanTw<PL, true> tws;
+    tws.load(tw, t);
+    const SYNC sync{};
+
+    float2 qv[E], qt[E];
+    auto fetch = [&](int item) {
+        const int wv = item / N, row = item % N;
+        const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
+        const float2* tr = a.cfg_trans[a.wave_cfg[wv]] + (size_t)a.slice * N * N + (size_t)row * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            qv[m] = src[t + L * m];
+            qt[m] = tr[t + L * m];
+        }
+    };
+    int item = blockIdx.x * LINES + line;
+    if (item < nitems) fetch(item);
+    for (; item < nitems; item += stride) {
+        float2 v[E], tr[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            v[m] = qv[m];
+            tr[m] = qt[m];
+        }
+        if (item + stride < nitems) fetch(item + stride);
+        fft_line<PL, true>(v, ln, t, tws, sync);
+        sync();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[m]);
+        fft_line<PL, false>(v, ln, t, tws, sync);
+        sync();
+        const int wv = item / N, row = item % N;
+        float2* dst = data + (size_t)wv * wave_stride + (size_t)row * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) dst[t + L * m] = v[m];
     }
 }
 

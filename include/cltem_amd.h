@@ -162,6 +162,21 @@ int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes
                    const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
                    int use_graph, double* accum);
 
+/* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..4),
+ *      kernel 1 = column pass (variants 0..7); see DESIGN.md for what each variant changes.
+ *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
+ *      (reps launches each; times in microseconds per launch), selects the fastest and
+ *      leaves the waves' contents undefined. ---- */
+int mse_set_kernel_variant(mse_engine* e, int kernel, int variant);
+int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_us, double* col_us,
+                 int* best_row, int* best_col);
+
+/* ---- bandwidth reference: copy n_waves waves in place with the access pattern of the row
+ *      sweep (pattern 0) or of the column sweep's LDS tiles (pattern 1, column variant
+ *      `variant`), no arithmetic; average microseconds per launch.  Used to calibrate HBM
+ *      traffic counters and as the achievable-bandwidth reference of each sweep. ---- */
+int mse_bandwidth_probe(mse_engine* e, int pattern, int variant, int n_waves, int reps, double* us_per_launch);
+
 /* ---- profiling: per-kernel device time measured with hipEvents on the engine stream ---- */
 int mse_profile_enable(mse_engine* e, int enable);
 /* kernel ids: 0 row pass, 1 column pass, 2 potential, 3 probe init, 4 other */
