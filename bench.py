@@ -75,6 +75,25 @@ def plain_rate(engine, g, pos, probe, dets):
     return len(pos) * ns / dt, np.array(out)
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "slice_sweeps_512_B64.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a copy kernel of known size), or None."""
+    try:
+        text = open(PROFILE_SUMMARY).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            for item in line.split(":", 1)[1].split():
+                k, v = item.split("=")
+                if k == kernel:
+                    return int(v)
+    return None
+
+
 def roofline(engine, g, pos, batch, probe, dets):
     N = g.resolution * g.resolution
     ns = g.n_slices
@@ -100,7 +119,9 @@ def roofline(engine, g, pos, batch, probe, dets):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": profiled_traffic(name) if (batch == 64 and g.resolution == 512) else None,
+        "traffic_source": "profiles/slice_sweeps_512_B64.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                          "calibrated with a copy kernel)",
         "launches": n,
         "avg_launch_us": round(tms * 1e3 / n, 2),
         "bytes_per_launch": int(b / n),
@@ -151,8 +172,13 @@ def main():
     ap.add_argument("--plain-probes", type=int, default=8)
     ap.add_argument("--cpu-probes", type=int, default=16)
     ap.add_argument("--check-probes", type=int, default=4)
+    ap.add_argument("--roof-batch", type=int, default=64, help="batch size of the roofline measurement")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1, help="replay each probe batch as a captured hipGraph")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: one batch of --batch probes through every slice, "
+                         "then 5 launches each of the row and column-tile copy kernels "
+                         "(the traffic-counter calibration); prints kernel timings only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -171,6 +197,19 @@ def main():
     N = g.resolution * g.resolution
     ns = g.n_slices
     log(f"{dev}, {cus} CUs; {len(spec)} atoms, {g.resolution}^2, {ns} slices, {len(pos)} probes")
+
+    if args.profile:
+        build_config(engine, 0, spec.displaced(w["u_rms"], w["seed"], 0))
+        engine.stem_batch(0, pos[: args.batch], args.batch, probe, dets, use_graph=False)
+        engine.synchronize()
+        roof = roofline(engine, g, pos, args.batch, probe, dets)
+        copy_row = engine.bandwidth_probe(0, args.batch, reps=5)
+        copy_col = engine.bandwidth_probe(1, args.batch, reps=5, variant=0)
+        print(json.dumps({"profile_run": True, "batch": args.batch, "roofline": roof,
+                          "copy_row_us": copy_row, "copy_col_us": copy_col,
+                          "copy_bytes_per_launch": args.batch * 16 * N}), flush=True)
+        engine.close()
+        return 0
 
     n_cfg = args.warmup + args.steps
     displaced = [spec.displaced(w["u_rms"], w["seed"], c) for c in range(n_cfg)]
@@ -222,7 +261,7 @@ def main():
     log(f"plain (B=1, sync per slice): {plain:.0f} probe.slices/s")
     best_b = max((b for b in sweep if b != "1"), key=lambda b: sweep[b], default=str(args.batch))
 
-    roof = roofline(engine, g, pos, int(best_b), probe, dets)
+    roof = roofline(engine, g, pos, args.roof_batch, probe, dets)
     log(f"roofline: {roof}")
 
     cpu = None

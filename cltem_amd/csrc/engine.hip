@@ -119,7 +119,10 @@ struct mse_engine {
     DevBuf<int> tmp_Z, tmp_bid, tmp_zid;
     DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
     DevBuf<unsigned char> cub_tmp;
-    int row_var = 0, col_var = 0;  // kernel variants (see row_pass / col_pass)
+    int row_var = 4, col_var = 7;  // kernel variants (see row_pass / col_pass); pipelined by default
+    hipGraphExec_t stem_exec = nullptr;
+    std::vector<unsigned char> stem_key;
+    DevBuf<long long> dev_offset;
     int num_cus = 256;
     int det_tiles = 0;             // column tiles of the last detector launch
     // profiling
@@ -131,6 +134,8 @@ struct mse_engine {
 };
 
 namespace {
+
+void destroy_stem_graph(mse_engine* e);
 
 struct DeviceGuard {
     int prev = -1;
@@ -477,6 +482,7 @@ int mse_destroy(mse_engine* e) {
             (void)hipEventDestroy(std::get<2>(t));
         }
         for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+        destroy_stem_graph(e);
         e->configs.clear();
         hipStream_t s = e->stream;
         delete e;
@@ -899,14 +905,27 @@ void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, c
     }
 }
 
-void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_out, int accumulate) {
+void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_out, int accumulate,
+                      const long long* dev_offset = nullptr) {
     const int nwd = nw * n_det;
     const int ntiles = e->det_tiles;
     launch(e, PK_OTHER, [&]() {
         mse::k_reduce_partials<<<(nwd + 255) / 256, 256, 0, e->stream>>>(e->partials.p, ntiles, nwd, dev_out,
-                                                                          accumulate, sqrt2 ? M_SQRT2 : 1.0);
+                                                                          accumulate, sqrt2 ? M_SQRT2 : 1.0,
+                                                                          dev_offset);
     });
 }
+
+// Captured STEM batch: probe init, every slice and the fused detector reduction of one batch
+// of `nb` probes, replayed per batch.  Positions and the output offset live in device
+// memory and are updated (outside the graph) before each replay.
+struct StemGraphKey {
+    int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices;
+    const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos;
+    mse_probe probe;
+};
+
+void destroy_stem_graph(mse_engine* e);
 }  // namespace
 
 extern "C" {
@@ -1031,18 +1050,77 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         da.det_bits = e->det_bits.p;
         da.ndet = n_det;
         da.partials = e->partials.p;
-        for (int p0 = 0; p0 < n_probes; p0 += batch) {
+        const bool graph = use_graph && !e->prof;
+        if (graph) {
+            e->dev_offset.ensure(1);
+            StemGraphKey k;
+            std::memset(&k, 0, sizeof(k));
+            k.res = e->res;
+            k.cfg = cfg;
+            k.nb = batch;
+            k.n_det = n_det;
+            k.sqrt2 = sqrt2;
+            k.row_var = e->row_var;
+            k.col_var = e->col_var;
+            k.n_slices = e->g.n_slices;
+            k.waves = e->waves.p;
+            k.partials = e->partials.p;
+            k.det_out = e->det_out.p;
+            k.det_bits = e->det_bits.p;
+            k.trans = c.trans.p;
+            k.probe_pos = e->probe_pos.p;
+            k.probe = *probe;
+            std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
+            if (!e->stem_exec || key != e->stem_key) {
+                destroy_stem_graph(e);
+                hipGraph_t gr = nullptr;
+                HIPX(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                probes_into_waves(e, batch, probe);
+                multislice(e, batch, 0, e->g.n_slices, true, &da);
+                reduce_detectors(e, batch, n_det, sqrt2, e->det_out.p, 1, e->dev_offset.p);
+                HIPX(hipStreamEndCapture(e->stream, &gr));
+                hipError_t st = hipGraphInstantiate(&e->stem_exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (st != hipSuccess) {
+                    e->stem_exec = nullptr;
+                    throw Error(MSE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(st));
+                }
+                e->stem_key = key;
+            }
+        }
+        std::vector<long long> offsets;
+        offsets.reserve((n_probes + batch - 1) / batch);
+        for (int p0 = 0; p0 < n_probes; p0 += batch) offsets.push_back((long long)p0 * n_det);
+        for (int p0 = 0, bi = 0; p0 < n_probes; p0 += batch, ++bi) {
             const int nb = std::min(batch, n_probes - p0);
             upload_positions(e, pos_xy + 2 * (size_t)p0, nb);
-            probes_into_waves(e, nb, probe);
-            multislice(e, nb, 0, e->g.n_slices, true, &da);
-            reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
+            if (graph && nb == batch) {
+                HIPX(hipMemcpyAsync(e->dev_offset.p, &offsets[bi], sizeof(long long), hipMemcpyHostToDevice,
+                                    e->stream));
+                HIPX(hipGraphLaunch(e->stem_exec, e->stream));
+            } else {
+                probes_into_waves(e, nb, probe);
+                multislice(e, nb, 0, e->g.n_slices, true, &da);
+                reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
+            }
         }
         HIPX(hipMemcpyAsync(accum, e->det_out.p, (size_t)n_probes * n_det * sizeof(double), hipMemcpyDeviceToHost,
                             e->stream));
         HIPX(hipStreamSynchronize(e->stream));
     });
 }
+
+}  // extern "C"
+
+namespace {
+void destroy_stem_graph(mse_engine* e) {
+    if (e->stem_exec) (void)hipGraphExecDestroy(e->stem_exec);
+    e->stem_exec = nullptr;
+    e->stem_key.clear();
+}
+}  // namespace
+
+extern "C" {
 
 int mse_set_kernel_variant(mse_engine* e, int kernel, int variant) {
     return run_guarded("mse_set_kernel_variant", e, [&]() {

@@ -463,15 +463,19 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
     }
 }
 
-// sum per-tile partials in fixed order: out[w*ndet + d] (+)= sum_tiles
+// sum per-tile partials in fixed order: out[w*ndet + d] (+)= sum_tiles; when dev_offset is
+// given the output starts dev_offset[0] entries further (read on the device, so a captured
+// graph can be replayed for successive probe batches)
 __global__ void k_reduce_partials(const double* __restrict__ partials, int ntiles, int nwd,
-                                  double* __restrict__ out, int accumulate, double factor) {
+                                  double* __restrict__ out, int accumulate, double factor,
+                                  const long long* __restrict__ dev_offset) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nwd) return;
     double s = 0.0;
     for (int k = 0; k < ntiles; ++k) s += partials[(size_t)i * ntiles + k];
     s *= factor;
-    out[i] = accumulate ? out[i] + s : s;
+    double* o = out + (dev_offset ? dev_offset[0] : 0LL);
+    o[i] = accumulate ? o[i] + s : s;
 }
 
 }  // namespace mse

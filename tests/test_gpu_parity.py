@@ -97,3 +97,33 @@ def test_stem_small_vs_oracle(engine, oracle):
     engine.multislice()
     split = engine.stem_detect(dets)
     assert np.abs(split - got).max() / np.abs(got).max() <= 1e-5
+
+
+def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
+    w = wl.stem_srtio3(nx=6, ny=6, nz=6, resolution=256, pixels=8, window_cells=(2, 3),
+                       detectors=((60.0, 200.0), (0.0, 20.0)))
+    g, og, p, spec = setup_case(engine, oracle, w)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    pos = wl.stem_positions(g)[:32]
+    probe = cltem_amd.make_probe(w["cond_ap_mrad"])
+    dets = cltem_amd.make_detectors(w["detectors"])
+    eager = engine.stem_batch(0, pos, 8, probe, dets, use_graph=False)
+    for rep in range(3):
+        # eager work and synchronisation between replays
+        engine.set_wave_count(2)
+        engine.set_wave_configs([0, 0])
+        engine.init_probes(pos[:2], probe)
+        engine.multislice(0, 2)
+        engine.synchronize()
+        replay = engine.stem_batch(0, pos, 8, probe, dets, use_graph=True)
+        np.testing.assert_array_equal(replay, eager)
+    # every kernel variant computes the same detector values
+    for rv in range(5):
+        for cv in range(8):
+            engine.set_kernel_variant(0, rv)
+            engine.set_kernel_variant(1, cv)
+            out = engine.stem_batch(0, pos[:8], 8, probe, dets)
+            assert np.max(np.abs(out - eager[:8]) / np.abs(eager[:8])) < 1e-5, (rv, cv)
+    engine.set_kernel_variant(0, 4)
+    engine.set_kernel_variant(1, 7)
