@@ -1,0 +1,157 @@
+"""GPU parity per scope row: potentials (all forms, projected and full 3D), beam tilt,
+probe aberrations and taper, FFT sizes, and binning across phonon configurations."""
+import numpy as np
+import pytest
+
+import cltem_amd
+from cltem_amd import specimens as sp
+from cltem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def prepare(engine, oracle, scen, spec, params="kirkland"):
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
+    og = oracle.geometry(oracle.scenario(**scen))
+    p = cltem_amd.load_params(params)
+    engine.set_geometry(g)
+    engine.set_parameterisation(p)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+    return g, og, p, o
+
+
+def ctem_compare(engine, oracle, g, og, p, o, tol=1e-4):
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    ew = engine.exit_wave(0)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    psi, recip = oracle.run_plane(trans, oracle.propagator(og))
+    err = rel_l2(ew, psi)
+    print(f"res {g.resolution}: {g.n_slices} slices, exit-wave rel-L2 {err:.3e}")
+    assert err <= tol
+    return trans
+
+
+@pytest.mark.parametrize("params", ["peng", "lobato"])
+def test_projected_peng_lobato(engine, oracle, params):
+    w = wl.ctem_srtio3(3, 3, 4, resolution=256, params=params)
+    spec = w["specimen"].displaced(0.08, 11, 0)
+    g, og, p, o = prepare(engine, oracle, w["scenario"], spec, params)
+    trans = ctem_compare(engine, oracle, g, og, p, o)
+    for s in range(0, g.n_slices, 3):
+        assert rel_l2(engine.read_transmission(0, s), trans[s]) <= 1e-5
+
+
+@pytest.mark.parametrize("params", ["kirkland", "lobato", "peng"])
+def test_full3d_amorphous(engine, oracle, params):
+    spec = sp.random_box({"Si": 60, "O": 120}, (14.0, 14.0, 9.0), 1.5, seed=3)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=300.0, slice_thickness=1.0, struct_min=tuple(lo),
+                struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]), full3d=1, full3d_integrals=20)
+    g, og, p, o = prepare(engine, oracle, scen, spec, params)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    errs = [rel_l2(engine.read_transmission(0, s), trans[s]) for s in range(g.n_slices)]
+    print(params, "full-3D transmission rel-L2 max", max(errs))
+    assert max(errs) <= 1e-5
+    ctem_compare(engine, oracle, g, og, p, o)
+
+
+def test_beam_tilt(engine, oracle):
+    w = wl.ctem_srtio3(3, 3, 5, resolution=256)
+    scen = dict(w["scenario"], beam_tilt_mrad=8.0, beam_azimuth=0.7)
+    g, og, p, o = prepare(engine, oracle, scen, w["specimen"])
+    ctem_compare(engine, oracle, g, og, p, o)
+
+
+def test_full3d_beam_tilt(engine, oracle):
+    spec = sp.random_box({"Si": 30, "O": 60}, (12.0, 12.0, 6.0), 1.6, seed=9)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, slice_thickness=1.0, struct_min=tuple(lo), struct_max=tuple(hi),
+                area=(lo[0], hi[0], lo[1], hi[1]), full3d=1, full3d_integrals=10, beam_tilt_mrad=6.0,
+                beam_azimuth=1.1)
+    g, og, p, o = prepare(engine, oracle, scen, spec, "kirkland")
+    ctem_compare(engine, oracle, g, og, p, o)
+
+
+@pytest.mark.parametrize("taper", [False, True])
+def test_probe_aberrations(engine, oracle, taper):
+    w = wl.stem_srtio3(nx=4, ny=4, nz=2, resolution=256, pixels=4, window_cells=(1, 2))
+    g, og, p, o = prepare(engine, oracle, w["scenario"], w["specimen"])
+    ab = {"C10": 40.0, "C12": (15.0, 0.4), "C21": (300.0, 1.0), "C23": (120.0, -0.5), "C30": 2.0e4,
+          "C32": (5e3, 0.2), "C34": (3e3, 0.9), "C41": (1e5, 0.1), "C43": (8e4, 0.3),
+          "C45": (6e4, 0.5), "C50": 1e6, "C52": (1e5, 0.6), "C54": (2e5, 0.7), "C56": (3e5, 0.8)}
+    probe = cltem_amd.make_probe(22.0, smooth_mrad=2.0 if taper else 0.0, taper=taper, aberrations=ab)
+    pos = wl.stem_positions(g)[[0, 5, 10]]
+    engine.set_wave_count(3)
+    engine.set_wave_configs([0, 0, 0])
+    engine.init_probes(pos, probe)
+    vec = cltem_amd.probe_aberration_vector(probe)
+    for i in range(3):
+        got = engine.exit_wave(i)
+        k = oracle.probe_kspace(og, pos[i, 0] - og.sim_x[0], pos[i, 1] - og.sim_y[0], vec, 22.0,
+                                2.0 if taper else 0.0, taper)
+        ref = oracle.fft2(k, inverse=True)
+        assert rel_l2(got, ref) <= 2e-6
+
+
+@pytest.mark.parametrize("res", [384, 768, 1024])
+def test_other_fft_sizes(engine, oracle, res):
+    w = wl.ctem_srtio3(2, 2, 3, resolution=res)
+    spec = w["specimen"].displaced(0.05, 5, 1)
+    g, og, p, o = prepare(engine, oracle, w["scenario"], spec)
+    ctem_compare(engine, oracle, g, og, p, o)
+
+
+def test_binning_bit_exact_phonon_configs(engine, oracle):
+    w = wl.stem_srtio3()
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    engine.set_geometry(g)
+    engine.set_parameterisation(cltem_amd.load_params("kirkland"))
+    for c in range(10):
+        spec = w["specimen"].displaced(w["u_rms"], w["seed"], c)
+        r = engine.sort_atoms(c % 3, spec.x, spec.y, spec.z, spec.Z)
+        o = oracle.sort_atoms(og, spec.x, spec.y, spec.z, spec.Z)
+        assert r["n_in_range"] == o["n_in_range"]
+        np.testing.assert_array_equal(r["block_start"], o["block_start"])
+        keep = r["block_ids"] >= 0
+        np.testing.assert_array_equal(r["block_ids"][keep], o["block_ids"])
+        np.testing.assert_array_equal(r["z_ids"][keep], o["z_ids"])
+
+
+def test_empty_specimen_plane_wave(engine):
+    # no atoms: the exit wave stays 1 (P(k=0) = 1, band limit keeps k = 0)
+    scen = dict(mode=1, resolution=512, slice_thickness=2.0, struct_min=(0, 0, 0),
+                struct_max=(10, 10, 10), area=(0, 10, 0, 10))
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
+    engine.set_geometry(g)
+    engine.set_parameterisation(cltem_amd.load_params("kirkland"))
+    engine.sort_atoms(0, [], [], [], [])
+    engine.build_transmission(0)
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    np.testing.assert_allclose(engine.exit_wave(0), 1.0, atol=2e-6)
+
+
+def test_errors_are_reported(engine):
+    scen = dict(mode=1, resolution=512, slice_thickness=2.0, struct_min=(0, 0, 0),
+                struct_max=(10, 10, 10), area=(0, 10, 0, 10))
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
+    engine.set_geometry(g)
+    with pytest.raises(cltem_amd.MseError):
+        engine.build_transmission(5)  # configuration never sorted
+    with pytest.raises(cltem_amd.MseError):
+        engine.sort_atoms(0, [1.0], [1.0], [1.0], [200])  # Z outside the parameterisation
+    bad = cltem_amd.plan_geometry(cltem_amd.make_scenario(**dict(scen, resolution=640)))
+    with pytest.raises(cltem_amd.MseError):
+        engine.set_geometry(bad)
