@@ -109,6 +109,7 @@ class Engine:
         _check(self.lib.mse_create(device, C.byref(h)))
         self.h = h
         self.g: MseGeometry | None = None
+        self._cfg_geom: dict = {}  # configurations with a geometry of their own
         self.n_waves = 0
 
     def close(self):
@@ -139,10 +140,16 @@ class Engine:
     def set_geometry(self, g: MseGeometry):
         _check(self.lib.mse_set_geometry(self.h, C.byref(g)))
         self.g = g
+        self._cfg_geom = {}  # mse_set_geometry returns every configuration to the engine's
 
     def set_parameterisation(self, p: Parameterisation):
         v = np.ascontiguousarray(p.values, dtype=np.float64)
         _check(self.lib.mse_set_parameterisation(self.h, p.form, p.i_per_atom, _dp(v), len(v)))
+
+    def config_geometry(self, config: int) -> MseGeometry:
+        """The geometry a configuration is binned and sliced with (its own one after
+        set_config_geometry, else the engine's)."""
+        return self._cfg_geom.get(config, self.g)
 
     def sort_atoms(self, config: int, x, y, z, Z, want_ids: bool = True):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -150,7 +157,7 @@ class Engine:
         z = np.ascontiguousarray(z, dtype=np.float64)
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         n = len(x)
-        g = self.g
+        g = self.config_geometry(config)
         bs = np.zeros(g.n_slices * g.blocks_x * g.blocks_y + 1, dtype=np.int32)
         bid = np.zeros(max(n, 1), dtype=np.int32)
         zid = np.zeros(max(n, 1), dtype=np.int32)
@@ -162,7 +169,7 @@ class Engine:
         return dict(block_ids=bid[:n], z_ids=zid[:n], block_start=bs, n_in_range=nr.value)
 
     def build_transmission(self, config: int, first: int = 0, count: int | None = None):
-        count = self.g.n_slices - first if count is None else count
+        count = self.config_geometry(config).n_slices - first if count is None else count
         _check(self.lib.mse_build_transmission(self.h, config, first, count))
 
     def read_transmission(self, config: int, slice_: int) -> np.ndarray:
@@ -171,6 +178,10 @@ class Engine:
         _check(self.lib.mse_read_transmission(self.h, config, slice_,
                                               out.ctypes.data_as(C.POINTER(C.c_float))))
         return out.view(np.complex64).reshape(res, res)
+
+    def set_config_geometry(self, config: int, g: MseGeometry):
+        _check(self.lib.mse_set_config_geometry(self.h, config, C.byref(g)))
+        self._cfg_geom[config] = g
 
     # -- waves --------------------------------------------------------------------------
     def set_wave_count(self, n: int):
@@ -187,6 +198,10 @@ class Engine:
     def init_probes(self, positions, probe: MseProbe):
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
         _check(self.lib.mse_init_probes(self.h, _dp(pos), len(pos), C.byref(probe)))
+
+    def init_probes_range(self, first_wave: int, positions, probe: MseProbe):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
+        _check(self.lib.mse_init_probes_range(self.h, first_wave, _dp(pos), len(pos), C.byref(probe)))
 
     def multislice(self, first: int = 0, count: int | None = None):
         count = self.g.n_slices - first if count is None else count
@@ -208,6 +223,39 @@ class Engine:
         out = np.zeros(2 * res * res)
         _check(self.lib.mse_exit_wave(self.h, wave, _dp(out)))
         return out.view(np.complex128).reshape(res, res)
+
+    def exit_intensity_sum(self, accum=None) -> np.ndarray:
+        res = self.g.resolution
+        out = np.zeros((res, res)) if accum is None else accum
+        assert out.shape == (res, res) and out.dtype == np.float64 and out.flags.c_contiguous
+        _check(self.lib.mse_exit_intensity_sum(self.h, int(accum is not None), _dp(out)))
+        return out
+
+    def diffraction_sum(self, wave_group, n_groups: int, accum=None) -> np.ndarray:
+        res = self.g.resolution
+        grp = np.ascontiguousarray(wave_group, dtype=np.int32)
+        out = np.zeros((n_groups, res, res)) if accum is None else accum
+        assert out.shape == (n_groups, res, res) and out.dtype == np.float64 and out.flags.c_contiguous
+        _check(self.lib.mse_diffraction_sum(self.h, _ip(grp), n_groups, int(accum is not None), _dp(out)))
+        return out
+
+    def forward_checkpointed(self, config: int = 0) -> np.ndarray:
+        """Run every slice of `config` from the current entrance waves, keeping sqrt(n)
+        checkpoints for adjoint_positions; returns |psi_exit|^2, shape (n_waves, res, res)."""
+        res = self.g.resolution
+        out = np.zeros((self.n_waves, res, res))
+        _check(self.lib.mse_forward_checkpointed(self.h, config, _dp(out)))
+        return out
+
+    def adjoint_positions(self, dl_di, n_atoms: int) -> np.ndarray:
+        """dL/d(x, y, z) of every atom of the last sort of the checkpointed configuration
+        for a loss with dL/dI = dl_di (shape (n_waves, res, res)); returns (n_atoms, 3)."""
+        res = self.g.resolution
+        w = np.ascontiguousarray(dl_di, dtype=np.float64)
+        assert w.size == self.n_waves * res * res
+        out = np.zeros((max(n_atoms, 0), 3))
+        _check(self.lib.mse_adjoint_positions(self.h, _dp(w), _dp(out)))
+        return out
 
     def stem_batch(self, config: int, positions, batch: int, probe: MseProbe, detectors,
                    sqrt2: bool = True, use_graph: bool = False, accum=None) -> np.ndarray:

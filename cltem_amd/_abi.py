@@ -120,14 +120,20 @@ SIGNATURES = [
     ("mse_sort_atoms", C.c_int, [_eng, C.c_int, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip, _ip, _ip]),
     ("mse_build_transmission", C.c_int, [_eng, C.c_int, C.c_int, C.c_int]),
     ("mse_read_transmission", C.c_int, [_eng, C.c_int, C.c_int, _fp]),
+    ("mse_set_config_geometry", C.c_int, [_eng, C.c_int, C.POINTER(MseGeometry)]),
     ("mse_set_wave_count", C.c_int, [_eng, C.c_int]),
     ("mse_set_wave_configs", C.c_int, [_eng, _ip, C.c_int]),
     ("mse_init_plane_wave", C.c_int, [_eng, C.c_double]),
     ("mse_init_probes", C.c_int, [_eng, _dp, C.c_int, C.POINTER(MseProbe)]),
+    ("mse_init_probes_range", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.POINTER(MseProbe)]),
     ("mse_multislice", C.c_int, [_eng, C.c_int, C.c_int]),
     ("mse_stem_detect", C.c_int, [_eng, C.POINTER(MseDetector), C.c_int, C.c_int, _dp]),
     ("mse_diffraction", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
+    ("mse_exit_intensity_sum", C.c_int, [_eng, C.c_int, _dp]),
+    ("mse_diffraction_sum", C.c_int, [_eng, _ip, C.c_int, C.c_int, _dp]),
+    ("mse_forward_checkpointed", C.c_int, [_eng, C.c_int, _dp]),
+    ("mse_adjoint_positions", C.c_int, [_eng, _dp, _dp]),
     ("mse_stem_batch", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
                                  C.POINTER(MseDetector), C.c_int, C.c_int, C.c_int, _dp]),
     ("mse_set_kernel_variant", C.c_int, [_eng, C.c_int, C.c_int]),

@@ -1,0 +1,187 @@
+// Adjoint slice loop with sqrt(N) checkpointing: gradients of an exit-intensity loss with
+// respect to the atom positions of one configuration.  clTEM has no counterpart; the
+// forward it differentiates is doMultiSliceStep (microscope/simulationgeneral.cpp:682-769)
+// over the transmission functions of initialiseSimulation (simulationgeneral.cpp:345-646).
+//
+// Waves are held as S = F_x psi (the engine's mixed domain).  One forward slice is
+//   psi' = U(t psi),  U = F2^-1 P F2 (propagator with the band-limit mask),
+// so for a real loss L with g = dL/dpsi* (times 2) the adjoint slice is
+//   u = U^H g' = F2^-1 conj(P) F2 g',  g = conj(t) u,  g_t = conj(psi) u,
+// which is the forward column pass with conj(P') followed by a row pass that multiplies
+// by conj(t) (and emits g_t).  Adjoint states are kept as res * F_x g so that the column
+// pass sees the same scaling as in the forward direction.  The band limit t = BL(exp(i phi))
+// is self-adjoint, so g_tau = BL(g_t) reuses the forward band-limit passes.
+#include "adjoint_kernels.hpp"
+#include "engine_internal.hpp"
+
+namespace mse_rt {
+namespace {
+
+int ceil_sqrt(int n) {
+    int k = 1;
+    while (k * k < n) ++k;
+    return k;
+}
+
+void require_cfg_waves(mse_engine* e, int cfg) {
+    for (int w = 0; w < e->n_waves; ++w)
+        REQUIRE(e->h_wave_cfg[w] == cfg, MSE_ERR_ARG,
+                "every wave must refer to configuration " + std::to_string(cfg));
+}
+
+void copy_planes(mse_engine* e, float2* dst, const float2* src, size_t count) {
+    HIPX(hipMemcpyAsync(dst, src, count * sizeof(float2), hipMemcpyDeviceToDevice, e->stream));
+}
+
+// BL(x) = F2^-1 M F2 x on one plane, in place (the passes of mse_build_transmission)
+void band_limit_plane(mse_engine* e, float2* p) {
+    mse::RowArgs ra = base_row_args(e);
+    ra.scale = 1.0f;
+    row_pass(e, mse::ROW_FFT, p, p, 1, e->N, ra);
+    mse::ColArgs ca = base_col_args(e);
+    ca.lim = e->band_lim.p;
+    ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
+    col_pass(e, mse::COL_BAND, p, p, 1, e->N, ca);
+    row_pass(e, mse::ROW_IFFT, p, p, 1, e->N, ra);
+}
+
+}  // namespace
+}  // namespace mse_rt
+
+extern "C" {
+
+int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity) {
+    return run_guarded("mse_forward_checkpointed", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        check_waves(e);
+        check_trans(e);
+        require_cfg_waves(e, cfg);
+        ConfigData& c = config(e, cfg);
+        const int n = cfg_geom(e, c).n_slices;
+        const int nw = e->n_waves;
+        const size_t plane = (size_t)nw * e->N;
+        const int K = ceil_sqrt(std::max(n, 1));
+        const int n_ck = (n + K - 1) / K;
+        e->adj_ready = false;
+        e->adj_ckpt.ensure(std::max(n_ck, 1) * plane);
+        e->adj_exit.ensure(plane);
+        for (int s = 0; s < n; ++s) {
+            if (s % K == 0) copy_planes(e, e->adj_ckpt.p + (size_t)(s / K) * plane, e->waves.p, plane);
+            slice_step(e, e->waves.p, nw, s);
+        }
+        copy_planes(e, e->adj_exit.p, e->waves.p, plane);
+        if (intensity) {
+            e->diff_img.ensure(plane);
+            mse::RowArgs ra = base_row_args(e);
+            ra.scale = (float)(1.0 / e->res);
+            ra.dout = e->diff_img.p;
+            row_pass(e, mse::ROW_IFFT_INTENSITY, e->waves.p, e->scratch.p, nw, e->N, ra);
+            HIPX(hipMemcpyAsync(intensity, e->diff_img.p, plane * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        }
+        HIPX(hipStreamSynchronize(e->stream));
+        e->adj_cfg = cfg;
+        e->adj_k = K;
+        e->adj_nw = nw;
+        e->adj_nslices = n;
+        e->adj_version = c.version;
+        e->adj_ready = true;
+    });
+}
+
+int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) {
+    return run_guarded("mse_adjoint_positions", e, [&]() {
+        REQUIRE(e && dl_di && grad_xyz, MSE_ERR_ARG, "NULL argument");
+        REQUIRE(e->adj_ready, MSE_ERR_STATE, "mse_forward_checkpointed has not been called");
+        ConfigData& c = config(e, e->adj_cfg);
+        REQUIRE(c.version == e->adj_version, MSE_ERR_STATE,
+                "atoms or transmission functions changed since mse_forward_checkpointed");
+        REQUIRE(e->n_waves == e->adj_nw, MSE_ERR_STATE, "wave count changed since mse_forward_checkpointed");
+        const mse_geometry& g = cfg_geom(e, c);
+        REQUIRE(!g.full3d, MSE_ERR_UNSUPPORTED, "position gradients are implemented for projected potentials");
+        const int n = e->adj_nslices, K = e->adj_k, nw = e->adj_nw;
+        const size_t N = e->N, plane = (size_t)nw * N;
+        const int n_atoms = c.n_input;
+
+        // dL/dI, phases phi_s and the gradient accumulator
+        e->adj_w.ensure(plane);
+        HIPX(hipMemcpyAsync(e->adj_w.p, dl_di, plane * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        e->adj_phase.ensure((size_t)std::max(n, 1) * N);
+        const mse::PotArgs pa = pot_args(e, g);
+        for (int s0 = 0; s0 < n; s0 += 16) {
+            const int cs = std::min(16, n - s0);
+            dim3 grid(e->res / 16, e->res / 16, cs);
+            float* o = e->adj_phase.p + (size_t)s0 * N;
+            launch(e, PK_POT, [&]() {
+                if (e->form == 0)
+                    mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+                else if (e->form == 1)
+                    mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+                else
+                    mse::k_phase_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+            });
+        }
+        const size_t ng = 3 * (size_t)std::max(n_atoms, 1);
+        e->adj_grad.ensure(ng);
+        launch(e, PK_OTHER, [&]() {
+            mse::k_fill_zero<<<(unsigned)((ng + 255) / 256), 256, 0, e->stream>>>(e->adj_grad.p, ng);
+        });
+        // first sorted atom of every slice
+        const int nblk = g.blocks_x * g.blocks_y;
+        std::vector<int> bs((size_t)n * nblk + 1);
+        HIPX(hipMemcpyAsync(bs.data(), c.block_start.p, bs.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+
+        e->adj_g.ensure(plane);
+        e->adj_gt.ensure(plane + N);  // per-wave g_t, then their sum
+        e->adj_seg.ensure((size_t)K * plane);
+        HIPX(hipStreamSynchronize(e->stream));
+
+        // seed: res F_x (2 w psi_N)
+        mse::RowArgs ra = base_row_args(e);
+        ra.weights = e->adj_w.p;
+        row_pass(e, mse::ROW_ADJ_SEED, e->adj_exit.p, e->adj_g.p, nw, N, ra);
+
+        mse::ColArgs ca = base_col_args(e);
+        float2* gsum = e->adj_gt.p + plane;
+        const int n_seg = (n + K - 1) / K;
+        for (int j = n_seg - 1; j >= 0; --j) {
+            const int start = j * K, len = std::min(K, n - start);
+            // recompute the forward states of the segment from its checkpoint
+            copy_planes(e, e->adj_seg.p, e->adj_ckpt.p + (size_t)j * plane, plane);
+            for (int i = 1; i < len; ++i) {
+                float2* cur = e->adj_seg.p + (size_t)i * plane;
+                copy_planes(e, cur, cur - plane, plane);
+                slice_step(e, cur, nw, start + i - 1);
+            }
+            for (int s = start + len - 1; s >= start; --s) {
+                col_pass(e, mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
+                mse::RowArgs rb = base_row_args(e);
+                rb.slice = s;
+                rb.aux_in = e->adj_seg.p + (size_t)(s - start) * plane;
+                rb.aux_out = e->adj_gt.p;
+                row_pass(e, mse::ROW_ADJ, e->adj_g.p, e->adj_g.p, nw, N, rb);
+                const int a0 = bs[(size_t)s * nblk], a1 = bs[(size_t)(s + 1) * nblk];
+                if (a1 <= a0) continue;  // no atoms: phi_s does not depend on positions
+                launch(e, PK_OTHER, [&]() {
+                    mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gsum);
+                });
+                band_limit_plane(e, gsum);
+                dim3 grid(a1 - a0);
+                const float* ph = e->adj_phase.p + (size_t)s * N;
+                launch(e, PK_OTHER, [&]() {
+                    if (e->form == 0)
+                        mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                    else if (e->form == 1)
+                        mse::k_atom_grad_projected<1><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                    else
+                        mse::k_atom_grad_projected<2><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                });
+            }
+        }
+        if (n_atoms > 0)
+            HIPX(hipMemcpyAsync(grad_xyz, e->adj_grad.p, 3 * (size_t)n_atoms * sizeof(double), hipMemcpyDeviceToHost,
+                                e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+}  // extern "C"

@@ -19,7 +19,7 @@ struct BinGeom {
     int blocks_x, blocks_y, n_slices;
 };
 
-__global__ void k_bin_atoms(const double* __restrict__ x, const double* __restrict__ y,
+static __global__ void k_bin_atoms(const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, int n, BinGeom g, unsigned* __restrict__ key,
                             unsigned* __restrict__ idx, int* __restrict__ bid_out,
                             int* __restrict__ zid_out) {
@@ -57,7 +57,7 @@ __global__ void k_bin_atoms(const double* __restrict__ x, const double* __restri
 }
 
 // block_start[k] = first sorted position with key >= k, for k in [0, nkeys]
-__global__ void k_block_starts(const unsigned* __restrict__ sorted_keys, int n, int nkeys,
+static __global__ void k_block_starts(const unsigned* __restrict__ sorted_keys, int n, int nkeys,
                                int* __restrict__ block_start) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nkeys) return;
@@ -72,7 +72,7 @@ __global__ void k_block_starts(const unsigned* __restrict__ sorted_keys, int n, 
     block_start[k] = lo;
 }
 
-__global__ void k_gather_atoms(const unsigned* __restrict__ order, int n, const double* __restrict__ x,
+static __global__ void k_gather_atoms(const unsigned* __restrict__ order, int n, const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ Z, double* __restrict__ sx,
                                double* __restrict__ sy, double* __restrict__ sz, int* __restrict__ sZ) {
@@ -256,17 +256,15 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
     return make_float2(c, s);
 }
 
-// Transmission function of slices [first, first + gridDim.z): one 16x16 pixel tile per
-// block; the atoms of each block row are staged through LDS in chunks of 256, with no
-// limit on how many a row holds.  out has one res*res plane per slice (relative to first).
+// Projected potential sum (sumz) at one pixel of slice first + blockIdx.z: one 16x16 pixel
+// tile per block; the atoms of each block row are staged through LDS in chunks of 256,
+// with no limit on how many a row holds.
 template <int FORM>
-__global__ __launch_bounds__(256) void k_potential_projected(
-    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
-    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
-    PotArgs a, int first) {
+__device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
+                                                 const int* __restrict__ aZ, const int* __restrict__ bsp,
+                                                 const float* __restrict__ params, const PotArgs& a, int first,
+                                                 double* sx, double* sy, int* sZ) {
 #pragma clang fp contract(off)
-    __shared__ double sx[256], sy[256];
-    __shared__ int sZ[256];
     const int lid = threadIdx.x;
     const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
@@ -309,7 +307,40 @@ __global__ __launch_bounds__(256) void k_potential_projected(
             }
         }
     }
+    return sumz;
+}
+
+// Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
+// (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
+// slice (relative to first).
+template <int FORM>
+__global__ __launch_bounds__(256) void k_potential_projected(
+    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
+    PotArgs a, int first) {
+    __shared__ double sx[256], sy[256];
+    __shared__ int sZ[256];
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, params, a, first, sx, sy, sZ);
+    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
+}
+
+// The phase sigma V of the same slices, reduced to [-pi, pi] as phase_to_t reduces it, so
+// that sincosf of the stored float reproduces the transmission function's exp(i sigma V)
+// before the band limit (used by the adjoint).
+template <int FORM>
+__global__ __launch_bounds__(256) void k_phase_projected(
+    float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
+    PotArgs a, int first) {
+    __shared__ double sx[256], sy[256];
+    __shared__ int sZ[256];
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, params, a, first, sx, sy, sZ);
+    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
+        (float)remainder(a.sigma * sumz, 2.0 * M_PI);
 }
 
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the

@@ -19,121 +19,15 @@
 
 #include "../../include/cltem_amd.h"
 #include "atom_kernels.hpp"
-#include "fft_wave.hpp"
-#include "slice_kernels.hpp"
+#include "engine_internal.hpp"
 
 namespace mse_geom {
 void kgrid(int res, double real_scale, double* kx, double* ky, double* bandwidthkmax);
 }
 
-namespace {
 
-thread_local std::string g_err;
+namespace mse_rt {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPX(expr)                                                                         \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            throw Error(MSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-#define REQUIRE(cond, code, msg)              \
-    do {                                      \
-        if (!(cond)) throw Error(code, msg);  \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void ensure(size_t count) {
-        if (count <= n && p) return;
-        release();
-        if (count == 0) return;
-        HIPX(hipMalloc((void**)&p, count * sizeof(T)));
-        n = count;
-    }
-};
-
-struct ConfigData {
-    DevBuf<double> x, y, z;
-    DevBuf<int> Z;
-    DevBuf<int> block_start;
-    int n_in_range = 0;
-    bool sorted = false;
-    DevBuf<float2> trans;  // n_slices * N
-};
-
-enum ProfKind { PK_ROW = 0, PK_COL = 1, PK_POT = 2, PK_PROBE = 3, PK_OTHER = 4, PK_COUNT = 5 };
-
-
-}  // namespace
-
-struct mse_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool have_geom = false;
-    mse_geometry g{};
-    int res = 0;
-    size_t N = 0;
-    DevBuf<float2> tw, px, py;
-    DevBuf<int> prop_lim, band_lim;
-    DevBuf<double> kx, ky;
-    std::vector<double> h_kx, h_ky;
-    DevBuf<float> params;
-    int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
-    std::vector<ConfigData> configs;
-    DevBuf<const float2*> cfg_ptrs;
-    int n_waves = 0;
-    DevBuf<float2> waves;
-    DevBuf<int> wave_cfg;
-    std::vector<int> h_wave_cfg;
-    DevBuf<double> probe_pos;
-    DevBuf<unsigned> det_bits;
-    std::vector<double> det_key;  // detectors the bitmasks were built for
-    DevBuf<double> partials;
-    DevBuf<double> det_out;
-    DevBuf<double> diff_img;
-    DevBuf<float2> scratch;
-    // sort scratch
-    DevBuf<double> tmp_xyz;
-    DevBuf<int> tmp_Z, tmp_bid, tmp_zid;
-    DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
-    DevBuf<unsigned char> cub_tmp;
-    int row_var = 4, col_var = 7;  // kernel variants (see row_pass / col_pass); pipelined by default
-    hipGraphExec_t stem_exec = nullptr;
-    std::vector<unsigned char> stem_key;
-    DevBuf<long long> dev_offset;
-    int num_cus = 256;
-    int det_tiles = 0;             // column tiles of the last detector launch
-    // profiling
-    bool prof = false;
-    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
-    std::vector<hipEvent_t> ev_pool;
-    double prof_ms[PK_COUNT] = {0};
-    int prof_n[PK_COUNT] = {0};
-};
-
-namespace {
 
 void destroy_stem_graph(mse_engine* e);
 
@@ -147,33 +41,6 @@ struct DeviceGuard {
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 };
-
-hipEvent_t take_event(mse_engine* e) {
-    if (!e->ev_pool.empty()) {
-        hipEvent_t ev = e->ev_pool.back();
-        e->ev_pool.pop_back();
-        return ev;
-    }
-    hipEvent_t ev;
-    HIPX(hipEventCreate(&ev));
-    return ev;
-}
-
-template <class F>
-void launch(mse_engine* e, int kind, F&& f) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (e->prof) {
-        a = take_event(e);
-        b = take_event(e);
-        HIPX(hipEventRecord(a, e->stream));
-    }
-    f();
-    HIPX(hipGetLastError());
-    if (e->prof) {
-        HIPX(hipEventRecord(b, e->stream));
-        e->pending.emplace_back(kind, a, b);
-    }
-}
 
 void drain_profile(mse_engine* e) {
     if (e->pending.empty()) return;
@@ -204,148 +71,33 @@ void dispatch_res(int res, F&& f) {
     }
 }
 
-// ---- sweeps ------------------------------------------------------------------------------
-// Kernel variants (chosen by mse_autotune or mse_set_kernel_variant):
-//   row    0: default plan, wave-level LDS sync   1: half lanes per line   2: block barriers
-//          3: twiddles read at use, >= 8 waves per SIMD
-//   column 0: default plan, 16 columns per block  1: half lanes per column
-//          2: 8 columns per block                 3: block barriers
-//          4: 16 columns, twiddles read at use, >= 8 waves per SIMD (2 blocks per CU)
-//          5: 8 columns, twiddles read at use, >= 8 waves per SIMD (4 blocks per CU)
-//          6: 32 columns (256-byte row runs), half lanes per column
-//   row    4: persistent lines with the next row prefetched into registers
-//   column 7: persistent blocks with the next tile prefetched into registers
-constexpr int kRowVariants = 5;
-constexpr int kColVariants = 8;
-
+// ---- sweeps: dispatch on the resolution to the per-size launchers ---------------------------
 void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets,
-              size_t wave_stride, const mse::RowArgs& args, int variant = -1) {
+              size_t wave_stride, const mse::RowArgs& args, int variant) {
     if (variant < 0) variant = e->row_var;
-    dispatch_res(e->res, [&](auto Nc) {
-        constexpr int N = decltype(Nc)::value;
-        auto go = [&](auto plan, auto sync, auto twreg) {
-            using PL = decltype(plan);
-            using SY = decltype(sync);
-            constexpr bool TR = decltype(twreg)::value;
-            constexpr int WPE = TR ? 1 : 8;
-            constexpr int LINES = 256 / PL::L;
-            dim3 grid(N / LINES, nlines_sets);
-            launch(e, PK_ROW, [&]() {
-                switch (mode) {
-                    case mse::ROW_IFFT_T_FFT:
-                        mse::k_row<PL, mse::ROW_IFFT_T_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                        break;
-                    case mse::ROW_PLANE_T_FFT:
-                        mse::k_row<PL, mse::ROW_PLANE_T_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                        break;
-                    case mse::ROW_FFT:
-                        mse::k_row<PL, mse::ROW_FFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                        break;
-                    case mse::ROW_IFFT:
-                        mse::k_row<PL, mse::ROW_IFFT, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                        break;
-                    case mse::ROW_COPY:
-                        mse::k_row<PL, mse::ROW_COPY, SY, TR, WPE><<<grid, 256, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args);
-                        break;
-                }
-            });
-        };
-        using P0 = typename mse::PlanFor<N>::P;
-        using P1 = typename mse::PlanHalfLanes<N>::P;
-        using REG = std::true_type;
-        using MEM = std::false_type;
-        if (mode == mse::ROW_IFFT_T_FFT && variant == 4 && in == out) {
-            constexpr int LINES = 256 / P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
-            launch(e, PK_ROW, [&]() {
-                mse::k_row_step_pipe<P0, mse::WaveSync><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
-                                                                                     nlines_sets);
-            });
-            return;
-        }
-        if (mode == mse::ROW_IFFT_T_FFT && variant == 1)
-            go(P1{}, mse::WaveSync{}, REG{});
-        else if (mode == mse::ROW_IFFT_T_FFT && variant == 2)
-            go(P0{}, mse::BlockSync{}, REG{});
-        else if (mode == mse::ROW_IFFT_T_FFT && variant == 3)
-            go(P0{}, mse::WaveSync{}, MEM{});
-        else
-            go(P0{}, mse::WaveSync{}, REG{});
-    });
+    switch (e->res) {
+        case 256: row_pass_n<256>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
+        case 384: row_pass_n<384>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
+        case 512: row_pass_n<512>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
+        case 768: row_pass_n<768>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
+        case 1024: row_pass_n<1024>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
+        default: dispatch_res(e->res, [](auto) {});
+    }
 }
 
 // returns the number of column tiles of the launch (detector partials per wave)
 int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
-             const mse::ColArgs& args, int kind = PK_COL, int variant = -1) {
+             const mse::ColArgs& args, int kind, int variant) {
     if (variant < 0) variant = e->col_var;
-    const bool hot = mode == mse::COL_PROP || mode == mse::COL_PROP_DETECT || mode == mse::COL_COPY;
-    if (!hot) variant = 0;
-    int tiles = 0;
-    dispatch_res(e->res, [&](auto Nc) {
-        constexpr int N = decltype(Nc)::value;
-        auto go = [&](auto plan, auto tc, auto sync, auto twreg) {
-            using PL = decltype(plan);
-            using SY = decltype(sync);
-            constexpr bool TR = decltype(twreg)::value;
-            constexpr int WPE = TR ? 1 : 8;
-            constexpr int TC = decltype(tc)::value;
-            constexpr int NT = TC * PL::L;
-            dim3 grid(N / TC, nsets);
-            tiles = N / TC;
-            launch(e, kind, [&]() {
-                switch (mode) {
-#define MSE_COL_CASE(M) \
-    case mse::M: mse::k_col<PL, TC, mse::M, SY, TR, WPE><<<grid, NT, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args); break;
-                    MSE_COL_CASE(COL_PROP)
-                    MSE_COL_CASE(COL_PROP_DETECT)
-                    MSE_COL_CASE(COL_BAND)
-                    MSE_COL_CASE(COL_PROBE)
-                    MSE_COL_CASE(COL_FFT_DETECT)
-                    MSE_COL_CASE(COL_FFT_DIFF)
-                    MSE_COL_CASE(COL_PROP_STORE_DETECT)
-                    MSE_COL_CASE(COL_COPY)
-#undef MSE_COL_CASE
-                }
-            });
-        };
-        using P0 = typename mse::PlanFor<N>::P;
-        using P1 = typename mse::PlanHalfLanes<N>::P;
-        using C16 = std::integral_constant<int, 16>;
-        using C8 = std::integral_constant<int, 8>;
-        using C32 = std::integral_constant<int, 32>;
-        using REG = std::true_type;
-        using MEM = std::false_type;
-        if (variant == 7 && mode == mse::COL_PROP && in == out) {
-            constexpr int NT = 16 * P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
-            tiles = N / 16;
-            launch(e, kind, [&]() {
-                mse::k_col_prop_pipe<P0, 16, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
-                                                                                          nsets);
-            });
-            return;
-        }
-        if (variant == 7) variant = 0;
-        if (variant == 1)
-            go(P1{}, C16{}, mse::WaveSync{}, REG{});
-        else if (variant == 2)
-            go(P0{}, C8{}, mse::WaveSync{}, REG{});
-        else if (variant == 3)
-            go(P0{}, C16{}, mse::BlockSync{}, REG{});
-        else if (variant == 4)
-            go(P0{}, C16{}, mse::WaveSync{}, MEM{});
-        else if (variant == 5)
-            go(P0{}, C8{}, mse::WaveSync{}, MEM{});
-        else if (variant == 6) {
-            if constexpr (N <= 512)  // 32 columns exceed the 160 KiB LDS above 512 points
-                go(P1{}, C32{}, mse::WaveSync{}, REG{});
-            else
-                go(P0{}, C16{}, mse::WaveSync{}, REG{});
-        }
-        else
-            go(P0{}, C16{}, mse::WaveSync{}, REG{});
-    });
-    return tiles;
+    switch (e->res) {
+        case 256: return col_pass_n<256>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
+        case 384: return col_pass_n<384>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
+        case 512: return col_pass_n<512>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
+        case 768: return col_pass_n<768>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
+        case 1024: return col_pass_n<1024>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
+        default: dispatch_res(e->res, [](auto) {});
+    }
+    return 0;
 }
 
 mse::ColArgs base_col_args(mse_engine* e) {
@@ -364,6 +116,8 @@ mse::ColArgs base_col_args(mse_engine* e) {
 mse::RowArgs base_row_args(mse_engine* e) {
     mse::RowArgs a{};
     a.cfg_trans = e->cfg_ptrs.p;
+    a.cfg_nslices = e->cfg_nslices.p;
+    a.dout = nullptr;
     a.wave_cfg = e->wave_cfg.p;
     a.slice = 0;
     a.plane_value = 0.f;
@@ -373,12 +127,59 @@ mse::RowArgs base_row_args(mse_engine* e) {
 
 void require_geom(mse_engine* e) { REQUIRE(e->have_geom, MSE_ERR_STATE, "mse_set_geometry has not been called"); }
 
+const mse_geometry& cfg_geom(mse_engine* e, const ConfigData& c) { return c.own_geom ? c.g : e->g; }
+
+// device tables of every configuration's transmission base pointer and slice count
 void upload_cfg_ptrs(mse_engine* e) {
     std::vector<const float2*> h(e->configs.size());
-    for (size_t i = 0; i < h.size(); ++i) h[i] = e->configs[i].trans.p;
-    e->cfg_ptrs.ensure(std::max<size_t>(h.size(), 1));
-    if (!h.empty())
+    std::vector<int> ns(e->configs.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+        h[i] = e->configs[i].trans.p;
+        ns[i] = cfg_geom(e, e->configs[i]).n_slices;
+    }
+    const size_t cap = std::max<size_t>(h.size(), 1);
+    if (e->cfg_ptrs.n < cap || e->cfg_nslices.n < cap) {
+        HIPX(hipStreamSynchronize(e->stream));
+        e->cfg_ptrs.ensure(cap);
+        e->cfg_nslices.ensure(cap);
+    }
+    if (!h.empty()) {
         HIPX(hipMemcpyAsync(e->cfg_ptrs.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipMemcpyAsync(e->cfg_nslices.p, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    }
+}
+
+mse::PotArgs pot_args(mse_engine* e, const mse_geometry& g) {
+    mse::PotArgs a{};
+    a.res = e->res;
+    a.pixelscale = g.real_scale;
+    a.startx = g.sim_x[0];
+    a.starty = g.sim_y[0];
+    a.min_x = g.full_x[0];
+    a.max_x = g.full_x[1];
+    a.min_y = g.full_y[0];
+    a.max_y = g.full_y[1];
+    a.blocks_x = g.blocks_x;
+    a.blocks_y = g.blocks_y;
+    a.load_x = g.load_x;
+    a.load_y = g.load_y;
+    a.load_z = g.load_z;
+    a.n_slices = g.n_slices;
+    const double beam_theta = M_PI_2 - g.beam_tilt_mrad * 0.001;  // ..._projected_d.cl:221
+    a.cos_phi = std::cos(g.beam_azimuth);
+    a.sin_phi = std::sin(g.beam_azimuth);
+    a.sin_2theta = std::sin(2.0 * beam_theta);
+    a.tan_theta = std::tan(beam_theta);
+    a.sigma = g.sigma_eff;
+    a.dz = g.slice_thickness;
+    a.z_min = g.z_lims[0];
+    a.shift_x = (g.k_vec[0] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;  // simulationgeneral.cpp:562
+    a.shift_y = (g.k_vec[1] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;
+    a.integrals = g.full3d_integrals;
+    a.per_atom = e->per_atom;
+    a.i_count = e->i_count;
+    return a;
 }
 
 ConfigData& config(mse_engine* e, int c) {
@@ -443,7 +244,7 @@ int run_guarded(const char* fn, mse_engine* e, const std::function<void()>& body
     }
 }
 
-}  // namespace
+}  // namespace mse_rt
 
 extern "C" {
 
@@ -528,7 +329,11 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
             e->n_waves = 0;
             e->det_key.clear();
         }
-        for (auto& c : e->configs) c.sorted = false;  // bins depend on the geometry
+        for (auto& c : e->configs) {  // bins depend on the geometry
+            c.sorted = false;
+            if (c.own_geom) c.trans.release();
+            c.own_geom = false;
+        }
         // k-grid exactly as the reference builds it (double)
         e->h_kx.assign(res, 0.0);
         e->h_ky.assign(res, 0.0);
@@ -632,7 +437,7 @@ int mse_sort_atoms(mse_engine* e, int cfg, const double* x, const double* y, con
                 REQUIRE(Z[i] >= 1 && Z[i] <= e->max_Z, MSE_ERR_ARG,
                         "atomic number " + std::to_string(Z[i]) + " outside the parameterisation");
         ConfigData& c = config(e, cfg);
-        const mse_geometry& g = e->g;
+        const mse_geometry& g = cfg_geom(e, c);
         const int nblk = g.blocks_x * g.blocks_y;
         const long long nkeys = (long long)g.n_slices * nblk;
         REQUIRE(nkeys < (1LL << 30), MSE_ERR_UNSUPPORTED, "too many bins");
@@ -685,6 +490,8 @@ int mse_sort_atoms(mse_engine* e, int cfg, const double* x, const double* y, con
                     mse::k_gather_atoms<<<(nr + 255) / 256, 256, 0, e->stream>>>(e->idx_sorted.p, nr, dx, dy, dzp,
                                                                                e->tmp_Z.p, c.x.p, c.y.p, c.z.p, c.Z.p);
                 });
+                c.orig.ensure(nr);
+                HIPX(hipMemcpyAsync(c.orig.p, e->idx_sorted.p, nr * sizeof(int), hipMemcpyDeviceToDevice, e->stream));
             }
         } else {
             std::vector<int> zeros(nkeys + 1, 0);
@@ -701,6 +508,8 @@ int mse_sort_atoms(mse_engine* e, int cfg, const double* x, const double* y, con
                                 e->stream));
         HIPX(hipStreamSynchronize(e->stream));
         if (n_in_range) *n_in_range = c.n_in_range;
+        c.n_input = n;
+        ++c.version;
         c.sorted = true;
     });
 }
@@ -712,40 +521,14 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         REQUIRE(e->form >= 0, MSE_ERR_STATE, "mse_set_parameterisation has not been called");
         ConfigData& c = config(e, cfg);
         REQUIRE(c.sorted, MSE_ERR_STATE, "atoms of this configuration are not sorted for this geometry");
-        const mse_geometry& g = e->g;
+        const mse_geometry& g = cfg_geom(e, c);
         REQUIRE(first >= 0 && count >= 0 && first + count <= g.n_slices, MSE_ERR_ARG, "slice range out of bounds");
         if (count == 0) return;
+        ++c.version;
         const bool fresh = c.trans.p == nullptr;
         c.trans.ensure((size_t)g.n_slices * e->N);
         if (fresh) upload_cfg_ptrs(e);
-        mse::PotArgs a{};
-        a.res = e->res;
-        a.pixelscale = g.real_scale;
-        a.startx = g.sim_x[0];
-        a.starty = g.sim_y[0];
-        a.min_x = g.full_x[0];
-        a.max_x = g.full_x[1];
-        a.min_y = g.full_y[0];
-        a.max_y = g.full_y[1];
-        a.blocks_x = g.blocks_x;
-        a.blocks_y = g.blocks_y;
-        a.load_x = g.load_x;
-        a.load_y = g.load_y;
-        a.load_z = g.load_z;
-        a.n_slices = g.n_slices;
-        const double beam_theta = M_PI_2 - g.beam_tilt_mrad * 0.001;  // ..._projected_d.cl:221
-        a.cos_phi = std::cos(g.beam_azimuth);
-        a.sin_phi = std::sin(g.beam_azimuth);
-        a.sin_2theta = std::sin(2.0 * beam_theta);
-        a.tan_theta = std::tan(beam_theta);
-        a.sigma = g.sigma_eff;
-        a.dz = g.slice_thickness;
-        a.z_min = g.z_lims[0];
-        a.shift_x = (g.k_vec[0] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;  // simulationgeneral.cpp:562
-        a.shift_y = (g.k_vec[1] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;
-        a.integrals = g.full3d_integrals;
-        a.per_atom = e->per_atom;
-        a.i_count = e->i_count;
+        const mse::PotArgs a = pot_args(e, g);
         float2* outp = c.trans.p + (size_t)first * e->N;
         // potential kernels chunk the slice range so one launch stays short
         const int chunk = g.full3d ? 1 : 16;
@@ -789,7 +572,7 @@ int mse_read_transmission(mse_engine* e, int cfg, int slice, float* out) {
         require_geom(e);
         ConfigData& c = config(e, cfg);
         REQUIRE(c.trans.p, MSE_ERR_STATE, "no transmission functions for this configuration");
-        REQUIRE(slice >= 0 && slice < e->g.n_slices, MSE_ERR_ARG, "slice out of range");
+        REQUIRE(slice >= 0 && slice < cfg_geom(e, c).n_slices, MSE_ERR_ARG, "slice out of range");
         HIPX(hipMemcpyAsync(out, c.trans.p + (size_t)slice * e->N, e->N * sizeof(float2), hipMemcpyDeviceToHost,
                             e->stream));
         HIPX(hipStreamSynchronize(e->stream));
@@ -831,7 +614,7 @@ int mse_set_wave_configs(mse_engine* e, const int* configs, int n_waves) {
 
 }  // extern "C"
 
-namespace {
+namespace mse_rt {
 __global__ void k_fill_plane(float2* __restrict__ w, int res, size_t N, int n_waves, float v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N * (size_t)n_waves) return;
@@ -892,6 +675,13 @@ void probes_into_waves(mse_engine* e, int n, const mse_probe* p) {
     col_pass(e, mse::COL_PROBE, e->waves.p, e->waves.p, n, e->N, ca, PK_PROBE);
 }
 
+void slice_step(mse_engine* e, float2* waves, int nw, int slice) {
+    mse::RowArgs ra = base_row_args(e);
+    ra.slice = slice;
+    row_pass(e, mse::ROW_IFFT_T_FFT, waves, waves, nw, e->N, ra);
+    col_pass(e, mse::COL_PROP, waves, waves, nw, e->N, base_col_args(e));
+}
+
 void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, const mse::ColArgs* det_args) {
     mse::RowArgs ra = base_row_args(e);
     mse::ColArgs ca = base_col_args(e);
@@ -924,9 +714,7 @@ struct StemGraphKey {
     const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos;
     mse_probe probe;
 };
-
-void destroy_stem_graph(mse_engine* e);
-}  // namespace
+}  // namespace mse_rt
 
 extern "C" {
 
@@ -1112,15 +900,120 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
 
 }  // extern "C"
 
-namespace {
+namespace mse_rt {
 void destroy_stem_graph(mse_engine* e) {
     if (e->stem_exec) (void)hipGraphExecDestroy(e->stem_exec);
     e->stem_exec = nullptr;
     e->stem_key.clear();
 }
-}  // namespace
+}  // namespace mse_rt
 
 extern "C" {
+
+int mse_set_config_geometry(mse_engine* e, int cfg, const mse_geometry* g) {
+    return run_guarded("mse_set_config_geometry", e, [&]() {
+        REQUIRE(e && g, MSE_ERR_ARG, "NULL argument");
+        require_geom(e);
+        const mse_geometry& m = e->g;
+        // only the slicing of the specimen may differ: same grid, field, blocks and optics
+        const bool same_xy = g->resolution == m.resolution && g->real_scale == m.real_scale &&
+                             g->full_x[0] == m.full_x[0] && g->full_x[1] == m.full_x[1] &&
+                             g->full_y[0] == m.full_y[0] && g->full_y[1] == m.full_y[1] &&
+                             g->sim_x[0] == m.sim_x[0] && g->sim_y[0] == m.sim_y[0] &&
+                             g->blocks_x == m.blocks_x && g->blocks_y == m.blocks_y && g->load_x == m.load_x &&
+                             g->load_y == m.load_y && g->load_z == m.load_z &&
+                             g->slice_thickness == m.slice_thickness && g->wavelength == m.wavelength &&
+                             g->sigma_eff == m.sigma_eff && g->k_vec[0] == m.k_vec[0] &&
+                             g->k_vec[1] == m.k_vec[1] && g->k_vec[2] == m.k_vec[2] &&
+                             g->full3d == m.full3d && g->full3d_integrals == m.full3d_integrals &&
+                             g->kmax_band == m.kmax_band && g->limit_factor == m.limit_factor;
+        REQUIRE(same_xy, MSE_ERR_ARG,
+                "configuration geometry must share the engine's grid, field, blocks, slice thickness and optics");
+        REQUIRE(g->n_slices >= 1 && g->n_slices <= m.n_slices, MSE_ERR_ARG,
+                "configuration has more slices than the engine geometry (set the thickest specimen's geometry "
+                "on the engine)");
+        ConfigData& c = config(e, cfg);
+        HIPX(hipStreamSynchronize(e->stream));
+        c.trans.release();
+        c.sorted = false;
+        c.g = *g;
+        c.own_geom = true;
+        upload_cfg_ptrs(e);
+    });
+}
+
+int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, int n, const mse_probe* probe) {
+    return run_guarded("mse_init_probes_range", e, [&]() {
+        REQUIRE(e && pos_xy && probe, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(first_wave >= 0 && n >= 1 && first_wave + n <= e->n_waves, MSE_ERR_ARG, "wave range out of bounds");
+        std::vector<double> rel(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            rel[2 * i] = pos_xy[2 * i] - e->g.sim_x[0];  // simulationcbed.cpp:54-59
+            rel[2 * i + 1] = pos_xy[2 * i + 1] - e->g.sim_y[0];
+        }
+        HIPX(hipMemcpyAsync(e->probe_pos.p + 2 * (size_t)first_wave, rel.data(), rel.size() * sizeof(double),
+                            hipMemcpyHostToDevice, e->stream));
+        mse::ColArgs ca = base_col_args(e);
+        ca.probe = probe_args(e, probe);
+        ca.probe.pos = e->probe_pos.p + 2 * (size_t)first_wave;
+        float2* w = e->waves.p + (size_t)first_wave * e->N;
+        col_pass(e, mse::COL_PROBE, w, w, n, e->N, ca, PK_PROBE);
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_exit_intensity_sum(mse_engine* e, int accumulate, double* out) {
+    return run_guarded("mse_exit_intensity_sum", e, [&]() {
+        REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        const int nw = e->n_waves;
+        e->diff_img.ensure((size_t)nw * e->N);
+        e->det_out.ensure(e->N);
+        if (accumulate)
+            HIPX(hipMemcpyAsync(e->det_out.p, out, e->N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        mse::RowArgs ra = base_row_args(e);
+        ra.scale = (float)(1.0 / e->res);
+        ra.dout = e->diff_img.p;
+        row_pass(e, mse::ROW_IFFT_INTENSITY, e->waves.p, e->scratch.p, nw, e->N, ra);
+        launch(e, PK_OTHER, [&]() {
+            mse::k_sum_waves<<<(unsigned)((e->N + 255) / 256), 256, 0, e->stream>>>(e->diff_img.p, nw, e->N,
+                                                                                   e->det_out.p, accumulate);
+        });
+        HIPX(hipMemcpyAsync(out, e->det_out.p, e->N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_diffraction_sum(mse_engine* e, const int* wave_group, int n_groups, int accumulate, double* out) {
+    return run_guarded("mse_diffraction_sum", e, [&]() {
+        REQUIRE(e && wave_group && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(n_groups >= 1, MSE_ERR_ARG, "n_groups must be positive");
+        const int nw = e->n_waves;
+        for (int w = 0; w < nw; ++w)
+            REQUIRE(wave_group[w] >= 0 && wave_group[w] < n_groups, MSE_ERR_ARG, "wave group out of range");
+        e->diff_img.ensure((size_t)nw * e->N);
+        e->det_out.ensure((size_t)n_groups * e->N);
+        e->tmp_bid.ensure(nw);
+        HIPX(hipMemcpyAsync(e->tmp_bid.p, wave_group, nw * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        if (accumulate)
+            HIPX(hipMemcpyAsync(e->det_out.p, out, (size_t)n_groups * e->N * sizeof(double), hipMemcpyHostToDevice,
+                                e->stream));
+        mse::ColArgs ca = base_col_args(e);
+        ca.scale = (float)(1.0 / e->res);
+        ca.diff = e->diff_img.p;
+        ca.diff_accumulate = 0;
+        col_pass(e, mse::COL_FFT_DIFF, e->waves.p, nullptr, nw, e->N, ca);
+        launch(e, PK_OTHER, [&]() {
+            mse::k_sum_groups<<<(unsigned)((e->N + 255) / 256), 256, 0, e->stream>>>(
+                e->diff_img.p, nw, e->N, e->tmp_bid.p, n_groups, e->det_out.p, accumulate);
+        });
+        HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)n_groups * e->N * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
 
 int mse_set_kernel_variant(mse_engine* e, int kernel, int variant) {
     return run_guarded("mse_set_kernel_variant", e, [&]() {
@@ -1275,3 +1168,4 @@ int mse_profile_reset(mse_engine* e) {
 }
 
 }  // extern "C"
+

@@ -1,0 +1,223 @@
+// Engine state shared by the translation units of libcltem_amd.so (not part of the ABI).
+#pragma once
+// cltem_amd engine: HIP runtime state and the C ABI of include/cltem_amd.h.
+//
+// One engine = one GPU, one HIP stream (default priority), device buffers owned by
+// the engine and reused across calls, as clTEM's SimulationGeneral owns its clMemory
+// members (microscope/simulationgeneral.h:74-95).  Errors are C++ exceptions inside
+// and become an int status plus a thread-local message at the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cltem_amd.h"
+#include "fft_wave.hpp"
+#include "slice_kernels.hpp"
+
+
+
+namespace mse {
+struct PotArgs;
+}
+
+namespace mse_rt {
+
+inline thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw Error(MSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+#define REQUIRE(cond, code, msg)              \
+    do {                                      \
+        if (!(cond)) throw Error(code, msg);  \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        HIPX(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+};
+
+struct ConfigData {
+    DevBuf<double> x, y, z;
+    DevBuf<int> Z;
+    DevBuf<int> block_start;
+    DevBuf<int> orig;       // input index of each sorted atom
+    int n_input = 0;        // atoms passed to the last sort (in range or not)
+    unsigned version = 0;   // bumped whenever the atoms or transmission functions change
+    int n_in_range = 0;
+    bool sorted = false;
+    DevBuf<float2> trans;  // n_slices * N
+    bool own_geom = false;  // z-geometry of its own (a specimen of another thickness)
+    mse_geometry g{};
+};
+
+enum ProfKind : int { PK_ROW = 0, PK_COL = 1, PK_POT = 2, PK_PROBE = 3, PK_OTHER = 4, PK_COUNT = 5 };
+
+
+
+}  // namespace mse_rt
+
+using namespace mse_rt;
+
+struct mse_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool have_geom = false;
+    mse_geometry g{};
+    int res = 0;
+    size_t N = 0;
+    DevBuf<float2> tw, px, py;
+    DevBuf<int> prop_lim, band_lim;
+    DevBuf<double> kx, ky;
+    std::vector<double> h_kx, h_ky;
+    DevBuf<float> params;
+    int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
+    std::vector<ConfigData> configs;
+    DevBuf<const float2*> cfg_ptrs;
+    DevBuf<int> cfg_nslices;
+    int n_waves = 0;
+    DevBuf<float2> waves;
+    DevBuf<int> wave_cfg;
+    std::vector<int> h_wave_cfg;
+    DevBuf<double> probe_pos;
+    DevBuf<unsigned> det_bits;
+    std::vector<double> det_key;  // detectors the bitmasks were built for
+    DevBuf<double> partials;
+    DevBuf<double> det_out;
+    DevBuf<double> diff_img;
+    DevBuf<float2> scratch;
+    // sort scratch
+    DevBuf<double> tmp_xyz;
+    DevBuf<int> tmp_Z, tmp_bid, tmp_zid;
+    DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
+    DevBuf<unsigned char> cub_tmp;
+    int row_var = 4, col_var = 7;  // kernel variants (see row_pass / col_pass); pipelined by default
+    hipGraphExec_t stem_exec = nullptr;
+    std::vector<unsigned char> stem_key;
+    DevBuf<long long> dev_offset;
+    int num_cus = 256;
+    int det_tiles = 0;             // column tiles of the last detector launch
+    // profiling
+    bool prof = false;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[PK_COUNT] = {0};
+    int prof_n[PK_COUNT] = {0};
+    // adjoint slice loop: checkpoints of the last checkpointed forward run
+    DevBuf<float2> adj_ckpt, adj_exit, adj_seg, adj_g, adj_gt;
+    DevBuf<float> adj_phase;
+    DevBuf<double> adj_w, adj_grad;
+    int adj_cfg = -1, adj_k = 0, adj_nw = 0, adj_nslices = 0;
+    unsigned adj_version = 0;
+    bool adj_ready = false;
+};
+
+namespace mse_rt {
+
+inline hipEvent_t take_event(mse_engine* e) {
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev;
+    HIPX(hipEventCreate(&ev));
+    return ev;
+}
+
+template <class F>
+inline void launch(mse_engine* e, int kind, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e->prof) {
+        a = take_event(e);
+        b = take_event(e);
+        HIPX(hipEventRecord(a, e->stream));
+    }
+    f();
+    HIPX(hipGetLastError());
+    if (e->prof) {
+        HIPX(hipEventRecord(b, e->stream));
+        e->pending.emplace_back(kind, a, b);
+    }
+}
+
+// ---- helpers defined in engine.hip ---------------------------------------------------------
+// potential-kernel arguments for a configuration's geometry
+mse::PotArgs pot_args(mse_engine* e, const mse_geometry& g);
+const mse_geometry& cfg_geom(mse_engine* e, const ConfigData& c);
+ConfigData& config(mse_engine* e, int c);
+void require_geom(mse_engine* e);
+void check_waves(mse_engine* e);
+void check_trans(mse_engine* e);
+int run_guarded(const char* fn, mse_engine* e, const std::function<void()>& body);
+mse::ColArgs base_col_args(mse_engine* e);
+mse::RowArgs base_row_args(mse_engine* e);
+void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets, size_t wave_stride,
+              const mse::RowArgs& args, int variant = -1);
+int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
+             const mse::ColArgs& args, int kind = PK_COL, int variant = -1);
+// one forward slice (row pass with t, column pass with the propagator) on nw waves in place
+void slice_step(mse_engine* e, float2* waves, int nw, int slice);
+
+// ---- sweeps ------------------------------------------------------------------------------
+// Kernel variants (chosen by mse_autotune or mse_set_kernel_variant):
+//   row    0: default plan, wave-level LDS sync   1: half lanes per line   2: block barriers
+//          3: twiddles read at use, >= 8 waves per SIMD
+//   column 0: default plan, 16 columns per block  1: half lanes per column
+//          2: 8 columns per block                 3: block barriers
+//          4: 16 columns, twiddles read at use, >= 8 waves per SIMD (2 blocks per CU)
+//          5: 8 columns, twiddles read at use, >= 8 waves per SIMD (4 blocks per CU)
+//          6: 32 columns (256-byte row runs), half lanes per column
+//   row    4: persistent lines with the next row prefetched into registers
+//   column 7: persistent blocks with the next tile prefetched into registers
+constexpr int kRowVariants = 5;
+constexpr int kColVariants = 8;
+
+// Sweep launchers, one translation unit per resolution (sweeps_<N>.hip).
+template <int N>
+void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets, size_t wave_stride,
+                const mse::RowArgs& args, int variant);
+template <int N>
+int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
+               const mse::ColArgs& args, int kind, int variant);
+
+}  // namespace mse_rt

@@ -1,0 +1,378 @@
+// Host side of the engine in the shape of clTEM's simulation classes (see cltem_host.hpp).
+#include "cltem_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace cltem_amd {
+
+void check(int rc) {
+    if (rc != MSE_OK) throw std::runtime_error(mse_last_error());
+}
+
+void AtomList::limits(double* lo, double* hi) const {
+    // CrystalStructure::updateLimits (structure/crystalstructure.cpp:238-255)
+    lo[0] = lo[1] = lo[2] = 1e300;
+    hi[0] = hi[1] = hi[2] = -1e300;
+    for (size_t i = 0; i < x.size(); ++i) {
+        lo[0] = std::min(lo[0], x[i]);
+        lo[1] = std::min(lo[1], y[i]);
+        lo[2] = std::min(lo[2], z[i]);
+        hi[0] = std::max(hi[0], x[i]);
+        hi[1] = std::max(hi[1], y[i]);
+        hi[2] = std::max(hi[2], z[i]);
+    }
+}
+
+static const char* kSymbols[] = {
+    "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl",
+    "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se",
+    "Br", "Kr", "Rb", "Sr", "Y",  "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb",
+    "Te", "I",  "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er",
+    "Tm", "Yb", "Lu", "Hf", "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At",
+    "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U",  "Np", "Pu", "Am", "Cm", "Bk", "Cf", "Es", "Fm", "Md", "No",
+    "Lr"};
+
+int element_number(const std::string& symbol) {
+    for (int i = 0; i < 103; ++i)
+        if (symbol == kSymbols[i]) return i + 1;
+    throw std::runtime_error("unknown element symbol " + symbol);
+}
+
+AtomList read_xyz(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("Error opening .xyz file.");
+    std::string line;
+    std::getline(in, line);
+    size_t count = std::stoul(line);
+    std::getline(in, line);
+    double scale = 1.0;
+    {
+        std::istringstream hs(line);
+        std::string tok;
+        while (hs >> tok)
+            if (tok == "nm") scale = 10.0;
+    }
+    AtomList a;
+    while (std::getline(in, line)) {
+        if (line.empty() || line == "\r") break;
+        std::istringstream ls(line);
+        std::string sym, sx, sy, sz;
+        if (!(ls >> sym >> sx >> sy >> sz))
+            throw std::runtime_error(".xyz file columns are fewer than header entries. line: " +
+                                     std::to_string(3 + a.size()));
+        a.Z.push_back(element_number(sym));
+        a.x.push_back((double)std::stof(sx) * scale);
+        a.y.push_back((double)std::stof(sy) * scale);
+        a.z.push_back((double)std::stof(sz) * scale);
+    }
+    if (a.size() != count)
+        throw std::runtime_error("Number of atoms does not match .xyz first line: " + std::to_string(a.size()) +
+                                 " instead of " + std::to_string(count));
+    return a;
+}
+
+AtomList srtio3(int nx, int ny, int nz, double a) {
+    const struct { int Z; double x, y, z; } basis[5] = {
+        {38, 0.0, 0.0, 0.0}, {22, 0.5, 0.5, 0.5}, {8, 0.5, 0.5, 0.0}, {8, 0.5, 0.0, 0.5}, {8, 0.0, 0.5, 0.5}};
+    AtomList out;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                for (const auto& b : basis) {
+                    out.Z.push_back(b.Z);
+                    out.x.push_back((double)(float)((i + b.x) * a));
+                    out.y.push_back((double)(float)((j + b.y) * a));
+                    out.z.push_back((double)(float)((k + b.z) * a));
+                }
+    return out;
+}
+
+// Philox4x32-10 (Salmon et al., SC11)
+void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+AtomList displaced(const AtomList& a, double u_rms, uint64_t seed, int config) {
+    AtomList o = a;
+    const uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
+    for (size_t i = 0; i < a.size(); ++i) {
+        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)config, 0u, 0u};
+        uint32_t r[4];
+        philox4x32_10(ctr, key, r);
+        double u[4];
+        for (int k = 0; k < 4; ++k) u[k] = ((double)r[k] + 0.5) / 4294967296.0;
+        const double rad0 = std::sqrt(-2.0 * std::log(u[0]));
+        const double rad1 = std::sqrt(-2.0 * std::log(u[2]));
+        const double n0 = rad0 * std::cos(2.0 * M_PI * u[1]);
+        const double n1 = rad0 * std::sin(2.0 * M_PI * u[1]);
+        const double n2 = rad1 * std::cos(2.0 * M_PI * u[3]);
+        o.x[i] = (double)(float)(a.x[i] + u_rms * n0);
+        o.y[i] = (double)(float)(a.y[i] + u_rms * n1);
+        o.z[i] = (double)(float)(a.z[i] + u_rms * n2);
+    }
+    return o;
+}
+
+// ---- SimulationManager ------------------------------------------------------------------
+void SimulationManager::updateImages(std::map<std::string, Image>& ims, int jobCount) {
+    std::lock_guard<std::mutex> lck(image_update_mutex);
+    for (auto& kv : ims) {
+        auto it = images.find(kv.first);
+        if (it == images.end()) {
+            images[kv.first] = kv.second;
+            continue;
+        }
+        Image& cur = it->second;
+        const Image& im = kv.second;
+        if (im.slices.size() != cur.slices.size() ||
+            (!im.slices.empty() && im.slices[0].size() != cur.slices[0].size()))
+            throw std::runtime_error("Tried to merge simulation jobs with different output size");
+        if (im.weighting.size() != cur.weighting.size())
+            throw std::runtime_error("Tried to merge simulation jobs with different weighting sizes");
+        for (size_t j = 0; j < cur.slices.size(); ++j)
+            for (size_t k = 0; k < cur.slices[j].size(); ++k) cur.slices[j][k] += im.slices[j][k];
+        for (size_t k = 0; k < cur.weighting.size(); ++k) cur.weighting[k] += im.weighting[k];
+    }
+    complete_jobs += jobCount;
+}
+
+std::map<std::string, Image> SimulationManager::averaged() const {
+    std::map<std::string, Image> out = images;
+    for (auto& kv : out) {
+        Image& im = kv.second;
+        for (auto& sl : im.slices) {
+            const size_t per = im.weighting.empty() ? 1 : sl.size() / im.weighting.size();
+            for (size_t k = 0; k < sl.size(); ++k) {
+                const double w = im.weighting.empty() ? 1.0 : im.weighting[k / per];
+                sl[k] = w > 0 ? sl[k] / w : 0.0;
+            }
+        }
+    }
+    return out;
+}
+
+mse_geometry SimulationManager::geometry() const {
+    mse_geometry g;
+    check(mse_plan_geometry(&scenario, &g));
+    return g;
+}
+
+// ---- SimulationGeneral --------------------------------------------------------------------
+SimulationGeneral::SimulationGeneral(int device) { check(mse_create(device, &eng)); }
+
+SimulationGeneral::~SimulationGeneral() {
+    if (eng) mse_destroy(eng);
+}
+
+void SimulationGeneral::setJob(std::shared_ptr<SimulationManager> m, int c, std::vector<int> px) {
+    sm = std::move(m);
+    config = c;
+    pixels = std::move(px);
+}
+
+static void load_params(const std::string& path, int* form, int* per, std::vector<double>* values) {
+    // utilities/stringutils.cpp:149-184 file format
+    std::ifstream in(path);
+    if (in.fail()) throw std::runtime_error("Error opening resource file: " + path);
+    std::string name, form_name, tmp;
+    std::getline(in, name);
+    std::getline(in, form_name);
+    std::getline(in, tmp);
+    *per = std::stoi(tmp);
+    double p;
+    values->clear();
+    while (in >> p) values->push_back(p);
+    auto trim = [](std::string s) {
+        while (!s.empty() && (s.back() == '\r' || s.back() == ' ')) s.pop_back();
+        return s;
+    };
+    form_name = trim(form_name);
+    if (form_name == "kirkland")
+        *form = 0;
+    else if (form_name == "peng")
+        *form = 1;
+    else if (form_name == "lobato")
+        *form = 2;
+    else
+        throw std::runtime_error("Unknown parameterisation form " + form_name + " in file " + path);
+}
+
+void SimulationGeneral::sortAtoms() {
+    const bool phonon = sm->phonon_configs > 1 || sm->u_rms > 0.0;
+    const AtomList a = phonon ? displaced(sm->atoms, sm->u_rms, sm->seed, config) : sm->atoms;
+    check(mse_sort_atoms(eng, 0, a.x.data(), a.y.data(), a.z.data(), a.Z.data(), (int)a.size(), nullptr, nullptr,
+                         nullptr, nullptr));
+}
+
+bool SimulationGeneral::initialiseSimulation() {
+    geom = sm->geometry();
+    check(mse_set_geometry(eng, &geom));
+    int form = 0, per = 0;
+    std::vector<double> values;
+    load_params(sm->params_file, &form, &per, &values);
+    check(mse_set_parameterisation(eng, form, per, values.data(), (int)values.size()));
+    sortAtoms();
+    check(mse_build_transmission(eng, 0, 0, geom.n_slices));
+    return true;
+}
+
+void SimulationGeneral::doMultiSliceStep(int slice) { check(mse_multislice(eng, slice, 1)); }
+
+std::vector<double> SimulationGeneral::getDiffractionImage(int parallel_ind) {
+    std::vector<double> out((size_t)geom.resolution * geom.resolution);
+    check(mse_diffraction(eng, parallel_ind, out.data()));
+    return out;
+}
+
+std::vector<double> SimulationGeneral::getExitWaveImage(unsigned t, unsigned l, unsigned b, unsigned r) {
+    const unsigned res = (unsigned)geom.resolution;
+    std::vector<double> full(2 * (size_t)res * res);
+    check(mse_exit_wave(eng, 0, full.data()));
+    // cropping of simulationgeneral.cpp:816-826
+    std::vector<double> out(2 * (size_t)(res - t - b) * (res - l - r));
+    size_t cnt = 0;
+    for (unsigned j = 0; j < res; ++j)
+        if (j >= b && j < res - t)
+            for (unsigned i = 0; i < res; ++i)
+                if (i >= l && i < res - r) {
+                    const size_t k = i + (size_t)j * res;
+                    out[cnt] = full[2 * k];
+                    out[cnt + 1] = full[2 * k + 1];
+                    cnt += 2;
+                }
+    return out;
+}
+
+// ---- CTEM -------------------------------------------------------------------------------
+bool SimulationCtem::initialiseSimulation() {
+    SimulationGeneral::initialiseSimulation();
+    check(mse_set_wave_count(eng, 1));
+    const int c0 = 0;
+    check(mse_set_wave_configs(eng, &c0, 1));
+    check(mse_init_plane_wave(eng, 1.0));  // simulationctem.cpp:77-82
+    n_waves = 1;
+    return true;
+}
+
+void SimulationCtem::simulate() {
+    initialiseSimulation();
+    for (int i = 0; i < geom.n_slices; ++i) doMultiSliceStep(i);
+    const int res = geom.resolution;
+    std::map<std::string, Image> ims;
+    Image ew(res, res, 1), diff(res, res, 1);
+    ew.slices[0] = getExitWaveImage();  // interleaved re/im (complex averaged, as clTEM)
+    std::fill(ew.weighting.begin(), ew.weighting.end(), 1.0);
+    diff.slices[0] = getDiffractionImage(0);
+    std::fill(diff.weighting.begin(), diff.weighting.end(), 1.0);
+    ims["EW"] = ew;
+    ims["Diff"] = diff;
+    sm->updateImages(ims, 1);
+}
+
+// ---- CBED -------------------------------------------------------------------------------
+void SimulationCbed::initialiseProbeWave(double posx, double posy, int n_parallel) {
+    if (n_parallel != 0) throw std::runtime_error("this adapter initialises probes from wave 0");
+    const double pos[2] = {posx, posy};
+    check(mse_init_probes(eng, pos, 1, &sm->probe));
+}
+
+void SimulationCbed::simulate() {
+    SimulationGeneral::initialiseSimulation();
+    check(mse_set_wave_count(eng, 1));
+    const int c0 = 0;
+    check(mse_set_wave_configs(eng, &c0, 1));
+    n_waves = 1;
+    // CbedPosition: the mode's area is the point (x, x, y, y)
+    initialiseProbeWave(sm->scenario.area[0], sm->scenario.area[2]);
+    for (int i = 0; i < geom.n_slices; ++i) doMultiSliceStep(i);
+    const int res = geom.resolution;
+    std::map<std::string, Image> ims;
+    Image diff(res, res, 1);
+    diff.slices[0] = getDiffractionImage(0);
+    std::fill(diff.weighting.begin(), diff.weighting.end(), 1.0);
+    ims["Diff"] = diff;
+    sm->updateImages(ims, 1);
+}
+
+// ---- STEM -------------------------------------------------------------------------------
+double SimulationStem::getStemPixel(double inner, double outer, double xc, double yc, int parallel_ind) {
+    if (parallel_ind < 0 || parallel_ind >= n_waves) throw std::runtime_error("parallel index out of range");
+    mse_detector d{inner, outer, xc, yc};
+    std::vector<double> out(n_waves);
+    check(mse_stem_detect(eng, &d, 1, sm->sqrt2 ? 1 : 0, out.data()));
+    return out[parallel_ind];
+}
+
+void SimulationStem::simulate() {
+    SimulationGeneral::initialiseSimulation();
+    const int nx = geom.stem_pixels[0], ny = geom.stem_pixels[1];
+    if (pixels.empty())
+        for (int p = 0; p < nx * ny; ++p) pixels.push_back(p);
+    // probe positions of simulationstem.cpp:156-187
+    std::vector<double> pos(2 * pixels.size());
+    for (size_t i = 0; i < pixels.size(); ++i) {
+        const int p = pixels[i];
+        pos[2 * i] = geom.stem_start[0] + (p % nx) * geom.stem_scale[0];
+        pos[2 * i + 1] = geom.stem_start[1] + (p / nx) * geom.stem_scale[1];
+    }
+    const int nd = (int)sm->detectors.size();
+    std::vector<mse_detector> dets(nd);
+    for (int d = 0; d < nd; ++d) dets[d] = sm->detectors[d].d;
+    std::vector<double> acc(pixels.size() * nd, 0.0);
+    const int batch = std::max(1, std::min(sm->parallel_pixels, (int)pixels.size()));
+    check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
+                         sm->sqrt2 ? 1 : 0, 1, acc.data()));
+    n_waves = batch;
+    std::map<std::string, Image> ims;
+    for (int d = 0; d < nd; ++d) {
+        Image im(nx, ny, 1);
+        for (size_t i = 0; i < pixels.size(); ++i) {
+            im.slices[0][pixels[i]] = acc[i * nd + d];
+            im.weighting[pixels[i]] = 1.0;
+        }
+        ims[sm->detectors[d].name] = im;
+    }
+    sm->updateImages(ims, 1);
+}
+
+void run_simulation(std::shared_ptr<SimulationManager> sm) {
+    const int configs = std::max(1, sm->phonon_configs);
+    std::unique_ptr<SimulationGeneral> worker;
+    if (sm->scenario.mode == 1)
+        worker.reset(new SimulationCtem(sm->device));
+    else if (sm->scenario.mode == 2)
+        worker.reset(new SimulationStem(sm->device));
+    else if (sm->scenario.mode == 3)
+        worker.reset(new SimulationCbed(sm->device));
+    else
+        throw std::runtime_error("unknown simulation mode");
+    for (int c = 0; c < configs; ++c) {
+        worker->setJob(sm, c);
+        worker->simulate();
+    }
+}
+
+}  // namespace cltem_amd

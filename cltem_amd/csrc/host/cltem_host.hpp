@@ -1,0 +1,148 @@
+// Host side of the engine in the shape of clTEM's simulation classes.
+//
+// clTEM's CTEM, CBED and STEM modes are written against SimulationGeneral<T>'s protected
+// methods (microscope/simulationgeneral.h:53-71), SimulationCbed::initialiseProbeWave
+// (microscope/simulationcbed.h:33) and SimulationStem::getStemPixel
+// (microscope/simulationstem.h:53-56), and merge results with
+// SimulationManager::updateImages (simulationmanager.cpp:274-350).  These classes keep those
+// names, argument meanings and the exception-on-failure behaviour of clError::Throw
+// (clwrapper/clerror.h:17-24), and implement them on the C ABI of include/cltem_amd.h.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/cltem_amd.h"
+
+namespace cltem_amd {
+
+// A failing engine call throws std::runtime_error with the engine's message.
+void check(int rc);
+
+struct AtomList {
+    std::vector<double> x, y, z;
+    std::vector<int> Z;
+    size_t size() const { return x.size(); }
+    void limits(double* lo, double* hi) const;
+};
+
+// Element symbol <-> atomic number (1..103), as Utils::ElementSymbolToNumber.
+int element_number(const std::string& symbol);
+
+// .xyz reader in clTEM's default-header form (structure/crystalstructure.cpp:44-188):
+// line 1 atom count, line 2 header/comment, then "A x y z" per atom; coordinates pass
+// through float like std::stof; a header containing "nm" scales by 10.
+AtomList read_xyz(const std::string& path);
+
+// SrTiO3 perovskite tiled nx*ny*nz along [001] (a = 3.905 A): Sr(0,0,0), Ti(1/2,1/2,1/2),
+// O(1/2,1/2,0), O(1/2,0,1/2), O(0,1/2,1/2); coordinates float32-exact.
+AtomList srtio3(int nx, int ny, int nz, double a = 3.905);
+
+// Philox4x32-10 frozen-phonon displacement: each axis + u_rms * N(0,1), counter
+// (atom, config, 0, 0), key (seed lo, seed hi), Box-Muller; results rounded to float32.
+AtomList displaced(const AtomList& a, double u_rms, uint64_t seed, int config);
+void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+// Image<double> with per-pixel weighting (utilities/commonstructs.h:31-199).
+struct Image {
+    int width = 0, height = 0;
+    std::vector<std::vector<double>> slices;
+    std::vector<double> weighting;
+    Image() = default;
+    Image(int w, int h, int depth) : width(w), height(h), slices(depth, std::vector<double>((size_t)w * h, 0.0)),
+                                     weighting((size_t)w * h, 0.0) {}
+};
+
+struct Detector {
+    std::string name;
+    mse_detector d;
+};
+
+// The inputs clTEM's SimulationManager holds for this path, and its result merging.
+struct SimulationManager {
+    mse_scenario scenario{};   // mode, resolution, voltage, slicing, padding, areas, tilt
+    std::string params_file;   // params/*.dat
+    mse_probe probe{};         // STEM/CBED condenser and aberrations
+    std::vector<Detector> detectors;
+    AtomList atoms;            // undisplaced structure
+    int parallel_pixels = 64;  // probes per batch
+    int phonon_configs = 1;    // frozen-phonon iterations (0 or 1 = no phonons)
+    double u_rms = 0.0;
+    uint64_t seed = 20261202;
+    bool sqrt2 = true;         // sum_reduction's sqrt(2) factor
+    int device = 0;
+
+    // SimulationManager::updateImages: FP64 sum of every job's images and weights
+    void updateImages(std::map<std::string, Image>& ims, int jobCount);
+    // result of a finished simulation: each image divided by its weighting
+    std::map<std::string, Image> averaged() const;
+    std::map<std::string, Image> images;
+    int complete_jobs = 0;
+
+    mse_geometry geometry() const;
+
+   private:
+    std::mutex image_update_mutex;
+};
+
+class SimulationGeneral {
+   public:
+    explicit SimulationGeneral(int device);
+    virtual ~SimulationGeneral();
+    SimulationGeneral(const SimulationGeneral&) = delete;
+    SimulationGeneral& operator=(const SimulationGeneral&) = delete;
+
+    // one job: a frozen-phonon configuration and (STEM) the raster pixels to simulate
+    void setJob(std::shared_ptr<SimulationManager> sm, int config, std::vector<int> pixels = {});
+    virtual void simulate() = 0;
+    mse_engine* engine() { return eng; }
+
+   protected:
+    void sortAtoms();
+    bool initialiseSimulation();
+    void doMultiSliceStep(int slice);
+    std::vector<double> getDiffractionImage(int parallel_ind);
+    std::vector<double> getExitWaveImage(unsigned t = 0, unsigned l = 0, unsigned b = 0, unsigned r = 0);
+
+    mse_engine* eng = nullptr;
+    std::shared_ptr<SimulationManager> sm;
+    int config = 0;
+    std::vector<int> pixels;
+    mse_geometry geom{};
+    int n_waves = 0;
+};
+
+class SimulationCtem : public SimulationGeneral {
+   public:
+    using SimulationGeneral::SimulationGeneral;
+    void simulate() override;  // simulationctem.cpp:285-400 without image formation
+   protected:
+    bool initialiseSimulation();
+};
+
+class SimulationCbed : public SimulationGeneral {
+   public:
+    using SimulationGeneral::SimulationGeneral;
+    void simulate() override;  // simulationcbed.cpp:122-210
+   protected:
+    void initialiseProbeWave(double posx, double posy, int n_parallel = 0);
+};
+
+class SimulationStem : public SimulationCbed {
+   public:
+    using SimulationCbed::SimulationCbed;
+    void simulate() override;  // simulationstem.cpp:143-278, batched through mse_stem_batch
+   protected:
+    double getStemPixel(double inner, double outer, double xc, double yc, int parallel_ind);
+};
+
+// Runs every job of a configured manager on one GPU (the role of SimulationRunner's
+// worker, threading/simulationrunner.cpp:52-131, for one device).
+void run_simulation(std::shared_ptr<SimulationManager> sm);
+
+}  // namespace cltem_amd

@@ -24,6 +24,9 @@ enum RowMode : int {
     ROW_FFT = 2,          // forward x-DFT (transmission band limit, step 1)
     ROW_IFFT = 3,         // inverse x-DFT (band limit step 3, exit-wave readback)
     ROW_COPY = 4,         // read and write the rows only (bandwidth reference)
+    ROW_IFFT_INTENSITY = 5,  // |inverse x-DFT|^2 into a double image per wave (exit intensity)
+    ROW_ADJ_SEED = 6,     // adjoint seed: g = 2 w psi for L = sum w |psi|^2, stored as res F_x g
+    ROW_ADJ = 7,          // adjoint row half: g_t = conj(psi_s) u, then res F_x(conj(t) u)
 };
 
 enum ColMode : int {
@@ -35,13 +38,19 @@ enum ColMode : int {
     COL_FFT_DIFF = 5,     // |fftshift Psi|^2 of a stored wave into a double image
     COL_PROP_STORE_DETECT = 6,  // last slice: propagate, reduce detectors and store
     COL_COPY = 7,         // move the tile through LDS only (bandwidth reference)
+    COL_PROP_CONJ = 8,    // adjoint column half: propagation by conj(P')
 };
 
 constexpr int kMaxDet = 8;
 
 struct RowArgs {
     const float2* const* cfg_trans;  // per configuration: base of its slices
+    const int* cfg_nslices;          // per configuration: its number of slices (t = 1 beyond)
     const int* wave_cfg;             // per wave: configuration
+    double* dout;                    // ROW_IFFT_INTENSITY: res*res doubles per wave
+    const double* weights;           // ROW_ADJ_SEED: dL/dI, res*res doubles per wave
+    const float2* aux_in;            // ROW_ADJ: forward state S_s = F_x psi_s (wave_stride per wave)
+    float2* aux_out;                 // ROW_ADJ: g_t = conj(psi_s) u in real space (wave_stride per wave)
     int slice;
     float plane_value;               // ROW_PLANE_T_FFT: psi amplitude times res
     float scale;                     // ROW_FFT / ROW_IFFT output scale
@@ -107,14 +116,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         for (int m = 0; m < E; ++m) out[base + t + L * m] = v[m];
         return;
     }
-    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_IFFT) {
+    if constexpr (MODE == ROW_ADJ_SEED || MODE == ROW_ADJ) {
+        // u = F_x^-1 v / res (real space)
+        fft_line<PL, true>(v, ln, t, tws, sync);
+        sync();
+        const float inv = 1.0f / N;
+        if constexpr (MODE == ROW_ADJ_SEED) {
+            const double* w = a.weights + (size_t)wave * N * N + (size_t)row * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const float f = 2.0f * (float)w[t + L * m] * inv;
+                v[m] = make_float2(v[m].x * f, v[m].y * f);  // g = 2 w psi
+            }
+        } else {
+            float2 ps[E];
+            const float2* sp = a.aux_in + (size_t)wave * wave_stride + (size_t)row * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) ps[m] = sp[t + L * m];
+            fft_line<PL, true>(ps, ln, t, tws, sync);
+            sync();
+            float2* gt = a.aux_out + (size_t)wave * wave_stride + (size_t)row * N;
+            const float2* tr = a.cfg_trans[a.wave_cfg[wave]] + (size_t)a.slice * N * N + (size_t)row * N;
+            const bool has_t = a.slice < a.cfg_nslices[a.wave_cfg[wave]];
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const float2 u = make_float2(v[m].x * inv, v[m].y * inv);
+                const float2 psi = make_float2(ps[m].x * inv, ps[m].y * inv);
+                gt[t + L * m] = cmulc(u, psi);  // conj(psi) u
+                const float2 tv = has_t ? tr[t + L * m] : make_float2(1.f, 0.f);
+                v[m] = cmulc(u, tv);  // conj(t) u
+            }
+        }
+        // stored adjoint state res F_x g
+        fft_line<PL, false>(v, ln, t, tws, sync);
+        const float rs = (float)N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) out[base + t + L * m] = make_float2(v[m].x * rs, v[m].y * rs);
+        return;
+    }
+    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_IFFT || MODE == ROW_IFFT_INTENSITY) {
         fft_line<PL, true>(v, ln, t, tws, sync);
         sync();
     }
-    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_PLANE_T_FFT) {
-        const float2* tr = a.cfg_trans[a.wave_cfg[wave]] + (size_t)a.slice * N * N + (size_t)row * N;
+    if constexpr (MODE == ROW_IFFT_INTENSITY) {
+        double* d = a.dout + (size_t)wave * N * N + (size_t)row * N;
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[t + L * m]);
+        for (int m = 0; m < E; ++m) {
+            const double re = (double)v[m].x * a.scale, im = (double)v[m].y * a.scale;
+            d[t + L * m] = re * re + im * im;
+        }
+        return;
+    }
+    if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_PLANE_T_FFT) {
+        const int cfg = a.wave_cfg[wave];
+        if (a.slice < a.cfg_nslices[cfg]) {  // below a thinner specimen the slice is vacuum
+            const float2* tr = a.cfg_trans[cfg] + (size_t)a.slice * N * N + (size_t)row * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[t + L * m]);
+        }
     }
     if constexpr (MODE != ROW_IFFT) {
         fft_line<PL, false>(v, ln, t, tws, sync);
@@ -246,7 +305,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     }
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_DETECT || MODE == COL_PROP_STORE_DETECT ||
-                  MODE == COL_BAND) {
+                  MODE == COL_BAND || MODE == COL_PROP_CONJ) {
         const int xs = (x < N / 2) ? x : N - x;  // |signed kx index|
         float2 pxv = make_float2(a.scale, 0.f);
         if constexpr (MODE != COL_BAND) pxv = a.px[x];
@@ -256,6 +315,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
             const bool keep = xs <= a.lim[y];
             float2 f = pxv;
             if constexpr (MODE != COL_BAND) f = cmul(pxv, a.py[y]);
+            if constexpr (MODE == COL_PROP_CONJ) f.y = -f.y;
             v[m] = keep ? cmul(v[m], f) : make_float2(0.f, 0.f);
         }
     } else if constexpr (MODE == COL_FFT_DETECT || MODE == COL_FFT_DIFF) {
@@ -303,7 +363,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     }
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
-                  MODE == COL_PROBE) {
+                  MODE == COL_PROBE || MODE == COL_PROP_CONJ) {
         if constexpr (MODE != COL_PROBE) sync();  // the forward pass's last reads precede new writes
         fft_line<PL, true>(v, ln, t, tws, sync);
         sync();
@@ -433,11 +493,16 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
     auto fetch = [&](int item) {
         const int wv = item / N, row = item % N;
         const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
-        const float2* tr = a.cfg_trans[a.wave_cfg[wv]] + (size_t)a.slice * N * N + (size_t)row * N;
+        const int cfg = a.wave_cfg[wv];
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            qv[m] = src[t + L * m];
-            qt[m] = tr[t + L * m];
+        for (int m = 0; m < E; ++m) qv[m] = src[t + L * m];
+        if (a.slice < a.cfg_nslices[cfg]) {
+            const float2* tr = a.cfg_trans[cfg] + (size_t)a.slice * N * N + (size_t)row * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) qt[m] = tr[t + L * m];
+        } else {  // vacuum below a thinner specimen
+#pragma unroll
+            for (int m = 0; m < E; ++m) qt[m] = make_float2(1.f, 0.f);
         }
     };
     int item = blockIdx.x * LINES + line;
@@ -466,7 +531,7 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
 // sum per-tile partials in fixed order: out[w*ndet + d] (+)= sum_tiles; when dev_offset is
 // given the output starts dev_offset[0] entries further (read on the device, so a captured
 // graph can be replayed for successive probe batches)
-__global__ void k_reduce_partials(const double* __restrict__ partials, int ntiles, int nwd,
+static __global__ void k_reduce_partials(const double* __restrict__ partials, int ntiles, int nwd,
                                   double* __restrict__ out, int accumulate, double factor,
                                   const long long* __restrict__ dev_offset) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -476,6 +541,27 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int ntile
     s *= factor;
     double* o = out + (dev_offset ? dev_offset[0] : 0LL);
     o[i] = accumulate ? o[i] + s : s;
+}
+
+// out[p] (+)= sum over waves of img[w][p], in wave order
+static __global__ void k_sum_waves(const double* __restrict__ img, int nw, size_t npix, double* __restrict__ out,
+                                   int accumulate) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    double s = accumulate ? out[p] : 0.0;
+    for (int w = 0; w < nw; ++w) s += img[(size_t)w * npix + p];
+    out[p] = s;
+}
+
+// out[g][p] (+)= sum of img[w][p] over the waves w of group g, in wave order
+static __global__ void k_sum_groups(const double* __restrict__ img, int nw, size_t npix,
+                                    const int* __restrict__ group, int ngroups, double* __restrict__ out,
+                                    int accumulate) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    if (!accumulate)
+        for (int g = 0; g < ngroups; ++g) out[(size_t)g * npix + p] = 0.0;
+    for (int w = 0; w < nw; ++w) out[(size_t)group[w] * npix + p] += img[(size_t)w * npix + p];
 }
 
 }  // namespace mse

@@ -67,3 +67,21 @@ def stem_positions(geom, pixels=None):
     x = geom.stem_start[0] + xi * geom.stem_scale[0]
     y = geom.stem_start[1] + yi * geom.stem_scale[1]
     return np.stack([x, y], axis=1)
+
+
+def adjoint_auc(n_atoms=2000, box=(40.0, 40.0, 40.0), resolution=256, slice_thickness=1.0,
+                voltage_kv=200.0, min_sep=1.0, seed=PHONON_SEED, noise=0.05, params="kirkland"):
+    """Adjoint (position-gradient) run: random Au/C (1:3) in a box, projected potential,
+    plane-wave CTEM in 1 A slices (2000 atoms, 40^3 A, 256^2, 200 kV: 40 slices with no
+    z padding).  `target` holds positions displaced by Gaussian noise (sigma = noise A)
+    from which the target intensity is simulated."""
+    n_au = n_atoms // 4
+    spec = sp.random_box({"Au": n_au, "C": n_atoms - n_au}, box, min_sep, seed)
+    lim = _limits(spec)
+    scen = dict(mode=1, resolution=resolution, voltage_kv=voltage_kv,
+                slice_thickness=slice_thickness, slice_offset=0.0, pad_z=(0.0, 0.0),
+                area=(lim["struct_min"][0], lim["struct_max"][0], lim["struct_min"][1],
+                      lim["struct_max"][1]), **lim)
+    tgt = spec.displaced(noise, seed ^ 0x5A5A, 0)
+    return dict(name=f"adjoint_auc_{n_atoms}_{resolution}", scenario=scen, specimen=spec,
+                params=params, target=tgt)

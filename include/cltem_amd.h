@@ -133,6 +133,13 @@ int mse_build_transmission(mse_engine* e, int config, int first, int count);
 /* read one transmission function back as interleaved float (2N) */
 int mse_read_transmission(mse_engine* e, int config, int slice, float* out);
 
+/* ---- a configuration with its own slicing: a specimen of another thickness sharing the
+ *      engine's grid, field, blocks, slice thickness and optics (only z_lims, pad_z, n_slices,
+ *      pre_slices may differ; n_slices may not exceed the engine geometry's).  Its slices
+ *      beyond n_slices are vacuum (t = 1) in the slice loop.  Replaces re-initialising
+ *      SimulationGeneral for every specimen (simulationgeneral.cpp:345-391). ---- */
+int mse_set_config_geometry(mse_engine* e, int config, const mse_geometry* g);
+
 /* ---- waves: clWaveFunctionReal/Recip[parallel_pixels] (simulationgeneral.h:85-86) ---- */
 int mse_set_wave_count(mse_engine* e, int n_waves);
 /* wave -> frozen-phonon configuration whose transmission functions it uses (default 0) */
@@ -142,6 +149,8 @@ int mse_init_plane_wave(mse_engine* e, double value);
 /* SimulationCbed::initialiseProbeWave for waves 0..n-1 at absolute positions pos_xy[2*i]
  * (Angstrom), made relative to the padded simulation start (simulationcbed.cpp:54-59) */
 int mse_init_probes(mse_engine* e, const double* pos_xy, int n, const mse_probe* probe);
+/* the same for waves first_wave..first_wave+n-1 (probes of different convergence per wave) */
+int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, int n, const mse_probe* probe);
 /* SimulationGeneral::doMultiSliceStep for slices [first, first+count) on every wave
  * (microscope/simulationgeneral.cpp:682-769), batched over the waves */
 int mse_multislice(mse_engine* e, int first, int count);
@@ -152,6 +161,13 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
 int mse_diffraction(mse_engine* e, int wave, double* out);
 /* getExitWaveImage (simulationgeneral.cpp:808-829) without cropping: interleaved re/im, 2N doubles */
 int mse_exit_wave(mse_engine* e, int wave, double* out);
+/* sum over all waves of |psi_exit|^2 into N doubles (out += when accumulate): the phonon
+ * average of the exit-wave intensity as SimulationManager::updateImages sums jobs in FP64
+ * (simulationmanager.cpp:274-350) */
+int mse_exit_intensity_sum(mse_engine* e, int accumulate, double* out);
+/* out[g*N + p] (+)= sum over waves w with wave_group[w] == g of |fftshift Psi_w(k)|^2 (FP64):
+ * CBED diffraction patterns accumulated over frozen-phonon configurations */
+int mse_diffraction_sum(mse_engine* e, const int* wave_group, int n_groups, int accumulate, double* out);
 
 /* ---- batched STEM hot path: SimulationStem::simulate (microscope/simulationstem.cpp:143-278)
  *      for n_probes positions of one configuration, in batches of `batch` waves, with the
@@ -171,6 +187,25 @@ int mse_set_kernel_variant(mse_engine* e, int kernel, int variant);
 int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_us, double* col_us,
                  int* best_row, int* best_col);
 
+/* ---- adjoint (position gradients).  No clTEM counterpart: clTEM only runs forward.  The
+ *      forward differentiated is doMultiSliceStep over all slices
+ *      (microscope/simulationgeneral.cpp:682-769) from the entrance waves, with the
+ *      projected transmission functions of initialiseSimulation
+ *      (simulationgeneral.cpp:345-646).  ---- */
+/* From the current entrance waves (all referring to configuration cfg), run every slice of
+ * cfg, keeping ceil(n/K) checkpoints of the waves, K = ceil(sqrt(n_slices)).  Leaves the
+ * exit waves in the engine like mse_multislice; when intensity is not NULL writes
+ * |psi_exit|^2 as n_waves * N doubles (wave-major, rows of res). */
+int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
+/* Gradient of a loss L(I) of the exit intensities of the last mse_forward_checkpointed run:
+ * dl_di = dL/dI (n_waves * N doubles, same layout as the intensities; for
+ * L = sum (I - I_target)^2 it is 2 (I - I_target)).  Recomputes each segment of K slices
+ * from its checkpoint and runs the adjoint slice loop backwards.  grad_xyz receives
+ * dL/dx, dL/dy, dL/dz for every atom of the last mse_sort_atoms call on cfg, in input order
+ * (3 doubles per atom; atoms outside the simulated volume get 0).  Projected potentials
+ * only (MSE_ERR_UNSUPPORTED for full 3D); there dL/dz = 0 because an atom's potential does
+ * not depend on its depth within its slice. */
+int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz);
 /* ---- bandwidth reference: copy n_waves waves in place with the access pattern of the row
  *      sweep (pattern 0) or of the column sweep's LDS tiles (pattern 1, column variant
  *      `variant`), no arithmetic; average microseconds per launch.  Used to calibrate HBM

@@ -1020,3 +1020,148 @@ int orc_run_stem_f32(const OrcGeometry* g, const double* trans, const double* P,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Loss of a CTEM plane-wave exit intensity and its central finite differences with respect
+// to atom coordinates: the checker of the adjoint slice loop.  The forward is the path
+// above (sortAtoms, transmission with band limit, doMultiSliceStep over every slice from
+// init_plane_wave value 1).  A perturbed coordinate only changes the transmission of the
+// slices whose potential sums see the atom (its z bin, +-load_z in full 3D), so each
+// evaluation rebuilds those slices and reruns the slice loop.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+struct CtemProblem {
+    const OrcGeometry* g;
+    const double* params;
+    int form, i_count;
+    size_t N;
+    std::vector<std::complex<double>> P;
+};
+
+struct SortedAtoms {
+    std::vector<double> x, y, z;
+    std::vector<int> Z, bid, zid, bs;
+};
+
+SortedAtoms sort_all(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z, int n) {
+    SortedAtoms s;
+    const size_t na = std::max(n, 1);
+    s.x.resize(na); s.y.resize(na); s.z.resize(na); s.Z.resize(na); s.bid.resize(na); s.zid.resize(na);
+    s.bs.resize((size_t)g->n_slices * g->blocks_x * g->blocks_y + 1);
+    int nf = 0, nr = 0;
+    orc_sort_atoms(g, x, y, z, Z, n, s.x.data(), s.y.data(), s.z.data(), s.Z.data(), s.bid.data(),
+                   s.zid.data(), s.bs.data(), &nf, &nr);
+    return s;
+}
+
+void slice_transmission(const CtemProblem& pb, const SortedAtoms& s, int slice, std::complex<double>* t) {
+    orc_transmission(pb.g, s.x.data(), s.y.data(), s.z.data(), s.Z.data(), s.bs.data(), pb.params, pb.form,
+                     pb.i_count, slice, 1, 1, reinterpret_cast<double*>(t), 1);
+}
+
+// exit intensity through the slice stack; slice s uses over[s] when present
+void exit_intensity(const CtemProblem& pb, const std::vector<const std::complex<double>*>& stack,
+                    std::vector<double>& I) {
+    typedef std::complex<double> C;
+    const int res = pb.g->resolution;
+    Fft2<double> fft(res, res);
+    std::vector<C> psi(pb.N, C(1.0, 0.0)), recip(pb.N);
+    for (size_t s = 0; s < stack.size(); ++s) slice_step<double>(fft, pb.N, psi.data(), recip.data(), stack[s], pb.P.data());
+    I.resize(pb.N);
+    for (size_t i = 0; i < pb.N; ++i) I[i] = std::norm(psi[i]);
+}
+
+double loss_of(const std::vector<double>& I, const double* target) {
+    double L = 0.0;
+    for (size_t i = 0; i < I.size(); ++i) {
+        const double d = I[i] - target[i];
+        L += d * d;
+    }
+    return L;
+}
+
+int z_bin(const OrcGeometry* g, double x, double y, double z) {
+    if (!(x > g->full_x[0] && x < g->full_x[1] && y > g->full_y[0] && y < g->full_y[1] && z > g->z_lims[0] &&
+          z < g->z_lims[1]))
+        return -1;
+    int zid = (int)std::floor((g->z_lims[1] - z) * (1.0 / (g->slice_thickness)));
+    zid -= (zid == g->n_slices);
+    return zid;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_ctem_intensity(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z,
+                       int n, const double* params, int form, int i_count, double* intensity, int nthreads) {
+    typedef std::complex<double> C;
+    CtemProblem pb{g, params, form, i_count, (size_t)g->resolution * g->resolution, {}};
+    pb.P.resize(pb.N);
+    propagator<double>(g, pb.P.data());
+    SortedAtoms s = sort_all(g, x, y, z, Z, n);
+    std::vector<C> T((size_t)g->n_slices * pb.N);
+    orc_transmission(g, s.x.data(), s.y.data(), s.z.data(), s.Z.data(), s.bs.data(), params, form, i_count, 0,
+                     g->n_slices, 1, reinterpret_cast<double*>(T.data()), nthreads);
+    std::vector<const C*> stack(g->n_slices);
+    for (int k = 0; k < g->n_slices; ++k) stack[k] = T.data() + (size_t)k * pb.N;
+    std::vector<double> I;
+    exit_intensity(pb, stack, I);
+    std::memcpy(intensity, I.data(), pb.N * sizeof(double));
+    return 0;
+}
+
+int orc_ctem_loss_fd(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z, int n,
+                     const double* params, int form, int i_count, const double* target, const int* atom,
+                     const int* axis, int m, double h, double* grad, double* loss0, int nthreads) {
+    typedef std::complex<double> C;
+    for (int i = 0; i < m; ++i)
+        if (atom[i] < 0 || atom[i] >= n || axis[i] < 0 || axis[i] > 2) return -1;
+    CtemProblem pb{g, params, form, i_count, (size_t)g->resolution * g->resolution, {}};
+    pb.P.resize(pb.N);
+    propagator<double>(g, pb.P.data());
+    const SortedAtoms s0 = sort_all(g, x, y, z, Z, n);
+    std::vector<C> T((size_t)g->n_slices * pb.N);
+    orc_transmission(g, s0.x.data(), s0.y.data(), s0.z.data(), s0.Z.data(), s0.bs.data(), params, form, i_count,
+                     0, g->n_slices, 1, reinterpret_cast<double*>(T.data()), nthreads);
+    std::vector<const C*> base(g->n_slices);
+    for (int k = 0; k < g->n_slices; ++k) base[k] = T.data() + (size_t)k * pb.N;
+    if (loss0) {
+        std::vector<double> I;
+        exit_intensity(pb, base, I);
+        *loss0 = loss_of(I, target);
+    }
+    const int lz = g->full3d ? g->load_z : 0;
+    std::vector<double> Lv(2 * (size_t)m, 0.0);
+    parallel_for(2 * m, nthreads, [&](int e) {
+        const int i = e / 2;
+        const double sgn = (e % 2) ? -1.0 : 1.0;
+        std::vector<double> px(x, x + n), py(y, y + n), pz(z, z + n);
+        double* c = axis[i] == 0 ? px.data() : axis[i] == 1 ? py.data() : pz.data();
+        const int a = atom[i];
+        const int k_old = z_bin(g, x[a], y[a], z[a]);
+        c[a] += sgn * h;
+        const int k_new = z_bin(g, px[a], py[a], pz[a]);
+        const SortedAtoms sp = sort_all(g, px.data(), py.data(), pz.data(), Z, n);
+        std::vector<int> redo;
+        for (int k0 : {k_old, k_new}) {
+            if (k0 < 0) continue;
+            for (int k = std::max(0, k0 - lz); k <= std::min(g->n_slices - 1, k0 + lz); ++k)
+                if (std::find(redo.begin(), redo.end(), k) == redo.end()) redo.push_back(k);
+        }
+        std::vector<C> mod(redo.size() * pb.N);
+        std::vector<const C*> stack = base;
+        for (size_t r = 0; r < redo.size(); ++r) {
+            slice_transmission(pb, sp, redo[r], mod.data() + r * pb.N);
+            stack[redo[r]] = mod.data() + r * pb.N;
+        }
+        std::vector<double> I;
+        exit_intensity(pb, stack, I);
+        Lv[e] = loss_of(I, target);
+    });
+    for (int i = 0; i < m; ++i) grad[i] = (Lv[2 * i] - Lv[2 * i + 1]) / (2.0 * h);
+    return 0;
+}
+
+}  // extern "C"

@@ -20,6 +20,9 @@
  *   orc_propagator / band limit ..... exact masks and exp(-i pi lambda k^2 dz)
  *   orc_probe ....................... numpy chi(), aperture-disc pixel count
  *   orc_run_* ....................... empty specimen identities, numpy multislice
+ *   orc_ctem_intensity .............. orc_run_plane over orc_transmission of the sorted atoms
+ *   orc_ctem_loss_fd ................ brute-force central differences (every slice rebuilt,
+ *                                     perturbed atoms moved in numpy), O(h^2) convergence
  */
 #ifndef MSE_ORACLE_H
 #define MSE_ORACLE_H
@@ -130,6 +133,17 @@ int orc_run_stem(const OrcGeometry* g, const double* trans, const double* P, con
 int orc_run_stem_f32(const OrcGeometry* g, const double* trans, const double* P, const double* pos,
                      int n_probes, const double* aberr, double cond_ap_mrad, double smooth_mrad,
                      int taper, const double* det, int ndet, int sqrt2, double* out, int nthreads);
+
+/* CTEM plane wave (value 1) through the full path from unsorted atoms: exit intensity
+ * |psi|^2, N doubles. */
+int orc_ctem_intensity(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z,
+                       int n, const double* params, int form, int i_count, double* intensity, int nthreads);
+/* L = sum_p (|psi_exit(p)|^2 - target(p))^2 of the same forward; grad[i] = central difference
+ * (L(+h) - L(-h)) / 2h in coordinate axis[i] (0 x, 1 y, 2 z) of atom atom[i] (input order).
+ * loss0 (optional) receives L at the given positions. */
+int orc_ctem_loss_fd(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z, int n,
+                     const double* params, int form, int i_count, const double* target, const int* atom,
+                     const int* axis, int m, double h, double* grad, double* loss0, int nthreads);
 
 #ifdef __cplusplus
 }

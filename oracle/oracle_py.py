@@ -123,6 +123,10 @@ def lib():
                                      C.c_double, C.c_int]
         L.orc_diffraction.argtypes = [C.c_int, dp, dp]
         L.orc_run_plane.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.orc_ctem_intensity.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, C.c_int,
+                                         C.c_int, dp, C.c_int]
+        L.orc_ctem_loss_fd.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, C.c_int,
+                                       C.c_int, dp, ip, ip, C.c_int, C.c_double, dp, dp, C.c_int]
         for f in (L.orc_run_stem, L.orc_run_stem_f32):
             f.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, C.c_int, dp, C.c_double, C.c_double,
                           C.c_int, dp, C.c_int, C.c_int, dp, C.c_int]
@@ -235,6 +239,32 @@ def transmission(g, sorted_atoms, params, form, icount, first=0, count=None, ban
                            _ip(keep[4]), _dp(keep[5]), form, icount, first, count,
                            int(band_limit), _dp(out), nthreads or nthreads_default())
     return out.view(np.complex128).reshape(count, res, res)
+
+
+def ctem_intensity(g, x, y, z, Z, params, form, icount, nthreads=None):
+    """|psi_exit|^2 of a plane wave (value 1) through the whole path from unsorted atoms."""
+    res = g.resolution
+    x, y, z, Z, prm = _f64(x), _f64(y), _f64(z), _i32(Z), _f64(params)
+    out = np.zeros(res * res)
+    rc = lib().orc_ctem_intensity(C.byref(g), _dp(x), _dp(y), _dp(z), _ip(Z), len(x), _dp(prm), form,
+                                  icount, _dp(out), nthreads or nthreads_default())
+    assert rc == 0
+    return out.reshape(res, res)
+
+
+def ctem_loss_fd(g, x, y, z, Z, params, form, icount, target, atoms, axes, h=1e-3, nthreads=None):
+    """Central differences of L = sum (|psi_exit|^2 - target)^2 in coordinate axes[i]
+    (0 x, 1 y, 2 z) of atom atoms[i]; returns (grad, L at the given positions)."""
+    x, y, z, Z, prm = _f64(x), _f64(y), _f64(z), _i32(Z), _f64(params)
+    tgt = _f64(target).reshape(-1)
+    at, ax = _i32(atoms), _i32(axes)
+    grad = np.zeros(len(at))
+    L0 = C.c_double(0.0)
+    rc = lib().orc_ctem_loss_fd(C.byref(g), _dp(x), _dp(y), _dp(z), _ip(Z), len(x), _dp(prm), form, icount,
+                                _dp(tgt), _ip(at), _ip(ax), len(at), h, _dp(grad), C.byref(L0),
+                                nthreads or nthreads_default())
+    assert rc == 0
+    return grad, L0.value
 
 
 def fft2(a, inverse=False):

@@ -1,0 +1,129 @@
+"""The C++ host side (clTEM-shaped SimulationGeneral/Ctem/Cbed/Stem classes and the mse_run
+driver) against the Python input generators (CPU) and the FP64 oracle (GPU)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cltem_amd
+from cltem_amd import specimens as sp
+from cltem_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "cltem_amd", "mse_run")
+PARAMS = os.path.join(ROOT, "cltem_amd", "params")
+
+
+def have_driver():
+    return os.path.exists(RUN)
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cpp_phonon_displacements_match_python(tmp_path):
+    f = tmp_path / "atoms.bin"
+    for cfg in (0, 3):
+        subprocess.run([RUN, "--srtio3", "4,3,2", "--urms", "0.08", "--seed", "20261202", "--dump-atoms",
+                        str(f), "--dump-config", str(cfg)], check=True)
+        raw = f.read_bytes()
+        n = 4 * 3 * 2 * 5
+        xyz = np.frombuffer(raw[:24 * n], dtype=np.float64).reshape(3, n)
+        Z = np.frombuffer(raw[24 * n:], dtype=np.int32)
+        ref = sp.srtio3(4, 3, 2).displaced(0.08, 20261202, cfg)
+        np.testing.assert_array_equal(Z, ref.Z)
+        # libm and numpy may differ in the last ulp of log/cos before the float32 rounding
+        for got, exp in zip(xyz, (ref.x, ref.y, ref.z)):
+            assert np.max(np.abs(got - exp)) <= 1e-5
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_rejects_bad_input(tmp_path):
+    r = subprocess.run([RUN, "--mode", "ctem"], capture_output=True, text=True)
+    assert r.returncode == 1 and "no structure" in r.stderr
+    r = subprocess.run([RUN, "--srtio3", "1,1,1", "--bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown option" in r.stderr
+
+
+def _load(prefix):
+    meta = json.load(open(prefix + ".json"))
+    out = {}
+    for k, v in meta.items():
+        if isinstance(v, dict):
+            out[k] = np.fromfile(v["file"], dtype=np.float64)
+    return meta, out
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_ctem_matches_oracle(tmp_path, oracle):
+    prefix = str(tmp_path / "ctem")
+    subprocess.run([RUN, "--mode", "ctem", "--srtio3", "3,3,4", "--res", "256", "--dz", "1.9525",
+                    "--params", os.path.join(PARAMS, "kirkland.dat"), "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    w = wl.ctem_srtio3(3, 3, 4)
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    s = w["specimen"]
+    p = cltem_amd.load_params("kirkland")
+    srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
+    psi, recip = oracle.run_plane(oracle.transmission(og, srt, p.values, 0, 3), oracle.propagator(og))
+    ew = out["EW"].view(np.complex128).reshape(256, 256)
+    assert np.linalg.norm(ew - psi) / np.linalg.norm(psi) <= 1e-4
+    dif = out["Diff"].reshape(256, 256)
+    ref = oracle.diffraction(recip)
+    assert np.linalg.norm(dif - ref) / np.linalg.norm(ref) <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_stem_phonon_average_matches_oracle(tmp_path, oracle):
+    prefix = str(tmp_path / "stem")
+    x0, x1 = 2 * 3.905, 3 * 3.905
+    subprocess.run([RUN, "--mode", "stem", "--srtio3", "5,5,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{x0},{x1},{x0},{x1}", "--stem-pixels", "4", "--det", "ADF,60,200",
+                    "--det", "BF,0,15", "--alpha", "20", "--batch", "8", "--phonons", "2",
+                    "--urms", "0.08", "--seed", "99", "--params", os.path.join(PARAMS, "kirkland.dat"),
+                    "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    assert meta["jobs"] == 2
+    w = wl.stem_srtio3(nx=5, ny=5, nz=3, resolution=256, pixels=4, window_cells=(2, 3))
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    p = cltem_amd.load_params("kirkland")
+    pos = wl.stem_positions(og)
+    acc = np.zeros((16, 2))
+    for c in range(2):
+        s = w["specimen"].displaced(0.08, 99, c)
+        srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
+        tr = oracle.transmission(og, srt, p.values, 0, 3)
+        acc += oracle.run_stem(og, tr, oracle.propagator(og), pos, np.array([[60, 200, 0, 0], [0, 15, 0, 0.]]))
+    acc /= 2
+    for d, name in enumerate(("ADF", "BF")):
+        got = out[name]
+        assert np.max(np.abs(got - acc[:, d]) / np.abs(acc[:, d])) <= 1e-3, name
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_cbed_matches_oracle(tmp_path, oracle):
+    prefix = str(tmp_path / "cbed")
+    cx = 1.5 * 3.905
+    subprocess.run([RUN, "--mode", "cbed", "--srtio3", "3,3,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{cx},{cx},{cx},{cx}", "--alpha", "15",
+                    "--params", os.path.join(PARAMS, "kirkland.dat"), "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    s = sp.srtio3(3, 3, 3)
+    lo, hi = s.limits
+    scen = dict(mode=3, resolution=256, slice_thickness=1.9525, struct_min=tuple(lo), struct_max=tuple(hi),
+                area=(cx, cx, cx, cx))
+    og = oracle.geometry(oracle.scenario(**scen))
+    p = cltem_amd.load_params("kirkland")
+    srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
+    tr = oracle.transmission(og, srt, p.values, 0, 3)
+    k = oracle.probe_kspace(og, cx - og.sim_x[0], cx - og.sim_y[0], None, 15.0)
+    psi = oracle.fft2(k, inverse=True)
+    P_ = oracle.propagator(og)
+    for sl in range(og.n_slices):
+        psi, recip = oracle.slice_step(psi, tr[sl], P_)
+    ref = oracle.diffraction(recip)
+    got = out["Diff"].reshape(256, 256)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-3
