@@ -203,9 +203,12 @@ class Engine:
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
         _check(self.lib.mse_init_probes_range(self.h, first_wave, _dp(pos), len(pos), C.byref(probe)))
 
-    def multislice(self, first: int = 0, count: int | None = None):
+    def multislice(self, first: int = 0, count: int | None = None, graph: bool = False):
+        """Run slices [first, first + count) for every wave; graph=True replays the whole
+        range from one captured hipGraph."""
         count = self.g.n_slices - first if count is None else count
-        _check(self.lib.mse_multislice(self.h, first, count))
+        fn = self.lib.mse_multislice_graph if graph else self.lib.mse_multislice
+        _check(fn(self.h, first, count))
 
     def stem_detect(self, detectors, sqrt2: bool = True) -> np.ndarray:
         out = np.zeros((self.n_waves, len(detectors)))

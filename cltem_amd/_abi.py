@@ -132,6 +132,7 @@ SIGNATURES = [
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_exit_intensity_sum", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_diffraction_sum", C.c_int, [_eng, _ip, C.c_int, C.c_int, _dp]),
+    ("mse_multislice_graph", C.c_int, [_eng, C.c_int, C.c_int]),
     ("mse_forward_checkpointed", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_adjoint_positions", C.c_int, [_eng, _dp, _dp]),
     ("mse_stem_batch", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),

@@ -750,6 +750,68 @@ int mse_multislice(mse_engine* e, int first, int count) {
     });
 }
 
+// Everything a captured slice loop depends on: launch shapes, kernel variants and every
+// device pointer baked into its kernel arguments.
+struct LoopGraphKey {
+    int res, nw, first, count, row_var, col_var, num_cus;
+    const void *waves, *cfg_ptrs, *cfg_nslices, *wave_cfg, *tw, *px, *py, *lim;
+};
+
+int mse_multislice_graph(mse_engine* e, int first, int count) {
+    return run_guarded("mse_multislice_graph", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        check_waves(e);
+        check_trans(e);
+        REQUIRE(first >= 0 && count >= 0 && first + count <= e->g.n_slices, MSE_ERR_ARG, "slice range out of bounds");
+        if (count == 0) return;
+        if (e->prof) {  // per-kernel events need eager launches
+            multislice(e, e->n_waves, first, count, false, nullptr);
+            return;
+        }
+        LoopGraphKey k;
+        std::memset(&k, 0, sizeof(k));
+        k.res = e->res;
+        k.nw = e->n_waves;
+        k.first = first;
+        k.count = count;
+        k.row_var = e->row_var;
+        k.col_var = e->col_var;
+        k.num_cus = e->num_cus;
+        k.waves = e->waves.p;
+        k.cfg_ptrs = e->cfg_ptrs.p;
+        k.cfg_nslices = e->cfg_nslices.p;
+        k.wave_cfg = e->wave_cfg.p;
+        k.tw = e->tw.p;
+        k.px = e->px.p;
+        k.py = e->py.p;
+        k.lim = e->prop_lim.p;
+        std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
+        if (!e->loop_exec || key != e->loop_key) {
+            if (e->loop_exec) (void)hipGraphExecDestroy(e->loop_exec);
+            e->loop_exec = nullptr;
+            e->loop_key.clear();
+            hipGraph_t gr = nullptr;
+            HIPX(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                multislice(e, e->n_waves, first, count, false, nullptr);
+            } catch (...) {
+                (void)hipStreamEndCapture(e->stream, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            HIPX(hipStreamEndCapture(e->stream, &gr));
+            hipError_t st = hipGraphInstantiate(&e->loop_exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (st != hipSuccess) {
+                e->loop_exec = nullptr;
+                throw Error(MSE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(st));
+            }
+            e->loop_key = key;
+        }
+        HIPX(hipGraphLaunch(e->loop_exec, e->stream));
+    });
+}
+
 int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out) {
     return run_guarded("mse_stem_detect", e, [&]() {
         REQUIRE(e && dets && out, MSE_ERR_ARG, "NULL argument");
@@ -905,6 +967,9 @@ void destroy_stem_graph(mse_engine* e) {
     if (e->stem_exec) (void)hipGraphExecDestroy(e->stem_exec);
     e->stem_exec = nullptr;
     e->stem_key.clear();
+    if (e->loop_exec) (void)hipGraphExecDestroy(e->loop_exec);
+    e->loop_exec = nullptr;
+    e->loop_key.clear();
 }
 }  // namespace mse_rt
 

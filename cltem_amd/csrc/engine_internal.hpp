@@ -133,6 +133,8 @@ struct mse_engine {
     int row_var = 4, col_var = 7;  // kernel variants (see row_pass / col_pass); pipelined by default
     hipGraphExec_t stem_exec = nullptr;
     std::vector<unsigned char> stem_key;
+    hipGraphExec_t loop_exec = nullptr;   // captured slice loop of mse_multislice_graph
+    std::vector<unsigned char> loop_key;
     DevBuf<long long> dev_offset;
     int num_cus = 256;
     int det_tiles = 0;             // column tiles of the last detector launch

@@ -72,16 +72,36 @@ def stem_positions(geom, pixels=None):
 def adjoint_auc(n_atoms=2000, box=(40.0, 40.0, 40.0), resolution=256, slice_thickness=1.0,
                 voltage_kv=200.0, min_sep=1.0, seed=PHONON_SEED, noise=0.05, params="kirkland"):
     """Adjoint (position-gradient) run: random Au/C (1:3) in a box, projected potential,
-    plane-wave CTEM in 1 A slices (2000 atoms, 40^3 A, 256^2, 200 kV: 40 slices with no
-    z padding).  `target` holds positions displaced by Gaussian noise (sigma = noise A)
-    from which the target intensity is simulated."""
+    plane-wave CTEM in 1 A slices (2000 atoms, 40^3 A, 256^2, 200 kV: 40 slices of
+    specimen plus clTEM's default +-3 A z padding, 46 slices).  `target` holds positions
+    displaced by Gaussian noise (sigma = noise A) from which the target intensity is
+    simulated."""
     n_au = n_atoms // 4
     spec = sp.random_box({"Au": n_au, "C": n_atoms - n_au}, box, min_sep, seed)
     lim = _limits(spec)
     scen = dict(mode=1, resolution=resolution, voltage_kv=voltage_kv,
-                slice_thickness=slice_thickness, slice_offset=0.0, pad_z=(0.0, 0.0),
+                slice_thickness=slice_thickness, slice_offset=0.0,
                 area=(lim["struct_min"][0], lim["struct_max"][0], lim["struct_min"][1],
                       lim["struct_max"][1]), **lim)
     tgt = spec.displaced(noise, seed ^ 0x5A5A, 0)
     return dict(name=f"adjoint_auc_{n_atoms}_{resolution}", scenario=scen, specimen=spec,
                 params=params, target=tgt)
+
+
+def ctem_amorphous_sio2(n_atoms=20000, box=(60.0, 60.0, 100.0), resolution=1024, slice_thickness=1.0,
+                        voltage_kv=300.0, min_sep=1.5, n_configs=30, u_rms=0.1, integrals=20,
+                        seed=PHONON_SEED, params="lobato"):
+    """Full-3D CTEM on an amorphous SiO2-like cell: Si:O = 1:2 by random sequential addition
+    in a box with a minimum separation, plane wave, frozen-phonon configurations batched
+    (20000 atoms in 60x60x100 A, 1024^2, 1 A slices (106 with clTEM's z padding), 300 kV,
+    Lobato with 20 integrals, 30 configurations of u_rms 0.1 A).  The output is the
+    phonon-averaged exit-wave intensity."""
+    n_si = n_atoms // 3
+    spec = sp.random_box({"Si": n_si, "O": n_atoms - n_si}, box, min_sep, seed)
+    lim = _limits(spec)
+    scen = dict(mode=1, resolution=resolution, voltage_kv=voltage_kv,
+                slice_thickness=slice_thickness, slice_offset=0.0, full3d=1, full3d_integrals=integrals,
+                area=(lim["struct_min"][0], lim["struct_max"][0], lim["struct_min"][1],
+                      lim["struct_max"][1]), **lim)
+    return dict(name=f"ctem_sio2_{n_atoms}_{resolution}_{n_configs}cfg", scenario=scen, specimen=spec,
+                params=params, n_configs=n_configs, u_rms=u_rms, seed=seed)

@@ -154,6 +154,11 @@ int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, i
 /* SimulationGeneral::doMultiSliceStep for slices [first, first+count) on every wave
  * (microscope/simulationgeneral.cpp:682-769), batched over the waves */
 int mse_multislice(mse_engine* e, int first, int count);
+/* mse_multislice replayed from a captured hipGraph: the whole slice range for every wave
+ * (probes, phonon configurations, specimens of other thicknesses) is one graph launch.
+ * Captured on first use and again whenever the wave set, slice range, kernel variants or
+ * engine buffers change; results are identical to mse_multislice. */
+int mse_multislice_graph(mse_engine* e, int first, int count);
 /* SimulationStem::getStemPixel for every wave and detector (microscope/simulationstem.cpp:42-131):
  * out[wave*n_det + d] = (sqrt2 ? sqrt(2) : 1) * sum over the annulus of |Psi_shifted|^2, FP64 */
 int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out);
