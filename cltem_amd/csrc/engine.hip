@@ -675,6 +675,47 @@ void probes_into_waves(mse_engine* e, int n, const mse_probe* p) {
     col_pass(e, mse::COL_PROBE, e->waves.p, e->waves.p, n, e->N, ca, PK_PROBE);
 }
 
+// Waves whose configuration has fewer slices than the engine's geometry (thinner
+// specimens batched with thicker ones) stop at their own exit surface, as separate clTEM
+// runs would.  For each slice, the active waves are those whose configuration still has
+// that slice; when they are the first k waves (waves ordered by thickness, or all waves)
+// the passes run on k sets directly, otherwise on a device list of their indices.
+void active_waves(mse_engine* e, int nw) {
+    std::vector<int> key(e->h_wave_cfg.begin(), e->h_wave_cfg.begin() + nw);
+    std::vector<int> ns(e->configs.size());
+    for (size_t c = 0; c < e->configs.size(); ++c) ns[c] = cfg_geom(e, e->configs[c]).n_slices;
+    key.push_back(-1);
+    key.insert(key.end(), ns.begin(), ns.end());
+    key.push_back(e->g.n_slices);
+    if (key == e->map_key && !e->map_count.empty()) return;
+    const int S = e->g.n_slices;
+    e->map_count.assign(S, 0);
+    e->map_offset.assign(S, -1);
+    std::vector<int> table;
+    for (int s = 0; s < S; ++s) {
+        std::vector<int> act;
+        for (int w = 0; w < nw; ++w)
+            if (ns[e->h_wave_cfg[w]] > s) act.push_back(w);
+        e->map_count[s] = (int)act.size();
+        bool prefix = true;
+        for (int i = 0; i < (int)act.size(); ++i) prefix = prefix && act[i] == i;
+        if (!prefix) {
+            e->map_offset[s] = (int)table.size();
+            table.insert(table.end(), act.begin(), act.end());
+        }
+    }
+    if (!table.empty()) {
+        if (e->wave_map.n < table.size()) {
+            HIPX(hipStreamSynchronize(e->stream));
+            e->wave_map.ensure(table.size());
+        }
+        HIPX(hipMemcpyAsync(e->wave_map.p, table.data(), table.size() * sizeof(int), hipMemcpyHostToDevice,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    }
+    e->map_key = key;
+}
+
 void slice_step(mse_engine* e, float2* waves, int nw, int slice) {
     mse::RowArgs ra = base_row_args(e);
     ra.slice = slice;
@@ -683,15 +724,23 @@ void slice_step(mse_engine* e, float2* waves, int nw, int slice) {
 }
 
 void multislice(mse_engine* e, int nw, int first, int count, bool last_detect, const mse::ColArgs* det_args) {
+    active_waves(e, nw);
     mse::RowArgs ra = base_row_args(e);
     mse::ColArgs ca = base_col_args(e);
     for (int s = first; s < first + count; ++s) {
+        const int na = e->map_count[s];
+        const int* map = e->map_offset[s] >= 0 ? e->wave_map.p + e->map_offset[s] : nullptr;
+        if (last_detect && s == first + count - 1)
+            REQUIRE(na == nw && !map, MSE_ERR_STATE, "detector slice reached by finished waves");
+        if (na == 0) continue;
         ra.slice = s;
-        row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nw, e->N, ra);
+        ra.wave_map = map;
+        ca.wave_map = map;
+        row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, na, e->N, ra);
         if (last_detect && s == first + count - 1)
             e->det_tiles = col_pass(e, mse::COL_PROP_DETECT, e->waves.p, e->waves.p, nw, e->N, *det_args);
         else
-            col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nw, e->N, ca);
+            col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, na, e->N, ca);
     }
 }
 
@@ -786,6 +835,11 @@ int mse_multislice_graph(mse_engine* e, int first, int count) {
         k.py = e->py.p;
         k.lim = e->prop_lim.p;
         std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
+        active_waves(e, e->n_waves);  // the per-slice wave sets are baked into the graph
+        const void* mp = e->wave_map.p;
+        key.insert(key.end(), (const unsigned char*)&mp, (const unsigned char*)&mp + sizeof(mp));
+        key.insert(key.end(), (const unsigned char*)e->map_key.data(),
+                   (const unsigned char*)(e->map_key.data() + e->map_key.size()));
         if (!e->loop_exec || key != e->loop_key) {
             if (e->loop_exec) (void)hipGraphExecDestroy(e->loop_exec);
             e->loop_exec = nullptr;
@@ -921,6 +975,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.probe_pos = e->probe_pos.p;
             k.probe = *probe;
             std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
+            active_waves(e, batch);  // host-side plan (no stream work may happen inside the capture)
             if (!e->stem_exec || key != e->stem_key) {
                 destroy_stem_graph(e);
                 hipGraph_t gr = nullptr;

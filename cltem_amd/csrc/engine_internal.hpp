@@ -135,6 +135,10 @@ struct mse_engine {
     std::vector<unsigned char> stem_key;
     hipGraphExec_t loop_exec = nullptr;   // captured slice loop of mse_multislice_graph
     std::vector<unsigned char> loop_key;
+    // waves still inside their specimen, per slice (see active_waves in engine.hip)
+    DevBuf<int> wave_map;
+    std::vector<int> map_key;                 // wave configurations and slice counts it was built for
+    std::vector<int> map_count, map_offset;   // per slice: active waves, offset into wave_map (-1: prefix)
     DevBuf<long long> dev_offset;
     int num_cus = 256;
     int det_tiles = 0;             // column tiles of the last detector launch

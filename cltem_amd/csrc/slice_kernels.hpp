@@ -49,6 +49,7 @@ struct RowArgs {
     const int* wave_cfg;             // per wave: configuration
     double* dout;                    // ROW_IFFT_INTENSITY: res*res doubles per wave
     const double* weights;           // ROW_ADJ_SEED: dL/dI, res*res doubles per wave
+    const int* wave_map;             // set -> wave index (NULL: identity); skips finished waves
     const float2* aux_in;            // ROW_ADJ: forward state S_s = F_x psi_s (wave_stride per wave)
     float2* aux_out;                 // ROW_ADJ: g_t = conj(psi_s) u in real space (wave_stride per wave)
     int slice;
@@ -66,6 +67,7 @@ struct ProbeArgs {
 };
 
 struct ColArgs {
+    const int* wave_map;     // set -> wave index (NULL: identity); skips finished waves
     const float2* px;        // propagator factors (COL_PROP*)
     const float2* py;
     const int* lim;          // per ky row: largest |kx index| kept (mask), -1 for none
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int line = tid / L;
     const int t = tid % L;
     const int row = blockIdx.x * LINES + line;
-    const int wave = blockIdx.y;
+    const int wave = a.wave_map ? a.wave_map[blockIdx.y] : (int)blockIdx.y;
     float2* ln = lds[line];
 
     PlanTw<PL, TWREG> tws;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     const int t = tid % L;
     const int x0 = blockIdx.x * TC;
     const int x = x0 + c;
-    const int wave = blockIdx.y;
+    const int wave = a.wave_map ? a.wave_map[blockIdx.y] : (int)blockIdx.y;
     float2* ln = tile + c * CS;
     const size_t wbase = (size_t)wave * wave_stride;
 
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
 
     float4 q[NQ];
     auto fetch = [&](int item) {
-        const int wv = item / ntiles;
+        const int wv = a.wave_map ? a.wave_map[item / ntiles] : item / ntiles;
         const int xx = (item % ntiles) * TC;
         const float4* in4 = reinterpret_cast<const float4*>(data + (size_t)wv * wave_stride + xx);
 #pragma unroll
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
     int item = blockIdx.x;
     if (item < nitems) fetch(item);
     for (; item < nitems; item += gridDim.x) {
-        const int wv = item / ntiles;
+        const int wv = a.wave_map ? a.wave_map[item / ntiles] : item / ntiles;
         const int x0 = (item % ntiles) * TC;
         const int x = x0 + c;
 #pragma unroll
@@ -491,7 +493,7 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
 
     float2 qv[E], qt[E];
     auto fetch = [&](int item) {
-        const int wv = item / N, row = item % N;
+        const int wv = a.wave_map ? a.wave_map[item / N] : item / N, row = item % N;
         const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
         const int cfg = a.wave_cfg[wv];
 #pragma unroll
@@ -521,7 +523,7 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[m]);
         fft_line<PL, false>(v, ln, t, tws, sync);
         sync();
-        const int wv = item / N, row = item % N;
+        const int wv = a.wave_map ? a.wave_map[item / N] : item / N, row = item % N;
         float2* dst = data + (size_t)wv * wave_stride + (size_t)row * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) dst[t + L * m] = v[m];

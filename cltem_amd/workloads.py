@@ -105,3 +105,28 @@ def ctem_amorphous_sio2(n_atoms=20000, box=(60.0, 60.0, 100.0), resolution=1024,
                       lim["struct_max"][1]), **lim)
     return dict(name=f"ctem_sio2_{n_atoms}_{resolution}_{n_configs}cfg", scenario=scen, specimen=spec,
                 params=params, n_configs=n_configs, u_rms=u_rms, seed=seed)
+
+
+def mixed_cbed(n_instances=48, cells_xy=5, thickness_cells=(10, 80), sizes=(256, 384, 512, 768),
+               alpha_mrad=(5.0, 25.0), n_configs=5, u_rms=0.08, slice_thickness=SRTIO3_A / 2,
+               voltage_kv=200.0, seed=PHONON_SEED):
+    """Mixed-size batched CBED: independent SrTiO3 [001] instances (cells_xy^2 cells wide)
+    with seeded thickness U{10..80} cells, grid size from `sizes`, convergence U(5, 25)
+    mrad and n_configs frozen-phonon configurations each; the probe sits at the centre of
+    the specimen (48 instances, 200 kV, 5 configurations of u_rms 0.08 A)."""
+    from .mixed import CbedInstance
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_instances):
+        nz = int(rng.integers(thickness_cells[0], thickness_cells[1] + 1))
+        res = int(sizes[int(rng.integers(0, len(sizes)))])
+        alpha = float(rng.uniform(*alpha_mrad))
+        spec = sp.srtio3(cells_xy, cells_xy, nz, SRTIO3_A)
+        lo, hi = spec.limits
+        cx, cy = 0.5 * cells_xy * SRTIO3_A, 0.5 * cells_xy * SRTIO3_A
+        scen = dict(mode=3, resolution=res, voltage_kv=voltage_kv, slice_thickness=slice_thickness,
+                    slice_offset=0.0, struct_min=tuple(lo), struct_max=tuple(hi), area=(cx, cx, cy, cy),
+                    area_padding=0.5 * cells_xy * SRTIO3_A)
+        out.append(CbedInstance(specimen=spec, scenario=scen, alpha_mrad=alpha, position=(cx, cy),
+                                n_configs=n_configs, u_rms=u_rms, seed=seed + i))
+    return out

@@ -98,3 +98,147 @@ def test_cbed_mixed_thickness_and_convergence(engine, oracle):
         err = rel_l2(got[i], ref)
         print(f"instance {i}: {og.n_slices} of {gmax.n_slices} slices, alpha {alphas[i]} mrad: rel-L2 {err:.3e}")
         assert err <= 1e-3
+
+
+def test_graph_slice_loop_replays_match_eager(engine, oracle):
+    """mse_multislice_graph: every replay (with eager work and synchronisation between the
+    replays) reproduces the eager slice loop bit for bit, for waves of several phonon
+    configurations and thicknesses."""
+    w = wl.ctem_srtio3(3, 3, 6, resolution=256)
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    p = cltem_amd.load_params(w["params"])
+    engine.set_geometry(g)
+    engine.set_parameterisation(p)
+    for c in range(3):
+        s = w["specimen"].displaced(0.08, 17, c)
+        engine.sort_atoms(c, s.x, s.y, s.z, s.Z)
+        engine.build_transmission(c)
+    engine.set_wave_count(6)
+    engine.set_wave_configs([0, 1, 2, 0, 1, 2])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    ref = np.stack([engine.exit_wave(i) for i in range(6)])
+    for rep in range(3):
+        engine.init_plane_wave(1.0)
+        engine.multislice(graph=True)
+        got = np.stack([engine.exit_wave(i) for i in range(6)])
+        assert np.array_equal(got, ref), f"replay {rep}"
+        # eager work between replays: a different slice range, then the full loop again
+        engine.init_plane_wave(1.0)
+        engine.multislice(0, 3)
+        engine.synchronize()
+    # a new slice range is captured anew
+    engine.init_plane_wave(1.0)
+    engine.multislice(0, 5, graph=True)
+    part = engine.exit_wave(0)
+    engine.init_plane_wave(1.0)
+    engine.multislice(0, 5)
+    assert np.array_equal(part, engine.exit_wave(0))
+
+
+def test_amorphous_sio2_full3d_1024(engine, oracle):
+    """The full-size amorphous SiO2 case (20000 atoms, 1024^2, 106 slices, full-3D Lobato,
+    20 integrals, 300 kV) with 3 frozen-phonon configurations through the graph-captured
+    loop: transmission of a mid-specimen slice against the FP64 oracle, graph against eager
+    slice loop, and the configuration mean against per-configuration runs."""
+    w = wl.ctem_amorphous_sio2(n_configs=3)
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    assert g.n_slices == 106 and g.resolution == 1024
+    p = cltem_amd.load_params(w["params"])
+    engine.set_geometry(g)
+    engine.set_parameterisation(p)
+    n_cfg = w["n_configs"]
+    for c in range(n_cfg):
+        s = w["specimen"].displaced(w["u_rms"], w["seed"], c)
+        engine.sort_atoms(c, s.x, s.y, s.z, s.Z)
+        engine.build_transmission(c)
+    # A3 parity at full size on one slice of configuration 0
+    s0 = w["specimen"].displaced(w["u_rms"], w["seed"], 0)
+    srt = oracle.sort_atoms(og, s0.x, s0.y, s0.z, s0.Z)
+    ref_t = oracle.transmission(og, srt, p.values, p.form, p.i_per_atom, first=53, count=1)[0]
+    err_t = rel_l2(engine.read_transmission(0, 53), ref_t)
+    print(f"slice 53 transmission vs FP64 oracle: rel-L2 {err_t:.3e}")
+    assert err_t <= 1e-5
+    engine.set_wave_count(n_cfg)
+    engine.set_wave_configs(list(range(n_cfg)))
+    engine.init_plane_wave(1.0)
+    engine.multislice(graph=True)
+    mean = engine.exit_intensity_sum() / n_cfg
+    per = np.zeros_like(mean)
+    engine.set_wave_count(1)
+    for c in range(n_cfg):
+        engine.set_wave_configs([c])
+        engine.init_plane_wave(1.0)
+        engine.multislice()
+        per += np.abs(engine.exit_wave(0)) ** 2
+    err = rel_l2(mean, per / n_cfg)
+    print(f"graph-batched configuration mean vs per-configuration eager runs: rel-L2 {err:.3e}")
+    assert err <= 1e-6
+    assert np.all(np.isfinite(mean)) and 0.5 < mean.mean() < 1.5
+
+
+@pytest.mark.parametrize("order", ["thin_first", "thick_first"])
+def test_thinner_specimens_stop_at_their_exit_surface(engine, oracle, order):
+    """Waves of a thinner specimen batched with a thicker one hold their own exit wave (the
+    result of a separate run over their own slices), whether the waves are ordered by
+    thickness (prefix sets) or not (device wave lists), eagerly and from the graph."""
+    thick = wl.ctem_srtio3(3, 3, 6, resolution=256)
+    thin = wl.ctem_srtio3(3, 3, 3, resolution=256)
+    gk = cltem_amd.plan_geometry(cltem_amd.make_scenario(**thick["scenario"]))
+    gn = cltem_amd.plan_geometry(cltem_amd.make_scenario(**thin["scenario"]))
+    assert gn.n_slices < gk.n_slices
+    p = cltem_amd.load_params("kirkland")
+    engine.set_geometry(gk)
+    engine.set_parameterisation(p)
+    engine.sort_atoms(0, thick["specimen"].x, thick["specimen"].y, thick["specimen"].z, thick["specimen"].Z)
+    engine.build_transmission(0)
+    engine.set_config_geometry(1, gn)
+    engine.sort_atoms(1, thin["specimen"].x, thin["specimen"].y, thin["specimen"].z, thin["specimen"].Z)
+    engine.build_transmission(1)
+    cfgs = [1, 0, 1] if order == "thin_first" else [0, 1, 1]
+    refs = {}
+    for c, w in ((0, thick), (1, thin)):
+        og = oracle.geometry(oracle.scenario(**w["scenario"]))
+        srt = oracle.sort_atoms(og, w["specimen"].x, w["specimen"].y, w["specimen"].z, w["specimen"].Z)
+        refs[c], _ = oracle.run_plane(oracle.transmission(og, srt, p.values, p.form, p.i_per_atom),
+                                      oracle.propagator(og))
+    engine.set_wave_count(3)
+    engine.set_wave_configs(cfgs)
+    for graph in (False, True, True):
+        engine.init_plane_wave(1.0)
+        engine.multislice(graph=graph)
+        for i, c in enumerate(cfgs):
+            err = rel_l2(engine.exit_wave(i), refs[c])
+            assert err <= 1e-4, (order, graph, i, err)
+    print(f"{order}: thin {gn.n_slices} and thick {gk.n_slices} slices match separate FP64 runs")
+
+
+def test_mixed_size_cbed_driver(oracle):
+    """cltem_amd.mixed: instances of two grid sizes and several thicknesses and convergence
+    angles, one engine per size running concurrently, each size graph-captured with its
+    own autotuned kernels; every phonon-averaged pattern against the FP64 oracle."""
+    from cltem_amd.mixed import run_mixed_cbed
+    inst = wl.mixed_cbed(n_instances=4, cells_xy=3, thickness_cells=(2, 5), sizes=(256, 384), n_configs=2)
+    inst[0].scenario["resolution"] = 256
+    inst[1].scenario["resolution"] = 384
+    patterns, groups = run_mixed_cbed(inst)
+    assert set(groups) <= {256, 384} and len(groups) == 2
+    p = cltem_amd.load_params("kirkland")
+    for i, it in enumerate(inst):
+        og = oracle.geometry(oracle.scenario(**it.scenario))
+        P_ = oracle.propagator(og)
+        ref = np.zeros((og.resolution, og.resolution))
+        for ph in range(it.n_configs):
+            d = it.specimen.displaced(it.u_rms, it.seed, ph)
+            srt = oracle.sort_atoms(og, d.x, d.y, d.z, d.Z)
+            tr = oracle.transmission(og, srt, p.values, p.form, p.i_per_atom)
+            psi = oracle.fft2(oracle.probe_kspace(og, it.position[0] - og.sim_x[0], it.position[1] - og.sim_y[0],
+                                                  None, it.alpha_mrad), inverse=True)
+            for sl in range(og.n_slices):
+                psi, recip = oracle.slice_step(psi, tr[sl], P_)
+            ref += oracle.diffraction(recip)
+        ref /= it.n_configs
+        err = rel_l2(patterns[i], ref)
+        print(f"instance {i}: {og.resolution}^2, {og.n_slices} slices, alpha {it.alpha_mrad:.1f} mrad: rel-L2 {err:.3e}")
+        assert err <= 1e-3
