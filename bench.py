@@ -18,6 +18,13 @@ launch sequence with a host sync after every slice, as clTEM's doMultiSliceStep
 does), the dominant kernel's roofline from hipEvent timings, and the CPU baseline
 (the FP32 C++ restatement timed on the host cores) with the GPU-vs-FP64 detector
 error on sampled probes.
+
+--workload sio2 measures the full-3D frozen-phonon CTEM case instead (amorphous SiO2,
+20,000 atoms, 1024^2, 106 slices, 300 kV, Lobato with 20 integrals, 30 configurations
+batched through one graph-captured slice loop): value is configuration.slices/s.
+--workload mixed measures the mixed-size batched CBED case (48 SrTiO3 instances on
+256^2..768^2 grids, 5 configurations each, one engine per size running concurrently):
+value is wave.slices/s over the concurrent timed window.
 """
 from __future__ import annotations
 
@@ -161,8 +168,138 @@ def cpu_baseline(w, g, spec0, pos, gpu_det, n_probes, n_check):
     }, err, n_check
 
 
+def kernel_roofline(engine, g, n_waves, label):
+    """Row/column pass times of one eager slice loop over all waves (each wave reading its
+    own transmission function), as achieved algorithmic GB/s against the HBM peak."""
+    N = g.resolution * g.resolution
+    ns = g.n_slices
+    engine.profile_reset()
+    engine.profile(True)
+    engine.multislice()
+    engine.synchronize()
+    engine.profile(False)
+    n_row, t_row = engine.profile_read(0)
+    n_col, t_col = engine.profile_read(1)
+    row_bytes = ns * n_waves * 24 * N   # read + write the wave, read its t
+    col_bytes = ns * n_waves * 16 * N
+    kern = {"row_pass": (n_row, t_row, row_bytes), "column_pass": (n_col, t_col, col_bytes)}
+    name = max(kern, key=lambda k: kern[k][1])
+    n, tms, b = kern[name]
+    achieved = b / (tms * 1e-3) / 1e9
+    return {
+        "kernel": f"{name} ({label})", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "launches": n, "avg_launch_us": round(tms * 1e3 / max(n, 1), 2), "bytes_per_launch": int(b / max(n, 1)),
+        "row_pass_GBs": round(row_bytes / (t_row * 1e-3) / 1e9, 1),
+        "column_pass_GBs": round(col_bytes / (t_col * 1e-3) / 1e9, 1),
+    }
+
+
+def bench_sio2(args):
+    w = wl.ctem_amorphous_sio2(n_configs=args.configs)
+    spec = w["specimen"]
+    n_cfg = w["n_configs"]
+    engine = cltem_amd.Engine(0)
+    dev, cus, _ = engine.device_info()
+    g, p = setup(engine, w)
+    N = g.resolution * g.resolution
+    ns = g.n_slices
+    log(f"{dev}: {len(spec)} atoms, {g.resolution}^2, {ns} slices, {n_cfg} configurations, full 3D")
+    t0 = time.perf_counter()
+    for c in range(n_cfg):
+        build_config(engine, c, spec.displaced(w["u_rms"], w["seed"], c))
+    engine.synchronize()
+    t_build = time.perf_counter() - t0
+    log(f"potentials: {t_build:.2f} s for {n_cfg} configurations")
+    tune = engine.autotune(0, n_cfg, reps=2)
+    engine.set_wave_count(n_cfg)
+    engine.set_wave_configs(list(range(n_cfg)))
+    mean = None
+    for _ in range(args.warmup):
+        engine.init_plane_wave(1.0)
+        engine.multislice(graph=True)
+        mean = engine.exit_intensity_sum()
+    engine.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        engine.init_plane_wave(1.0)
+        engine.multislice(graph=True)
+        mean = engine.exit_intensity_sum()
+    engine.synchronize()
+    t_total = time.perf_counter() - t0
+    value = args.steps * n_cfg * ns / t_total
+    log(f"timed {args.steps} steps in {t_total:.3f} s -> {value:.1f} configuration.slices/s")
+    engine.init_plane_wave(1.0)
+    roof = kernel_roofline(engine, g, n_cfg, f"1024^2, {n_cfg} configurations")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle_py as orc
+        og = orc.geometry(orc.scenario(**w["scenario"]))
+        k = min(args.cpu_slices, ns)
+        trans = np.stack([engine.read_transmission(0, s) for s in range(k)]).astype(np.complex128)
+        t0 = time.perf_counter()
+        orc.run_plane(trans, orc.propagator(og))
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(k / dt, 3), "unit": "configuration·slices/s", "cores": 1, "kind": "port",
+               "sample": f"{k} slices of configuration 0 through the FP64 C++ slice loop (the GPU-built "
+                         f"transmission functions read back; the oracle's own full-3D potential build "
+                         f"at this size is not timed)"}
+    engine.close()
+    line = {
+        "metric": "full-3D frozen-phonon CTEM configuration·slices/s at 1024² (30 configurations batched)",
+        "value": round(value, 2), "unit": "configuration·slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_total / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (amorphous SiO2 by seeded random sequential addition, seeded Philox phonons)",
+        "config": {"workload": f"full-3D CTEM, amorphous SiO2 {len(spec)} atoms 60x60x100 A, {g.resolution}^2, "
+                               f"{ns} slices, 300 kV, Lobato 20 integrals, {n_cfg} configurations u_rms 0.1 A, "
+                               f"phonon-averaged exit intensity", "device": dev},
+        "kernel_tuning": tune, "potential_build_s": round(t_build, 3),
+        "exit_intensity_mean": float(mean.mean() / n_cfg) if mean is not None else None,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_mixed(args):
+    from cltem_amd.mixed import run_mixed_cbed
+    inst = wl.mixed_cbed()
+    log(f"{len(inst)} CBED instances, sizes "
+        f"{sorted(set(i.scenario['resolution'] for i in inst))}, {inst[0].n_configs} configurations each")
+    patterns, groups = run_mixed_cbed(inst, warmup=args.warmup, repeats=args.steps)
+    t0 = min(r.t_start for r in groups.values())
+    t1 = max(r.t_end for r in groups.values())
+    wave_slices = sum(r.wave_slices for r in groups.values()) * args.steps
+    value = wave_slices / (t1 - t0)
+    per = {str(res): {"instances": len(r.instances), "wave_slices": r.wave_slices,
+                      "build_s": round(r.build_s, 3), "pass_s": round(r.loop_s, 4),
+                      "wave_slices_per_s": round(r.wave_slices / r.loop_s, 1),
+                      "tuning": {"best_row": r.tuning["best_row"], "best_col": r.tuning["best_col"]}
+                      if r.tuning else None}
+           for res, r in sorted(groups.items())}
+    log(f"mixed: {value:.0f} wave.slices/s; {per}")
+    line = {
+        "metric": "mixed-size batched CBED wave·slices/s (48 instances, 256²-768², concurrent per-size engines)",
+        "value": round(value, 1), "unit": "wave·slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (SrTiO3 5x5 cells, seeded thickness/size/convergence, seeded Philox phonons)",
+        "config": {"workload": "mixed-size CBED, 48 SrTiO3 [001] instances, 10-80 cells thick, grids "
+                               "256/384/512/768, 5-25 mrad, 5 configurations u_rms 0.08 A, FP64 "
+                               "diffraction sums"},
+        "per_size": per,
+        "pattern_totals": [float(p.sum()) for p in patterns[:4]],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", choices=("stem", "sio2", "mixed"), default="stem")
+    ap.add_argument("--configs", type=int, default=30, help="sio2: frozen-phonon configurations")
+    ap.add_argument("--cpu-slices", type=int, default=24, help="sio2: slices in the CPU baseline sample")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed frozen-phonon configurations")
     ap.add_argument("--warmup", type=int, default=1)
@@ -182,9 +319,13 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(probes and phonon configurations shard across GPUs with one FP64 reduction, "
-              "see DESIGN.md)", file=sys.stderr)
+              "(probes and phonon configurations would shard across GPUs with one FP64 "
+              "all-reduce of the detector or intensity sums at the end)", file=sys.stderr)
         return 2
+    if args.workload == "sio2":
+        return bench_sio2(args)
+    if args.workload == "mixed":
+        return bench_mixed(args)
 
     w = wl.stem_srtio3()
     spec = w["specimen"]
@@ -259,7 +400,6 @@ def main():
         log(f"slice loop B={B}: {sweep[str(B)]:.0f} probe.slices/s")
     plain, _ = plain_rate(engine, g, pos[: args.plain_probes], probe, dets)
     log(f"plain (B=1, sync per slice): {plain:.0f} probe.slices/s")
-    best_b = max((b for b in sweep if b != "1"), key=lambda b: sweep[b], default=str(args.batch))
 
     roof = roofline(engine, g, pos, args.roof_batch, probe, dets)
     log(f"roofline: {roof}")

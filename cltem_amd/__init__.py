@@ -277,14 +277,14 @@ class Engine:
         _check(self.lib.mse_set_kernel_variant(self.h, kernel, variant))
 
     def autotune(self, config: int = 0, n_waves: int = 64, reps: int = 5):
-        row = np.zeros(8)
-        col = np.zeros(8)
+        row = np.zeros(16)
+        col = np.zeros(16)
         br = C.c_int(0)
         bc = C.c_int(0)
         _check(self.lib.mse_autotune(self.h, config, n_waves, reps, _dp(row), _dp(col), C.byref(br),
                                      C.byref(bc)))
         self.n_waves = max(self.n_waves, n_waves)
-        nr, nc = 5, 8
+        nr, nc = 6, 9  # row / column kernel variants
         return dict(row_us=row[:nr].tolist(), col_us=col[:nc].tolist(), best_row=br.value,
                     best_col=bc.value)
 

@@ -214,9 +214,11 @@ void slice_step(mse_engine* e, float2* waves, int nw, int slice);
 //          5: 8 columns, twiddles read at use, >= 8 waves per SIMD (4 blocks per CU)
 //          6: 32 columns (256-byte row runs), half lanes per column
 //   row    4: persistent lines with the next row prefetched into registers
+//          5: as 4 with half lanes per line (res <= 512; otherwise as 4)
 //   column 7: persistent blocks with the next tile prefetched into registers
-constexpr int kRowVariants = 5;
-constexpr int kColVariants = 8;
+//          8: as 7 with half lanes per column, two blocks per CU (res <= 512; otherwise as 7)
+constexpr int kRowVariants = 6;
+constexpr int kColVariants = 9;
 
 // Sweep launchers, one translation unit per resolution (sweeps_<N>.hip).
 template <int N>

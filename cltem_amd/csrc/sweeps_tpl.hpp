@@ -49,7 +49,18 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
         using P1 = typename mse::PlanHalfLanes<N>::P;
         using REG = std::true_type;
         using MEM = std::false_type;
-        if (mode == mse::ROW_IFFT_T_FFT && variant == 4 && in == out) {
+        if constexpr (N <= 512) {  // half-lane lines keep the prefetch registers within budget up to 512
+            if (mode == mse::ROW_IFFT_T_FFT && variant == 5 && in == out) {
+                constexpr int LINES = 256 / P1::L;
+                const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+                launch(e, PK_ROW, [&]() {
+                    mse::k_row_step_pipe<P1, mse::WaveSync><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p,
+                                                                                         args, nlines_sets);
+                });
+                return;
+            }
+        }
+        if (mode == mse::ROW_IFFT_T_FFT && variant >= 4 && in == out) {
             constexpr int LINES = 256 / P0::L;
             const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
             launch(e, PK_ROW, [&]() {
@@ -110,7 +121,19 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
         using C32 = std::integral_constant<int, 32>;
         using REG = std::true_type;
         using MEM = std::false_type;
-        if (variant == 7 && mode == mse::COL_PROP && in == out) {
+        if constexpr (N <= 512) {
+            if (variant == 8 && mode == mse::COL_PROP && in == out) {  // half lanes: two blocks per CU
+                constexpr int NT = 16 * P1::L;
+                const int blocks = std::max(1, std::min(e->num_cus * (2048 / NT), (N / 16) * nsets));
+                tiles = N / 16;
+                launch(e, kind, [&]() {
+                    mse::k_col_prop_pipe<P1, 16, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride,
+                                                                                            e->tw.p, args, nsets);
+                });
+                return tiles;
+            }
+        }
+        if (variant >= 7 && mode == mse::COL_PROP && in == out) {
             constexpr int NT = 16 * P0::L;
             const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
             tiles = N / 16;
@@ -120,7 +143,7 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
             });
             return tiles;
         }
-        if (variant == 7) variant = 0;
+        if (variant >= 7) variant = 0;
         if (variant == 1)
             go(P1{}, C16{}, mse::WaveSync{}, REG{});
         else if (variant == 2)

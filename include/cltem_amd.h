@@ -183,8 +183,12 @@ int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes
                    const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
                    int use_graph, double* accum);
 
-/* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..4),
- *      kernel 1 = column pass (variants 0..7); see DESIGN.md for what each variant changes.
+/* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..5):
+ *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at
+ *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes.
+ *      kernel 1 = column pass (variants 0..8): 0 16-column tiles, 1 half lanes, 2 8-column
+ *      tiles, 3 block barriers, 4/5 twiddles read at use (16/8 columns), 6 32-column tiles,
+ *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes.
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
  *      (reps launches each; times in microseconds per launch), selects the fastest and
  *      leaves the waves' contents undefined. ---- */
