@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import MseDetector, MseGeometry, MseProbe, MseScenario
+from ._abi import MseDetector, MseGeometry, MseImaging, MseProbe, MseScenario
 from .params import Parameterisation, load as load_params  # noqa: F401
 
 __all__ = ["Engine", "MseError", "plan_geometry", "make_scenario", "make_probe", "make_detectors",
@@ -89,6 +89,28 @@ def probe_aberration_vector(p: MseProbe) -> np.ndarray:
     out = [p.C10, *p.C12, *p.C21, *p.C23, p.C30, *p.C32, *p.C34, *p.C41, *p.C43, *p.C45, p.C50,
            *p.C52, *p.C54, *p.C56]
     return np.array(out, dtype=np.float64)
+
+
+def make_imaging(obj_ap_mrad=20.0, obj_smooth_mrad=0.0, beta_mrad=0.0, delta=0.0, aberrations=None) -> MseImaging:
+    """Objective-lens parameters of the CTEM image (aberrations as in make_probe)."""
+    im = MseImaging()
+    ab = aberrations or {}
+    for name in ("C10", "C30", "C50"):
+        setattr(im, name, float(ab.get(name, 0.0)))
+    for name in ("C12", "C21", "C23", "C32", "C34", "C41", "C43", "C45", "C52", "C54", "C56"):
+        mag, ang = ab.get(name, (0.0, 0.0))
+        z = mag * np.exp(1j * ang)
+        getattr(im, name)[:] = (z.real, z.imag)
+    im.obj_ap_mrad = obj_ap_mrad
+    im.obj_smooth_mrad = obj_smooth_mrad
+    im.beta_mrad = beta_mrad
+    im.delta = delta
+    return im
+
+
+def imaging_vector(im: MseImaging) -> np.ndarray:
+    """The 25 aberration doubles in the oracle's order (C10, C12 re/im, ..., C56 re/im)."""
+    return probe_aberration_vector(im)
 
 
 def make_detectors(rows):
@@ -226,6 +248,15 @@ class Engine:
         out = np.zeros(2 * res * res)
         _check(self.lib.mse_exit_wave(self.h, wave, _dp(out)))
         return out.view(np.complex128).reshape(res, res)
+
+    def ctem_image(self, imaging: MseImaging, accum=None) -> np.ndarray:
+        """CTEM image (objective aperture, aberrations, coherence envelopes) of every wave,
+        summed over the waves in FP64; the exit waves are kept."""
+        res = self.g.resolution
+        out = np.zeros((res, res)) if accum is None else accum
+        assert out.shape == (res, res) and out.dtype == np.float64 and out.flags.c_contiguous
+        _check(self.lib.mse_ctem_image(self.h, C.byref(imaging), int(accum is not None), _dp(out)))
+        return out
 
     def exit_intensity_sum(self, accum=None) -> np.ndarray:
         res = self.g.resolution

@@ -93,6 +93,29 @@ class MseProbe(C.Structure):
     ]
 
 
+class MseImaging(C.Structure):
+    _fields_ = [
+        ("C10", C.c_double),
+        ("C12", C.c_double * 2),
+        ("C21", C.c_double * 2),
+        ("C23", C.c_double * 2),
+        ("C30", C.c_double),
+        ("C32", C.c_double * 2),
+        ("C34", C.c_double * 2),
+        ("C41", C.c_double * 2),
+        ("C43", C.c_double * 2),
+        ("C45", C.c_double * 2),
+        ("C50", C.c_double),
+        ("C52", C.c_double * 2),
+        ("C54", C.c_double * 2),
+        ("C56", C.c_double * 2),
+        ("obj_ap_mrad", C.c_double),
+        ("obj_smooth_mrad", C.c_double),
+        ("beta_mrad", C.c_double),
+        ("delta", C.c_double),
+    ]
+
+
 class MseDetector(C.Structure):
     _fields_ = [
         ("inner_mrad", C.c_double),
@@ -130,6 +153,7 @@ SIGNATURES = [
     ("mse_stem_detect", C.c_int, [_eng, C.POINTER(MseDetector), C.c_int, C.c_int, _dp]),
     ("mse_diffraction", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
+    ("mse_ctem_image", C.c_int, [_eng, C.POINTER(MseImaging), C.c_int, _dp]),
     ("mse_exit_intensity_sum", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_diffraction_sum", C.c_int, [_eng, _ip, C.c_int, C.c_int, _dp]),
     ("mse_multislice_graph", C.c_int, [_eng, C.c_int, C.c_int]),

@@ -1083,6 +1083,57 @@ int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, i
     });
 }
 
+int mse_ctem_image(mse_engine* e, const mse_imaging* im, int accumulate, double* out) {
+    return run_guarded("mse_ctem_image", e, [&]() {
+        REQUIRE(e && im && out, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(im->obj_ap_mrad >= 0 && im->obj_smooth_mrad >= 0 && im->beta_mrad >= 0, MSE_ERR_ARG,
+                "apertures and beta must be non-negative");
+        const int nw = e->n_waves;
+        const double lam = e->g.wavelength;
+        mse::ColArgs ca = base_col_args(e);
+        mse::ProbeArgs& p = ca.probe;
+        p.wavelength = lam;
+        p.cond_ap2 = (im->obj_ap_mrad * 0.001) / lam;  // ctem_image_d.cl: obj_ap2, ap_smooth_radius
+        p.smooth_r = (im->obj_smooth_mrad * 0.001) / lam;
+        p.beta2 = (im->beta_mrad * 0.001) / lam;
+        p.delta = im->delta;
+        p.C10 = im->C10;
+        p.C30 = im->C30;
+        p.C50 = im->C50;
+        p.C12 = make_double2(im->C12[0], im->C12[1]);
+        p.C21 = make_double2(im->C21[0], im->C21[1]);
+        p.C23 = make_double2(im->C23[0], im->C23[1]);
+        p.C32 = make_double2(im->C32[0], im->C32[1]);
+        p.C34 = make_double2(im->C34[0], im->C34[1]);
+        p.C41 = make_double2(im->C41[0], im->C41[1]);
+        p.C43 = make_double2(im->C43[0], im->C43[1]);
+        p.C45 = make_double2(im->C45[0], im->C45[1]);
+        p.C52 = make_double2(im->C52[0], im->C52[1]);
+        p.C54 = make_double2(im->C54[0], im->C54[1]);
+        p.C56 = make_double2(im->C56[0], im->C56[1]);
+        // column pass: F_y, x transfer(kx, ky) / res^2, F_y^-1 into a work copy of the waves;
+        // row pass: |F_x^-1|^2 = |unitary IFFT2(transfer x Psi)|^2 in double
+        ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
+        e->img_tmp.ensure((size_t)nw * e->N);
+        col_pass(e, mse::COL_CTEM_IMAGE, e->waves.p, e->img_tmp.p, nw, e->N, ca, PK_OTHER);
+        e->diff_img.ensure((size_t)nw * e->N);
+        e->det_out.ensure(e->N);
+        if (accumulate)
+            HIPX(hipMemcpyAsync(e->det_out.p, out, e->N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        mse::RowArgs ra = base_row_args(e);
+        ra.scale = 1.0f;
+        ra.dout = e->diff_img.p;
+        row_pass(e, mse::ROW_IFFT_INTENSITY, e->img_tmp.p, e->img_tmp.p, nw, e->N, ra);
+        launch(e, PK_OTHER, [&]() {
+            mse::k_sum_waves<<<(unsigned)((e->N + 255) / 256), 256, 0, e->stream>>>(e->diff_img.p, nw, e->N,
+                                                                                   e->det_out.p, accumulate);
+        });
+        HIPX(hipMemcpyAsync(out, e->det_out.p, e->N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
 int mse_exit_intensity_sum(mse_engine* e, int accumulate, double* out) {
     return run_guarded("mse_exit_intensity_sum", e, [&]() {
         REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");

@@ -125,6 +125,7 @@ struct mse_engine {
     DevBuf<double> det_out;
     DevBuf<double> diff_img;
     DevBuf<float2> scratch;
+    DevBuf<float2> img_tmp;  // CTEM image formation work copy of the waves
     // sort scratch
     DevBuf<double> tmp_xyz;
     DevBuf<int> tmp_Z, tmp_bid, tmp_zid;

@@ -39,6 +39,7 @@ enum ColMode : int {
     COL_PROP_STORE_DETECT = 6,  // last slice: propagate, reduce detectors and store
     COL_COPY = 7,         // move the tile through LDS only (bandwidth reference)
     COL_PROP_CONJ = 8,    // adjoint column half: propagation by conj(P')
+    COL_CTEM_IMAGE = 9,   // objective-lens transfer of Psi(k) (ctem_image), back to the mixed domain
 };
 
 constexpr int kMaxDet = 8;
@@ -64,6 +65,7 @@ struct ProbeArgs {
     double C10, C30, C50;
     double2 C12, C21, C23, C32, C34, C41, C43, C45, C52, C54, C56;
     int taper;
+    double beta2, delta;        // COL_CTEM_IMAGE: illumination semi-angle (1/Angstrom), defocus spread
 };
 
 struct ColArgs {
@@ -198,6 +200,48 @@ __device__ __forceinline__ double2 dpow(double2 a, int n) {
     return r;
 }
 
+// ctem_image (kernels/ctem_image_d.cl) for one k-space pixel, in double: the factor that
+// multiplies Psi(k): smoothed objective aperture x temporal and spatial coherence envelopes x
+// exp(-i chi), chi = 2 pi / lambda * sum of the aberration terms in w = lambda (kx + i ky).
+__device__ __forceinline__ float2 ctem_transfer(const ProbeArgs& p, double kx, double ky) {
+    const double lam = p.wavelength;
+    const double k = sqrt(kx * kx + ky * ky);
+    if (!(k < p.cond_ap2 + p.smooth_r)) return make_float2(0.f, 0.f);
+    const double2 w = make_double2(lam * kx, lam * ky);
+    const double2 wc = make_double2(w.x, -w.y);
+    const double w2 = w.x * w.x + w.y * w.y;
+    const double temporal = exp(-0.25 * M_PI * M_PI * p.delta * p.delta * w2 * w2 / (lam * lam));
+    const double defs = p.C10 + p.C30 * w2 + p.C50 * w2 * w2;
+    const double spatial = exp(-1.0 * M_PI * M_PI * p.beta2 * p.beta2 * w2 * (defs * defs));
+    const double2 wc2 = dpow(wc, 2), wc3 = dpow(wc, 3), wc4 = dpow(wc, 4), wc5 = dpow(wc, 5), wc6 = dpow(wc, 6);
+    const double2 w_2 = dpow(w, 2);
+    double cchi = 0.5 * p.C10 * w2;
+    cchi += 0.5 * dmul(p.C12, wc2).x;
+    cchi += dmul(p.C21, dmul(wc2, w)).x / 3.0;
+    cchi += dmul(p.C23, wc3).x / 3.0;
+    cchi += 0.25 * p.C30 * w2 * w2;
+    cchi += 0.25 * dmul(p.C32, dmul(wc3, w)).x;
+    cchi += 0.25 * dmul(p.C34, wc4).x;
+    cchi += 0.2 * dmul(p.C41, dmul(wc3, w_2)).x;
+    cchi += 0.2 * dmul(p.C43, dmul(wc4, w)).x;
+    cchi += 0.2 * dmul(p.C45, wc5).x;
+    cchi += p.C50 * w2 * w2 * w2 / 6.0;
+    cchi += dmul(p.C52, dmul(wc4, w_2)).x / 6.0;
+    cchi += dmul(p.C54, dmul(wc5, w)).x / 6.0;
+    cchi += dmul(p.C56, wc6).x / 6.0;
+    double chi = 2.0 * M_PI * cchi / lam;
+    chi = remainder(chi, 2.0 * M_PI);
+    double edge = 1.0;
+    if (fabs(k - p.cond_ap2) < p.smooth_r) {  // 1 - smoothstep(ap - r, ap + r, k)
+        const double t = fmin(fmax((k - (p.cond_ap2 - p.smooth_r)) / (2.0 * p.smooth_r), 0.0), 1.0);
+        edge = 1.0 - t * t * (3.0 - 2.0 * t);
+    }
+    const double amp = edge * temporal * spatial;
+    double sc, cc;
+    sincos(chi, &sc, &cc);
+    return make_float2((float)(amp * cc), (float)(-amp * sc));
+}
+
 // init_probe_wave (kernels/init_probe_wave_f.cl:49-118) for one k-space pixel, in double.
 __device__ __forceinline__ float2 probe_value(const ProbeArgs& p, double kx, double ky, double px,
                                               double py) {
@@ -323,6 +367,10 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     } else if constexpr (MODE == COL_FFT_DETECT || MODE == COL_FFT_DIFF) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    } else if constexpr (MODE == COL_CTEM_IMAGE) {
+        const double kxv = a.kx[x];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cscale(cmul(v[m], ctem_transfer(a.probe, kxv, a.ky[t + L * m])), a.scale);
     }
 
     if constexpr (MODE == COL_PROP_DETECT || MODE == COL_PROP_STORE_DETECT ||
@@ -365,7 +413,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     }
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
-                  MODE == COL_PROBE || MODE == COL_PROP_CONJ) {
+                  MODE == COL_PROBE || MODE == COL_PROP_CONJ || MODE == COL_CTEM_IMAGE) {
         if constexpr (MODE != COL_PROBE) sync();  // the forward pass's last reads precede new writes
         fft_line<PL, true>(v, ln, t, tws, sync);
         sync();

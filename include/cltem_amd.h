@@ -92,6 +92,23 @@ typedef struct mse_probe {
     int edge_taper;          /* 1: smoothstep edge of init_probe_wave_f; 0: init_probe_wave_d */
 } mse_probe;
 
+/* Objective-lens imaging of a CTEM exit wave: the MicroscopeParameters fields that
+ * SimulationCtem::simulateImagePerfect passes to ctem_image (microscope/simulationctem.cpp:113-171).
+ * Aberrations as in mse_probe; apertures and the illumination semi-angle in mrad. */
+typedef struct mse_imaging {
+    double C10;
+    double C12[2], C21[2], C23[2];
+    double C30;
+    double C32[2], C34[2];
+    double C41[2], C43[2], C45[2];
+    double C50;
+    double C52[2], C54[2], C56[2];
+    double obj_ap_mrad;      /* ObjectiveAperture */
+    double obj_smooth_mrad;  /* ObjectiveApertureSmoothing */
+    double beta_mrad;        /* Alpha: illumination semi-angle (spatial coherence envelope) */
+    double delta;            /* Delta: defocus spread, Angstrom (temporal coherence envelope) */
+} mse_imaging;
+
 /* StemDetector (utilities/commonstructs.h:292-300): radii and centre in mrad. */
 typedef struct mse_detector {
     double inner_mrad, outer_mrad, xc_mrad, yc_mrad;
@@ -166,6 +183,13 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
 int mse_diffraction(mse_engine* e, int wave, double* out);
 /* getExitWaveImage (simulationgeneral.cpp:808-829) without cropping: interleaved re/im, 2N doubles */
 int mse_exit_wave(mse_engine* e, int wave, double* out);
+/* CTEM image of every wave, summed over the waves into N doubles (out += when accumulate):
+ * ctem_image (kernels/ctem_image_d.cl: objective aperture with smoothed edge, aberration
+ * phase exp(-i chi), temporal and spatial coherence envelopes) applied to Psi(k), the inverse
+ * FFT and sqabs (SimulationCtem::simulateImagePerfect, microscope/simulationctem.cpp:113-171),
+ * fused into one column pass and one row pass; the exit waves are left unchanged.  With one
+ * wave per frozen-phonon configuration this is the FP64 image sum of updateImages. */
+int mse_ctem_image(mse_engine* e, const mse_imaging* im, int accumulate, double* out);
 /* sum over all waves of |psi_exit|^2 into N doubles (out += when accumulate): the phonon
  * average of the exit-wave intensity as SimulationManager::updateImages sums jobs in FP64
  * (simulationmanager.cpp:274-350) */

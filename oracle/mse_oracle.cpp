@@ -1165,3 +1165,85 @@ int orc_ctem_loss_fd(const OrcGeometry* g, const double* x, const double* y, con
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// CTEM image formation: ctem_image_d (kernels/ctem_image_d.cl) on Psi(k), the inverse FFT
+// and sqabs_d (SimulationCtem::simulateImagePerfect, microscope/simulationctem.cpp:113-171).
+// lens = 25 aberration doubles in the probe order, then obj_ap_mrad, ap_smooth_mrad,
+// beta_mrad, delta.  recip is the unshifted Psi(k) (2N doubles); out has N doubles.
+// ---------------------------------------------------------------------------------------
+extern "C" int orc_ctem_image(const OrcGeometry* g, const double* recip, const double* lens, double* out) {
+    typedef std::complex<double> C;
+    const int res = g->resolution;
+    const size_t N = (size_t)res * res;
+    std::vector<double> kx, ky;
+    double bw;
+    kgrid<double>(g, kx, ky, &bw);
+    const double wavelength = g->wavelength;
+    auto cpow = [](C a, int n) {
+        C temp = a;
+        for (int j = 1; j < n; j++) temp = temp * a;
+        return temp;
+    };
+    auto modsq = [](C a) { return a.real() * a.real() + a.imag() * a.imag(); };
+    const double* ab = lens;
+    const double C10 = ab[0];
+    const C C12(ab[1], ab[2]), C21(ab[3], ab[4]), C23(ab[5], ab[6]);
+    const double C30 = ab[7];
+    const C C32(ab[8], ab[9]), C34(ab[10], ab[11]);
+    const C C41(ab[12], ab[13]), C43(ab[14], ab[15]), C45(ab[16], ab[17]);
+    const double C50 = ab[18];
+    const C C52(ab[19], ab[20]), C54(ab[21], ab[22]), C56(ab[23], ab[24]);
+    const double obj_ap = lens[25], ap_smooth = lens[26], beta = lens[27], delta = lens[28];
+    const C* in = reinterpret_cast<const C*>(recip);
+    std::vector<C> img(N);
+    for (int yid = 0; yid < res; ++yid)
+        for (int xid = 0; xid < res; ++xid) {
+            const size_t id = (size_t)xid + (size_t)yid * res;
+            const double obj_ap2 = (obj_ap * 0.001) / wavelength;
+            const double beta2 = (beta * 0.001) / wavelength;
+            const double k = std::sqrt((kx[xid] * kx[xid]) + (ky[yid] * ky[yid]));
+            const double ap_smooth_radius = (ap_smooth * 0.001) / wavelength;
+            if (k < obj_ap2 + ap_smooth_radius) {
+                const C w(wavelength * kx[xid], wavelength * ky[yid]);
+                const C wc = std::conj(w);
+                const double temporalCoh =
+                    std::exp(-0.25 * M_PI * M_PI * delta * delta * modsq(w) * modsq(w) / (wavelength * wavelength));
+                const double spatialCoh = std::exp(-1.0 * M_PI * M_PI * beta2 * beta2 * modsq(w) *
+                                                   std::pow((C10 + C30 * modsq(w) + C50 * modsq(w) * modsq(w)), 2));
+                const double tC10 = 0.5 * C10 * modsq(w);
+                const C tC12 = 0.5 * (C12 * cpow(wc, 2));
+                const C tC21 = (C21 * (cpow(wc, 2) * w)) / 3.0;
+                const C tC23 = (C23 * cpow(wc, 3)) / 3.0;
+                const double tC30 = 0.25 * C30 * modsq(w) * modsq(w);
+                const C tC32 = 0.25 * (C32 * (cpow(wc, 3) * w));
+                const C tC34 = 0.25 * (C34 * cpow(wc, 4));
+                const C tC41 = 0.2 * (C41 * (cpow(wc, 3) * cpow(w, 2)));
+                const C tC43 = 0.2 * (C43 * (cpow(wc, 4) * w));
+                const C tC45 = 0.2 * (C45 * cpow(wc, 5));
+                const double tC50 = C50 * modsq(w) * modsq(w) * modsq(w) / 6.0;
+                const C tC52 = (C52 * (cpow(wc, 4) * cpow(w, 2))) / 6.0;
+                const C tC54 = (C54 * (cpow(wc, 5) * w)) / 6.0;
+                const C tC56 = (C56 * cpow(wc, 6)) / 6.0;
+                const double cchi = tC10 + tC12.real() + tC21.real() + tC23.real() + tC30 + tC32.real() +
+                                    tC34.real() + tC41.real() + tC43.real() + tC45.real() + tC50 + tC52.real() +
+                                    tC54.real() + tC56.real();
+                const double chi = 2.0 * M_PI * cchi / wavelength;
+                double edge_factor = 1.0;
+                if (std::fabs(k - obj_ap2) < ap_smooth_radius) {
+                    const double e0 = obj_ap2 - ap_smooth_radius, e1 = obj_ap2 + ap_smooth_radius;
+                    const double t = std::min(std::max((k - e0) / (e1 - e0), 0.0), 1.0);
+                    edge_factor = 1.0 - t * t * (3.0 - 2.0 * t);
+                }
+                const double f = edge_factor * temporalCoh * spatialCoh;
+                img[id] = C(f * (in[id].real() * std::cos(chi) + in[id].imag() * std::sin(chi)),
+                            f * (in[id].imag() * std::cos(chi) - in[id].real() * std::sin(chi)));
+            } else {
+                img[id] = C(0.0, 0.0);
+            }
+        }
+    Fft2<double> fft(res, res);
+    fft.run(img.data(), true);
+    for (size_t i = 0; i < N; ++i) out[i] = std::norm(img[i]);  // sqabs_d
+    return 0;
+}

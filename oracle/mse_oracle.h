@@ -20,6 +20,8 @@
  *   orc_propagator / band limit ..... exact masks and exp(-i pi lambda k^2 dz)
  *   orc_probe ....................... numpy chi(), aperture-disc pixel count
  *   orc_run_* ....................... empty specimen identities, numpy multislice
+ *   orc_ctem_image .................. aberration-free wide aperture = |psi|^2; two-beam
+ *                                     closed form with defocus, C30, both envelopes, edge
  *   orc_ctem_intensity .............. orc_run_plane over orc_transmission of the sorted atoms
  *   orc_ctem_loss_fd ................ brute-force central differences (every slice rebuilt,
  *                                     perturbed atoms moved in numpy), O(h^2) convergence
@@ -134,6 +136,9 @@ int orc_run_stem_f32(const OrcGeometry* g, const double* trans, const double* P,
                      int n_probes, const double* aberr, double cond_ap_mrad, double smooth_mrad,
                      int taper, const double* det, int ndet, int sqrt2, double* out, int nthreads);
 
+/* CTEM image (ctem_image + IFFT + sqabs) of the unshifted Psi(k) recip (2N doubles);
+ * lens = 25 aberrations (probe order), obj_ap_mrad, ap_smooth_mrad, beta_mrad, delta. */
+int orc_ctem_image(const OrcGeometry* g, const double* recip, const double* lens, double* out);
 /* CTEM plane wave (value 1) through the full path from unsorted atoms: exit intensity
  * |psi|^2, N doubles. */
 int orc_ctem_intensity(const OrcGeometry* g, const double* x, const double* y, const double* z, const int* Z,

@@ -123,6 +123,7 @@ def lib():
                                      C.c_double, C.c_int]
         L.orc_diffraction.argtypes = [C.c_int, dp, dp]
         L.orc_run_plane.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.orc_ctem_image.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp]
         L.orc_ctem_intensity.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, C.c_int,
                                          C.c_int, dp, C.c_int]
         L.orc_ctem_loss_fd.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, C.c_int,
@@ -239,6 +240,19 @@ def transmission(g, sorted_atoms, params, form, icount, first=0, count=None, ban
                            _ip(keep[4]), _dp(keep[5]), form, icount, first, count,
                            int(band_limit), _dp(out), nthreads or nthreads_default())
     return out.view(np.complex128).reshape(count, res, res)
+
+
+def ctem_image(g, recip, aberr=None, obj_ap_mrad=20.0, obj_smooth_mrad=0.0, beta_mrad=0.0, delta=0.0):
+    """CTEM image |IFFT(ctem_image(Psi))|^2 of the unshifted Psi(k) `recip`."""
+    res = g.resolution
+    lens = np.zeros(29)
+    if aberr is not None:
+        lens[:25] = aberr
+    lens[25:] = (obj_ap_mrad, obj_smooth_mrad, beta_mrad, delta)
+    r = _cplx(recip)
+    out = np.zeros(res * res)
+    assert lib().orc_ctem_image(C.byref(g), _dp(r), _dp(lens), _dp(out)) == 0
+    return out.reshape(res, res)
 
 
 def ctem_intensity(g, x, y, z, Z, params, form, icount, nthreads=None):

@@ -155,3 +155,41 @@ def test_errors_are_reported(engine):
     bad = cltem_amd.plan_geometry(cltem_amd.make_scenario(**dict(scen, resolution=640)))
     with pytest.raises(cltem_amd.MseError):
         engine.set_geometry(bad)
+
+
+def test_ctem_image_formation(engine, oracle):
+    """CTEM image (objective aperture with smoothed edge, defocus, C30, two-fold
+    astigmatism, both coherence envelopes) of two phonon configurations, summed in FP64,
+    against the FP64 oracle; the exit waves are left as they were."""
+    w = wl.ctem_srtio3(3, 3, 4, resolution=256)
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    p = cltem_amd.load_params("kirkland")
+    engine.set_geometry(g)
+    engine.set_parameterisation(p)
+    ab = {"C10": -60.0, "C30": 1.0e7, "C12": (20.0, 0.4)}
+    im = cltem_amd.make_imaging(obj_ap_mrad=25.0, obj_smooth_mrad=2.0, beta_mrad=0.4, delta=25.0, aberrations=ab)
+    lens = cltem_amd.imaging_vector(im)
+    ref = np.zeros((256, 256))
+    for c in range(2):
+        s = w["specimen"].displaced(0.08, 41, c)
+        engine.sort_atoms(c, s.x, s.y, s.z, s.Z)
+        engine.build_transmission(c)
+        srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
+        _, recip = oracle.run_plane(oracle.transmission(og, srt, p.values, p.form, p.i_per_atom),
+                                    oracle.propagator(og))
+        ref += oracle.ctem_image(og, recip, lens, obj_ap_mrad=25.0, obj_smooth_mrad=2.0, beta_mrad=0.4,
+                                 delta=25.0)
+    engine.set_wave_count(2)
+    engine.set_wave_configs([0, 1])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    ew = engine.exit_wave(0)
+    got = engine.ctem_image(im)
+    err = rel_l2(got, ref)
+    print(f"CTEM image (2 configurations): rel-L2 {err:.3e}, contrast {got.std() / got.mean():.3f}")
+    assert err <= 1e-4
+    assert got.std() / got.mean() > 0.05  # a real image, not a flat field
+    assert np.array_equal(engine.exit_wave(0), ew)
+    acc = engine.ctem_image(im, got.copy())
+    np.testing.assert_allclose(acc, 2 * got, rtol=1e-15)
