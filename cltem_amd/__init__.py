@@ -258,6 +258,19 @@ class Engine:
         _check(self.lib.mse_ctem_image(self.h, C.byref(imaging), int(accum is not None), _dp(out)))
         return out
 
+    def ccd_image(self, image, dqe, ntf, binning: int = 1, dose_per_pixel: float = 100.0,
+                  conversion: float = 1.0, seed: int = 0) -> np.ndarray:
+        """CCD DQE filter, seeded Poisson dose and NTF filter applied to a CTEM image."""
+        res = self.g.resolution
+        img = np.ascontiguousarray(image, dtype=np.float64)
+        d = np.ascontiguousarray(dqe, dtype=np.float64)
+        n = np.ascontiguousarray(ntf, dtype=np.float64)
+        assert img.shape == (res, res) and d.shape == n.shape
+        out = np.zeros((res, res))
+        _check(self.lib.mse_ccd_image(self.h, _dp(img), _dp(d), _dp(n), len(d), binning, dose_per_pixel,
+                                      conversion, C.c_ulonglong(seed), _dp(out)))
+        return out
+
     def exit_intensity_sum(self, accum=None) -> np.ndarray:
         res = self.g.resolution
         out = np.zeros((res, res)) if accum is None else accum

@@ -154,6 +154,8 @@ SIGNATURES = [
     ("mse_diffraction", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_ctem_image", C.c_int, [_eng, C.POINTER(MseImaging), C.c_int, _dp]),
+    ("mse_ccd_image", C.c_int, [_eng, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_ulonglong,
+                                _dp]),
     ("mse_exit_intensity_sum", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_diffraction_sum", C.c_int, [_eng, _ip, C.c_int, C.c_int, _dp]),
     ("mse_multislice_graph", C.c_int, [_eng, C.c_int, C.c_int]),

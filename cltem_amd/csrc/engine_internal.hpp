@@ -126,6 +126,7 @@ struct mse_engine {
     DevBuf<double> diff_img;
     DevBuf<float2> scratch;
     DevBuf<float2> img_tmp;  // CTEM image formation work copy of the waves
+    DevBuf<float> ccd_filter;  // sqrt(DQE) and NTF factors per pixel (2 res*res)
     // sort scratch
     DevBuf<double> tmp_xyz;
     DevBuf<int> tmp_Z, tmp_bid, tmp_zid;

@@ -40,6 +40,7 @@ enum ColMode : int {
     COL_COPY = 7,         // move the tile through LDS only (bandwidth reference)
     COL_PROP_CONJ = 8,    // adjoint column half: propagation by conj(P')
     COL_CTEM_IMAGE = 9,   // objective-lens transfer of Psi(k) (ctem_image), back to the mixed domain
+    COL_FILTER = 10,      // multiply the spectrum by a real per-pixel filter (CCD DQE / NTF)
 };
 
 constexpr int kMaxDet = 8;
@@ -73,6 +74,7 @@ struct ColArgs {
     const float2* px;        // propagator factors (COL_PROP*)
     const float2* py;
     const int* lim;          // per ky row: largest |kx index| kept (mask), -1 for none
+    const float* filter;     // COL_FILTER: res*res real factors, row-major [ky][kx] (unshifted)
     const double* kx;        // k-grid (COL_PROBE)
     const double* ky;
     float scale;             // COL_BAND / COL_FFT_*: multiply after the forward DFT
@@ -367,6 +369,9 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     } else if constexpr (MODE == COL_FFT_DETECT || MODE == COL_FFT_DIFF) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    } else if constexpr (MODE == COL_FILTER) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale * a.filter[(size_t)(t + L * m) * N + x]);
     } else if constexpr (MODE == COL_CTEM_IMAGE) {
         const double kxv = a.kx[x];
 #pragma unroll
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     }
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
-                  MODE == COL_PROBE || MODE == COL_PROP_CONJ || MODE == COL_CTEM_IMAGE) {
+                  MODE == COL_PROBE || MODE == COL_PROP_CONJ || MODE == COL_CTEM_IMAGE || MODE == COL_FILTER) {
         if constexpr (MODE != COL_PROBE) sync();  // the forward pass's last reads precede new writes
         fft_line<PL, true>(v, ln, t, tws, sync);
         sync();

@@ -111,6 +111,7 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
                     MSE_COL_CASE(COL_COPY)
                     MSE_COL_CASE(COL_PROP_CONJ)
                     MSE_COL_CASE(COL_CTEM_IMAGE)
+                    MSE_COL_CASE(COL_FILTER)
 #undef MSE_COL_CASE
                 }
             });

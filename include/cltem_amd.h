@@ -190,6 +190,16 @@ int mse_exit_wave(mse_engine* e, int wave, double* out);
  * fused into one column pass and one row pass; the exit waves are left unchanged.  With one
  * wave per frozen-phonon configuration this is the FP64 image sum of updateImages. */
 int mse_ctem_image(mse_engine* e, const mse_imaging* im, int accumulate, double* out);
+/* CCD response and dose of a CTEM image (SimulationCtem::simulateImageDose,
+ * microscope/simulationctem.cpp:174-263): image (N doubles, e.g. from mse_ctem_image) ->
+ * FFT, x sqrt(DQE), IFFT (ccd_dqe, kernels/ccd_dqe_d.cl) -> conversion * Poisson(dose_per_pixel
+ * * binning^2 * real part) -> FFT, x NTF, IFFT (ccd_ntf, kernels/ccd_ntf_d.cl) -> real part
+ * into out.  dqe/ntf are n_table radial samples (clTEM's ccds/*.dat hold 725) at radius /
+ * binning in pixels of the unshifted FFT, linearly interpolated, clamped to the last entry.
+ * The noise comes from Philox4x32-10 keyed by seed (counter = pixel), so a seed reproduces
+ * an image exactly; clTEM's generator is clock-seeded. */
+int mse_ccd_image(mse_engine* e, const double* image, const double* dqe, const double* ntf, int n_table,
+                  int binning, double dose_per_pixel, double conversion, unsigned long long seed, double* out);
 /* sum over all waves of |psi_exit|^2 into N doubles (out += when accumulate): the phonon
  * average of the exit-wave intensity as SimulationManager::updateImages sums jobs in FP64
  * (simulationmanager.cpp:274-350) */

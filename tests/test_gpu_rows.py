@@ -193,3 +193,32 @@ def test_ctem_image_formation(engine, oracle):
     assert np.array_equal(engine.exit_wave(0), ew)
     acc = engine.ctem_image(im, got.copy())
     np.testing.assert_allclose(acc, 2 * got, rtol=1e-15)
+
+
+def test_ccd_dose_matches_oracle_draw_for_draw(engine, oracle):
+    """CCD DQE, seeded Poisson dose and NTF on a CTEM image: with unit tables the engine's
+    Poisson draws equal the oracle generator's draw for draw (both sampler branches); with
+    the K2 tables and 2x binning the whole image matches the FP64 restatement."""
+    from cltem_amd.ccd import TABLE_LEN, load_ccd
+    from oracle import ccd_oracle as co
+    w = wl.ctem_srtio3(3, 3, 4, resolution=256)
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    engine.set_geometry(g)
+    rng = np.random.default_rng(8)
+    x = np.linspace(0, 6 * np.pi, 256)
+    img = 1.0 + 0.5 * np.sin(x)[None, :] * np.cos(0.5 * x)[:, None] + 0.01 * rng.standard_normal((256, 256))
+    ones = np.ones(TABLE_LEN)
+    for dose in (3.0, 40.0):  # means below 10 (inversion) and above (PTRS)
+        got = engine.ccd_image(img, ones, ones, binning=1, dose_per_pixel=dose, seed=77)
+        counts = np.round(got)
+        assert np.abs(got - counts).max() < 1e-2
+        want, _ = co.ccd_image(img, ones, ones, 1, dose, seed=77)
+        same = np.mean(counts == np.round(want))
+        print(f"dose {dose}: {same * 100:.4f}% of pixels draw the same count as the oracle")
+        assert same >= 0.999
+    k2 = load_ccd("k2")
+    got = engine.ccd_image(img, k2.dqe, k2.ntf, binning=2, dose_per_pixel=25.0, conversion=1.5, seed=3)
+    want, _ = co.ccd_image(img, k2.dqe, k2.ntf, 2, 25.0, conversion=1.5, seed=3)
+    err = rel_l2(got, want)
+    print(f"K2, binning 2, 100 e/pixel: rel-L2 {err:.3e}")
+    assert err <= 1e-3
