@@ -221,6 +221,15 @@ class Engine:
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
         _check(self.lib.mse_init_probes(self.h, _dp(pos), len(pos), C.byref(probe)))
 
+    def set_focus_spread(self, delta_c10=None):
+        """Per-probe focus changes (Angstrom) added to C10 by later probe set-ups; None clears."""
+        d = np.zeros(0) if delta_c10 is None else np.ascontiguousarray(delta_c10, dtype=np.float64)
+        _check(self.lib.mse_set_focus_spread(self.h, _dp(d), len(d)))
+
+    def set_reference_offset(self, config: int, dx: float, dy: float):
+        """Shift the potential grid origin of `config` (source-size incoherence)."""
+        _check(self.lib.mse_set_reference_offset(self.h, config, dx, dy))
+
     def init_probes_range(self, first_wave: int, positions, probe: MseProbe):
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
         _check(self.lib.mse_init_probes_range(self.h, first_wave, _dp(pos), len(pos), C.byref(probe)))

@@ -106,7 +106,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         e->adj_w.ensure(plane);
         HIPX(hipMemcpyAsync(e->adj_w.p, dl_di, plane * sizeof(double), hipMemcpyHostToDevice, e->stream));
         e->adj_phase.ensure((size_t)std::max(n, 1) * N);
-        const mse::PotArgs pa = pot_args(e, g);
+        const mse::PotArgs pa = pot_args(e, c);
         for (int s0 = 0; s0 < n; s0 += 16) {
             const int cs = std::min(16, n - s0);
             dim3 grid(e->res / 16, e->res / 16, cs);

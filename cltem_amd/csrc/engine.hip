@@ -150,12 +150,13 @@ void upload_cfg_ptrs(mse_engine* e) {
     }
 }
 
-mse::PotArgs pot_args(mse_engine* e, const mse_geometry& g) {
+mse::PotArgs pot_args(mse_engine* e, const ConfigData& c) {
+    const mse_geometry& g = cfg_geom(e, c);
     mse::PotArgs a{};
     a.res = e->res;
     a.pixelscale = g.real_scale;
-    a.startx = g.sim_x[0];
-    a.starty = g.sim_y[0];
+    a.startx = g.sim_x[0] + c.ref_off[0];  // reference_perturb (simulationgeneral.cpp:559-560)
+    a.starty = g.sim_y[0] + c.ref_off[1];
     a.min_x = g.full_x[0];
     a.max_x = g.full_x[1];
     a.min_y = g.full_y[0];
@@ -528,7 +529,7 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         const bool fresh = c.trans.p == nullptr;
         c.trans.ensure((size_t)g.n_slices * e->N);
         if (fresh) upload_cfg_ptrs(e);
-        const mse::PotArgs a = pot_args(e, g);
+        const mse::PotArgs a = pot_args(e, c);
         float2* outp = c.trans.p + (size_t)first * e->N;
         // potential kernels chunk the slice range so one launch stays short
         const int chunk = g.full3d ? 1 : 16;
@@ -657,16 +658,30 @@ mse::ProbeArgs probe_args(mse_engine* e, const mse_probe* p) {
     a.C54 = make_double2(p->C54[0], p->C54[1]);
     a.C56 = make_double2(p->C56[0], p->C56[1]);
     a.taper = p->edge_taper;
+    a.dC10 = e->h_focus.empty() ? nullptr : e->probe_df.p;
     return a;
 }
 
-void upload_positions(mse_engine* e, const double* pos_xy, int n) {
+// positions of n probes (relative to the padded start) into waves first_wave.., and, when
+// focus changes are set, those of probes first_probe.. of the caller's list
+void upload_positions(mse_engine* e, const double* pos_xy, int n, int first_wave = 0, int first_probe = 0) {
     std::vector<double> rel(2 * (size_t)n);
     for (int i = 0; i < n; ++i) {
         rel[2 * i] = pos_xy[2 * i] - e->g.sim_x[0];  // simulationcbed.cpp:54-59
         rel[2 * i + 1] = pos_xy[2 * i + 1] - e->g.sim_y[0];
     }
-    HIPX(hipMemcpyAsync(e->probe_pos.p, rel.data(), rel.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPX(hipMemcpyAsync(e->probe_pos.p + 2 * (size_t)first_wave, rel.data(), rel.size() * sizeof(double),
+                        hipMemcpyHostToDevice, e->stream));
+    if (!e->h_focus.empty()) {
+        REQUIRE((size_t)first_probe + n <= e->h_focus.size(), MSE_ERR_ARG,
+                "fewer focus changes than probes (mse_set_focus_spread)");
+        if (e->probe_df.n < (size_t)first_wave + n) {
+            HIPX(hipStreamSynchronize(e->stream));
+            e->probe_df.ensure(std::max<size_t>((size_t)first_wave + n, e->n_waves));
+        }
+        HIPX(hipMemcpyAsync(e->probe_df.p + first_wave, e->h_focus.data() + first_probe, n * sizeof(double),
+                            hipMemcpyHostToDevice, e->stream));
+    }
 }
 
 void probes_into_waves(mse_engine* e, int n, const mse_probe* p) {
@@ -760,7 +775,7 @@ void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_o
 // memory and are updated (outside the graph) before each replay.
 struct StemGraphKey {
     int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices;
-    const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos;
+    const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos, *probe_df;
     mse_probe probe;
 };
 }  // namespace mse_rt
@@ -973,6 +988,13 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.det_bits = e->det_bits.p;
             k.trans = c.trans.p;
             k.probe_pos = e->probe_pos.p;
+            if (!e->h_focus.empty()) {  // the device array the captured probe kernel reads
+                if (e->probe_df.n < (size_t)batch) {
+                    HIPX(hipStreamSynchronize(e->stream));
+                    e->probe_df.ensure(batch);
+                }
+                k.probe_df = e->probe_df.p;
+            }
             k.probe = *probe;
             std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
             active_waves(e, batch);  // host-side plan (no stream work may happen inside the capture)
@@ -998,7 +1020,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         for (int p0 = 0; p0 < n_probes; p0 += batch) offsets.push_back((long long)p0 * n_det);
         for (int p0 = 0, bi = 0; p0 < n_probes; p0 += batch, ++bi) {
             const int nb = std::min(batch, n_probes - p0);
-            upload_positions(e, pos_xy + 2 * (size_t)p0, nb);
+            upload_positions(e, pos_xy + 2 * (size_t)p0, nb, 0, p0);
             if (graph && nb == batch) {
                 HIPX(hipMemcpyAsync(e->dev_offset.p, &offsets[bi], sizeof(long long), hipMemcpyHostToDevice,
                                     e->stream));
@@ -1062,21 +1084,36 @@ int mse_set_config_geometry(mse_engine* e, int cfg, const mse_geometry* g) {
     });
 }
 
+int mse_set_focus_spread(mse_engine* e, const double* delta_c10, int n) {
+    return run_guarded("mse_set_focus_spread", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(n >= 0 && (n == 0 || delta_c10), MSE_ERR_ARG, "NULL focus changes");
+        for (int i = 0; i < n; ++i) REQUIRE(std::isfinite(delta_c10[i]), MSE_ERR_ARG, "focus change not finite");
+        HIPX(hipStreamSynchronize(e->stream));  // earlier uploads read the previous list
+        e->h_focus.assign(delta_c10, delta_c10 + n);
+    });
+}
+
+int mse_set_reference_offset(mse_engine* e, int cfg, double dx, double dy) {
+    return run_guarded("mse_set_reference_offset", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(std::isfinite(dx) && std::isfinite(dy), MSE_ERR_ARG, "offset not finite");
+        ConfigData& c = config(e, cfg);
+        c.ref_off[0] = dx;
+        c.ref_off[1] = dy;
+    });
+}
+
 int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, int n, const mse_probe* probe) {
     return run_guarded("mse_init_probes_range", e, [&]() {
         REQUIRE(e && pos_xy && probe, MSE_ERR_ARG, "NULL argument");
         check_waves(e);
         REQUIRE(first_wave >= 0 && n >= 1 && first_wave + n <= e->n_waves, MSE_ERR_ARG, "wave range out of bounds");
-        std::vector<double> rel(2 * (size_t)n);
-        for (int i = 0; i < n; ++i) {
-            rel[2 * i] = pos_xy[2 * i] - e->g.sim_x[0];  // simulationcbed.cpp:54-59
-            rel[2 * i + 1] = pos_xy[2 * i + 1] - e->g.sim_y[0];
-        }
-        HIPX(hipMemcpyAsync(e->probe_pos.p + 2 * (size_t)first_wave, rel.data(), rel.size() * sizeof(double),
-                            hipMemcpyHostToDevice, e->stream));
+        upload_positions(e, pos_xy, n, first_wave, 0);
         mse::ColArgs ca = base_col_args(e);
         ca.probe = probe_args(e, probe);
         ca.probe.pos = e->probe_pos.p + 2 * (size_t)first_wave;
+        if (ca.probe.dC10) ca.probe.dC10 += first_wave;
         float2* w = e->waves.p + (size_t)first_wave * e->N;
         col_pass(e, mse::COL_PROBE, w, w, n, e->N, ca, PK_PROBE);
         HIPX(hipStreamSynchronize(e->stream));

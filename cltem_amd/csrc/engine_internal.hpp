@@ -83,6 +83,7 @@ struct ConfigData {
     DevBuf<int> orig;       // input index of each sorted atom
     int n_input = 0;        // atoms passed to the last sort (in range or not)
     unsigned version = 0;   // bumped whenever the atoms or transmission functions change
+    double ref_off[2] = {0.0, 0.0};  // potential grid origin shift (source-size incoherence)
     int n_in_range = 0;
     bool sorted = false;
     DevBuf<float2> trans;  // n_slices * N
@@ -119,6 +120,8 @@ struct mse_engine {
     DevBuf<int> wave_cfg;
     std::vector<int> h_wave_cfg;
     DevBuf<double> probe_pos;
+    DevBuf<double> probe_df;            // per wave: focus change of the probe (chromatic spread)
+    std::vector<double> h_focus;        // per probe of the next probe set-ups (mse_set_focus_spread)
     DevBuf<unsigned> det_bits;
     std::vector<double> det_key;  // detectors the bitmasks were built for
     DevBuf<double> partials;
@@ -190,7 +193,7 @@ inline void launch(mse_engine* e, int kind, F&& f) {
 
 // ---- helpers defined in engine.hip ---------------------------------------------------------
 // potential-kernel arguments for a configuration's geometry
-mse::PotArgs pot_args(mse_engine* e, const mse_geometry& g);
+mse::PotArgs pot_args(mse_engine* e, const ConfigData& c);
 const mse_geometry& cfg_geom(mse_engine* e, const ConfigData& c);
 ConfigData& config(mse_engine* e, int c);
 void require_geom(mse_engine* e);

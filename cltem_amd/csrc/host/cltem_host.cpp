@@ -136,6 +136,46 @@ AtomList displaced(const AtomList& a, double u_rms, uint64_t seed, int config) {
     return o;
 }
 
+// ---- incoherent effects -----------------------------------------------------------------
+static void uniforms4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double u[4]) {
+    const uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {c0, c1, c2, c3};
+    uint32_t r[4];
+    philox4x32_10(ctr, key, r);
+    for (int k = 0; k < 4; ++k) u[k] = ((double)r[k] + 0.5) / 4294967296.0;
+}
+
+void ChromaticSpread::setHalfWidthHalfMaxs(double hwhm_neg, double hwhm_pos) {
+    const double f = 2.0 * std::sqrt(2.0 * std::log(2.0));
+    sigma_neg = std::fabs(2.0 * hwhm_neg / f);
+    sigma_pos = std::fabs(2.0 * hwhm_pos / f);
+}
+
+void SourceSize::setFullWidthHalfMax(double fwhm) { sigma = fwhm / (2.0 * std::sqrt(2.0 * std::log(2.0))); }
+
+double SimulationManager::focusChange(int job, int probe) const {
+    if (!chromatic.enabled) return 0.0;
+    double u[4];
+    uniforms4(seed, (uint32_t)job, 0xC0u, (uint32_t)probe, 1u, u);
+    // generateDist: a selector picks the negative or positive half-Gaussian in proportion
+    // sigma_neg : sigma_pos; ratio = (n/p) / (n/p + 1) is written n / (n + p)
+    const double tot = chromatic.sigma_neg + chromatic.sigma_pos;
+    const double ratio = tot > 0.0 ? chromatic.sigma_neg / tot : 0.0;
+    const double mag = std::fabs(std::sqrt(-2.0 * std::log(u[1])) * std::cos(2.0 * M_PI * u[2]));
+    const double de = u[0] >= ratio ? mag * chromatic.sigma_pos : -mag * chromatic.sigma_neg;
+    return 10000.0 * chromatic.cc_mm * de / scenario.voltage_kv;
+}
+
+void SimulationManager::sourceOffset(int job, double* dx, double* dy) const {
+    *dx = *dy = 0.0;
+    if (!source.enabled) return;
+    double u[4];
+    uniforms4(seed, (uint32_t)job, 0x50u, 0u, 2u, u);
+    const double r = std::sqrt(-2.0 * std::log(u[0]));
+    *dx = r * std::cos(2.0 * M_PI * u[1]) * source.sigma;
+    *dy = r * std::sin(2.0 * M_PI * u[1]) * source.sigma;
+}
+
 // ---- SimulationManager ------------------------------------------------------------------
 void SimulationManager::updateImages(std::map<std::string, Image>& ims, int jobCount) {
     std::lock_guard<std::mutex> lck(image_update_mutex);
@@ -235,6 +275,8 @@ bool SimulationGeneral::initialiseSimulation() {
     load_params(sm->params_file, &form, &per, &values);
     check(mse_set_parameterisation(eng, form, per, values.data(), (int)values.size()));
     sortAtoms();
+    // simulationgeneral.cpp:559-560: the potential grid origin carries the source-size offset
+    check(mse_set_reference_offset(eng, 0, reference_perturb_x, reference_perturb_y));
     check(mse_build_transmission(eng, 0, 0, geom.n_slices));
     return true;
 }
@@ -268,6 +310,7 @@ std::vector<double> SimulationGeneral::getExitWaveImage(unsigned t, unsigned l, 
 
 // ---- CTEM -------------------------------------------------------------------------------
 bool SimulationCtem::initialiseSimulation() {
+    reference_perturb_x = reference_perturb_y = 0.0;  // simulationctem.cpp:59-60
     SimulationGeneral::initialiseSimulation();
     check(mse_set_wave_count(eng, 1));
     const int c0 = 0;
@@ -293,14 +336,21 @@ void SimulationCtem::simulate() {
 }
 
 // ---- CBED -------------------------------------------------------------------------------
+bool SimulationCbed::initialiseSimulation() {
+    sm->sourceOffset(config, &reference_perturb_x, &reference_perturb_y);
+    return SimulationGeneral::initialiseSimulation();
+}
+
 void SimulationCbed::initialiseProbeWave(double posx, double posy, int n_parallel) {
     if (n_parallel != 0) throw std::runtime_error("this adapter initialises probes from wave 0");
     const double pos[2] = {posx, posy};
+    const double df = sm->focusChange(config, 0);  // simulationcbed.cpp:62-71
+    check(mse_set_focus_spread(eng, &df, sm->chromatic.enabled ? 1 : 0));
     check(mse_init_probes(eng, pos, 1, &sm->probe));
 }
 
 void SimulationCbed::simulate() {
-    SimulationGeneral::initialiseSimulation();
+    initialiseSimulation();
     check(mse_set_wave_count(eng, 1));
     const int c0 = 0;
     check(mse_set_wave_configs(eng, &c0, 1));
@@ -327,7 +377,7 @@ double SimulationStem::getStemPixel(double inner, double outer, double xc, doubl
 }
 
 void SimulationStem::simulate() {
-    SimulationGeneral::initialiseSimulation();
+    SimulationCbed::initialiseSimulation();
     const int nx = geom.stem_pixels[0], ny = geom.stem_pixels[1];
     if (pixels.empty())
         for (int p = 0; p < nx * ny; ++p) pixels.push_back(p);
@@ -343,6 +393,10 @@ void SimulationStem::simulate() {
     for (int d = 0; d < nd; ++d) dets[d] = sm->detectors[d].d;
     std::vector<double> acc(pixels.size() * nd, 0.0);
     const int batch = std::max(1, std::min(sm->parallel_pixels, (int)pixels.size()));
+    // every probe draws its own focus change (initialiseProbeWave is called per probe)
+    std::vector<double> df(sm->chromatic.enabled ? pixels.size() : 0);
+    for (size_t i = 0; i < df.size(); ++i) df[i] = sm->focusChange(config, pixels[i]);
+    check(mse_set_focus_spread(eng, df.data(), (int)df.size()));
     check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
                          sm->sqrt2 ? 1 : 0, 1, acc.data()));
     n_waves = batch;

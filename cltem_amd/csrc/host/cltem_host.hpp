@@ -63,6 +63,22 @@ struct Detector {
     mse_detector d;
 };
 
+// Incoherent effects drawn once per job (incoherence/chromaticaberration.cpp,
+// incoherence/probesourcesize.cpp), from Philox4x32-10 keyed by the manager's seed instead of
+// clTEM's clock-seeded generators, so a run can be repeated.
+struct ChromaticSpread {
+    bool enabled = false;
+    double cc_mm = 0.0;                      // chromatic aberration coefficient
+    double sigma_neg = 0.0, sigma_pos = 0.0; // energy spread (eV) below / above the mean
+    // setHalfWidthHalfMaxs (chromaticaberration.cpp:108-120): sigma = 2 hwhm / (2 sqrt(2 ln 2))
+    void setHalfWidthHalfMaxs(double hwhm_neg, double hwhm_pos);
+};
+struct SourceSize {
+    bool enabled = false;
+    double sigma = 0.0;                      // Angstrom
+    void setFullWidthHalfMax(double fwhm);   // probesourcesize.cpp:17-19
+};
+
 // The inputs clTEM's SimulationManager holds for this path, and its result merging.
 struct SimulationManager {
     mse_scenario scenario{};   // mode, resolution, voltage, slicing, padding, areas, tilt
@@ -76,6 +92,14 @@ struct SimulationManager {
     uint64_t seed = 20261202;
     bool sqrt2 = true;         // sum_reduction's sqrt(2) factor
     int device = 0;
+    ChromaticSpread chromatic;
+    SourceSize source;
+
+    // ChromaticAberration::getFocusChange for probe `probe` of job `job` (Angstrom): an
+    // asymmetric Gaussian in energy, 10000 * Cc[mm] * dE / kV (chromaticaberration.cpp:43-58, 99-106)
+    double focusChange(int job, int probe) const;
+    // ProbeSourceSize::getOffset for x and y of job `job` (Angstrom)
+    void sourceOffset(int job, double* dx, double* dy) const;
 
     // SimulationManager::updateImages: FP64 sum of every job's images and weights
     void updateImages(std::map<std::string, Image>& ims, int jobCount);
@@ -112,6 +136,7 @@ class SimulationGeneral {
     mse_engine* eng = nullptr;
     std::shared_ptr<SimulationManager> sm;
     int config = 0;
+    double reference_perturb_x = 0.0, reference_perturb_y = 0.0;  // source-size offset of the job
     std::vector<int> pixels;
     mse_geometry geom{};
     int n_waves = 0;
@@ -130,6 +155,7 @@ class SimulationCbed : public SimulationGeneral {
     using SimulationGeneral::SimulationGeneral;
     void simulate() override;  // simulationcbed.cpp:122-210
    protected:
+    bool initialiseSimulation();  // draws the source-size offset (simulationcbed.cpp:102-118)
     void initialiseProbeWave(double posx, double posy, int n_parallel = 0);
 };
 

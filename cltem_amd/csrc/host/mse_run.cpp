@@ -4,9 +4,12 @@
 //           (--xyz file.xyz | --srtio3 NX,NY,NZ) --res 512 --dz 1.9525 --kv 200
 //           [--area x0,x1,y0,y1] [--stem-pixels N] [--det name,inner,outer[,xc,yc]]...
 //           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
-//           [--full3d integrals] [--device D] --out PREFIX
+//           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
+//           [--device D] --out PREFIX
 //   mse_run ... --dump-atoms FILE [--dump-config C]   (writes the displaced atoms of phonon
 //           configuration C as x,y,z doubles and Z ints, and exits without using the GPU)
+//   mse_run ... --dump-incoherence N [--dump-config C]   (prints the focus changes of probes
+//           0..N-1 and the source offset of job C, one number per line, and exits)
 //
 // Writes PREFIX_<image>.bin (doubles, averaged over the phonon configurations as
 // SimulationManager::updateImages's weighting does) and PREFIX.json describing them.
@@ -47,6 +50,7 @@ int main(int argc, char** argv) {
         std::string out = "mse_out";
         std::string dump;
         int dump_config = 0;
+        int dump_incoherence = -1;
         bool have_area = false;
         int stem_px = 64;
         for (int i = 1; i < argc; ++i) {
@@ -102,17 +106,35 @@ int main(int argc, char** argv) {
             } else if (a == "--full3d") {
                 s.full3d = 1;
                 s.full3d_integrals = std::stoi(next());
+            } else if (a == "--chromatic") {
+                auto v = split_doubles(next());
+                sm->chromatic.enabled = true;
+                sm->chromatic.cc_mm = v.at(0);
+                sm->chromatic.setHalfWidthHalfMaxs(v.at(1), v.at(2));
+            } else if (a == "--source-size") {
+                sm->source.enabled = true;
+                sm->source.setFullWidthHalfMax(std::stod(next()));
             } else if (a == "--device") {
                 sm->device = std::stoi(next());
             } else if (a == "--out") {
                 out = next();
             } else if (a == "--dump-atoms") {
                 dump = next();
+            } else if (a == "--dump-incoherence") {
+                dump_incoherence = std::stoi(next());
             } else if (a == "--dump-config") {
                 dump_config = std::stoi(next());
             } else {
                 throw std::runtime_error("unknown option " + a);
             }
+        }
+        if (dump_incoherence >= 0) {
+            std::cout.precision(17);
+            for (int i = 0; i < dump_incoherence; ++i) std::cout << sm->focusChange(dump_config, i) << "\n";
+            double dx, dy;
+            sm->sourceOffset(dump_config, &dx, &dy);
+            std::cout << dx << "\n" << dy << std::endl;
+            return 0;
         }
         if (sm->atoms.size() == 0) throw std::runtime_error("no structure: use --xyz or --srtio3");
         if (!dump.empty()) {

@@ -67,6 +67,7 @@ struct ProbeArgs {
     double2 C12, C21, C23, C32, C34, C41, C43, C45, C52, C54, C56;
     int taper;
     double beta2, delta;        // COL_CTEM_IMAGE: illumination semi-angle (1/Angstrom), defocus spread
+    const double* dC10;         // per wave: focus change added to C10 (chromatic spread), or NULL
 };
 
 struct ColArgs {
@@ -246,7 +247,7 @@ __device__ __forceinline__ float2 ctem_transfer(const ProbeArgs& p, double kx, d
 
 // init_probe_wave (kernels/init_probe_wave_f.cl:49-118) for one k-space pixel, in double.
 __device__ __forceinline__ float2 probe_value(const ProbeArgs& p, double kx, double ky, double px,
-                                              double py) {
+                                              double py, double dC10) {
     const double k = sqrt(kx * kx + ky * ky);
     if (!(k < p.cond_ap2 + p.smooth_r)) return make_float2(0.f, 0.f);
     const double lam = p.wavelength;
@@ -254,7 +255,7 @@ __device__ __forceinline__ float2 probe_value(const ProbeArgs& p, double kx, dou
     const double2 w = make_double2(lam * kx, lam * ky);
     const double2 wc = make_double2(w.x, -w.y);
     const double w2 = w.x * w.x + w.y * w.y;
-    double cchi = 0.5 * p.C10 * w2;
+    double cchi = 0.5 * (p.C10 + dC10) * w2;
     cchi += 0.5 * dmul(p.C12, dpow(wc, 2)).x;
     cchi += dmul(p.C21, dmul(dpow(wc, 2), w)).x / 3.0;
     cchi += dmul(p.C23, dpow(wc, 3)).x / 3.0;
@@ -324,8 +325,9 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     if constexpr (MODE == COL_PROBE) {
         const double kxv = a.kx[x];
         const double ppx = a.probe.pos[2 * wave], ppy = a.probe.pos[2 * wave + 1];
+        const double df = a.probe.dC10 ? a.probe.dC10[wave] : 0.0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = probe_value(a.probe, kxv, a.ky[t + L * m], ppx, ppy);
+        for (int m = 0; m < E; ++m) v[m] = probe_value(a.probe, kxv, a.ky[t + L * m], ppx, ppy, df);
     } else {
         // coalesced load of the TC x N tile: TC consecutive complex per row, 16 bytes a lane
         const float4* in4 = reinterpret_cast<const float4*>(in + wbase + x0);

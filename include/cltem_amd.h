@@ -166,6 +166,17 @@ int mse_init_plane_wave(mse_engine* e, double value);
 /* SimulationCbed::initialiseProbeWave for waves 0..n-1 at absolute positions pos_xy[2*i]
  * (Angstrom), made relative to the padded simulation start (simulationcbed.cpp:54-59) */
 int mse_init_probes(mse_engine* e, const double* pos_xy, int n, const mse_probe* probe);
+/* Chromatic focus spread (ChromaticAberration::getFocusChange, incoherence/chromaticaberration.cpp,
+ * added to C10 in SimulationCbed::initialiseProbeWave, microscope/simulationcbed.cpp:62-71):
+ * delta_c10[i] (Angstrom) is added to C10 for the i-th probe position of each later
+ * mse_init_probes / mse_init_probes_range / mse_stem_batch call, so every probe of a batch
+ * can carry its own focus.  n = 0 clears the list. */
+int mse_set_focus_spread(mse_engine* e, const double* delta_c10, int n);
+/* Source-size offset (SimulationCbed::initialiseSimulation drawing reference_perturb_x/y,
+ * microscope/simulationcbed.cpp:104-118, applied to the potential grid origin at
+ * simulationgeneral.cpp:559-560): the transmission functions of configuration cfg built after
+ * this call sample the potential on a grid shifted by (dx, dy) Angstrom. */
+int mse_set_reference_offset(mse_engine* e, int cfg, double dx, double dy);
 /* the same for waves first_wave..first_wave+n-1 (probes of different convergence per wave) */
 int mse_init_probes_range(mse_engine* e, int first_wave, const double* pos_xy, int n, const mse_probe* probe);
 /* SimulationGeneral::doMultiSliceStep for slices [first, first+count) on every wave

@@ -38,6 +38,28 @@ def test_cpp_phonon_displacements_match_python(tmp_path):
 
 
 @pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cpp_incoherence_draws_match_python():
+    from cltem_amd import incoherence as inc
+    for job in (0, 2):
+        r = subprocess.run([RUN, "--chromatic", "1.4,0.4,0.6", "--source-size", "0.8", "--kv", "300", "--seed", "7",
+                            "--dump-incoherence", "6", "--dump-config", str(job)],
+                           capture_output=True, text=True, check=True)
+        vals = np.array([float(v) for v in r.stdout.split()])
+        sn, spos = inc.sigmas_from_hwhm(0.4, 0.6)
+        df = inc.focus_changes(7, job, np.arange(6), 1.4, sn, spos, 300.0)
+        dx, dy = inc.source_offset(7, job, inc.source_sigma_from_fwhm(0.8))
+        np.testing.assert_allclose(vals[:6], df, rtol=1e-12)
+        np.testing.assert_allclose(vals[6:], [dx, dy], rtol=1e-12)
+    # both halves of the asymmetric spread occur, scaled by their own sigma
+    df = inc.focus_changes(7, 0, np.arange(4000), 1.4, sn, spos, 300.0)
+    frac_neg = np.mean(df < 0)
+    assert abs(frac_neg - sn / (sn + spos)) < 0.03
+    scale = 10000 * 1.4 / 300.0
+    assert abs(np.mean(-df[df < 0]) / scale - sn * np.sqrt(2 / np.pi)) < 0.05 * sn
+    assert abs(np.mean(df[df > 0]) / scale - spos * np.sqrt(2 / np.pi)) < 0.05 * spos
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
 def test_driver_rejects_bad_input(tmp_path):
     r = subprocess.run([RUN, "--mode", "ctem"], capture_output=True, text=True)
     assert r.returncode == 1 and "no structure" in r.stderr
@@ -127,3 +149,56 @@ def test_driver_cbed_matches_oracle(tmp_path, oracle):
     ref = oracle.diffraction(recip)
     got = out["Diff"].reshape(256, 256)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_cbed_chromatic_and_source_size_match_oracle(tmp_path, oracle):
+    """CBED averaged over 3 jobs, each with its own focus change and source-size offset of
+    the potential grid, against FP64 oracle runs with the same per-job draws."""
+    import ctypes
+    from cltem_amd import incoherence as inc
+    prefix = str(tmp_path / "cbed_inc")
+    cx = 1.5 * 3.905
+    seed = 11
+    subprocess.run([RUN, "--mode", "cbed", "--srtio3", "3,3,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{cx},{cx},{cx},{cx}", "--alpha", "15", "--phonons", "3", "--urms", "0",
+                    "--seed", str(seed), "--chromatic", "1.2,0.5,0.8", "--source-size", "1.0",
+                    "--params", os.path.join(PARAMS, "kirkland.dat"), "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    assert meta["jobs"] == 3
+    s = sp.srtio3(3, 3, 3)
+    lo, hi = s.limits
+    scen = dict(mode=3, resolution=256, slice_thickness=1.9525, struct_min=tuple(lo), struct_max=tuple(hi),
+                area=(cx, cx, cx, cx))
+    og = oracle.geometry(oracle.scenario(**scen))
+    p = cltem_amd.load_params("kirkland")
+    P_ = oracle.propagator(og)
+    sn, spos = inc.sigmas_from_hwhm(0.5, 0.8)
+    ref = np.zeros((256, 256))
+    for job in range(3):
+        d = s.displaced(0.0, seed, job)
+        dx, dy = inc.source_offset(seed, job, inc.source_sigma_from_fwhm(1.0))
+        og_t = type(og).from_buffer_copy(og)
+        og_t.sim_x[0] += dx
+        og_t.sim_y[0] += dy
+        srt = oracle.sort_atoms(og, d.x, d.y, d.z, d.Z)
+        tr = oracle.transmission(og_t, srt, p.values, 0, 3)
+        ab = np.zeros(25)
+        ab[0] = inc.focus_changes(seed, job, [0], 1.2, sn, spos, 200.0)[0]
+        psi = oracle.fft2(oracle.probe_kspace(og, cx - og.sim_x[0], cx - og.sim_y[0], ab, 15.0), inverse=True)
+        for sl in range(og.n_slices):
+            psi, recip = oracle.slice_step(psi, tr[sl], P_)
+        ref += oracle.diffraction(recip)
+    ref /= 3
+    got = out["Diff"].reshape(256, 256)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"CBED with focus spread and source size, 3 jobs: rel-L2 {err:.3e}")
+    assert err <= 1e-3
+    # the effects are visible: the same run without them differs
+    plain = str(tmp_path / "cbed_plain")
+    subprocess.run([RUN, "--mode", "cbed", "--srtio3", "3,3,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{cx},{cx},{cx},{cx}", "--alpha", "15", "--params",
+                    os.path.join(PARAMS, "kirkland.dat"), "--out", plain], check=True)
+    _, outp = _load(plain)
+    assert np.linalg.norm(outp["Diff"] - out["Diff"]) / np.linalg.norm(out["Diff"]) > 1e-3
