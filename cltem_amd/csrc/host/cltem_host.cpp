@@ -1,6 +1,8 @@
 // Host side of the engine in the shape of clTEM's simulation classes (see cltem_host.hpp).
 #include "cltem_host.hpp"
 
+#include <complex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -410,6 +412,120 @@ void SimulationStem::simulate() {
         ims[sm->detectors[d].name] = im;
     }
     sm->updateImages(ims, 1);
+}
+
+// ---- output writers -----------------------------------------------------------------------
+namespace {
+void put16(std::vector<unsigned char>& b, uint16_t v) {
+    b.push_back(v & 0xff);
+    b.push_back(v >> 8);
+}
+void put32(std::vector<unsigned char>& b, uint32_t v) {
+    for (int k = 0; k < 4; ++k) b.push_back((v >> (8 * k)) & 0xff);
+}
+}  // namespace
+
+void save_tiff(const std::string& path, const std::vector<double>& data, int width, int height) {
+    if ((size_t)width * height != data.size())
+        throw std::runtime_error("Attempting to save image with incommensurate data size and image dimensions");
+    // little-endian baseline TIFF: header, pixel strip, then the IFD
+    std::vector<unsigned char> b;
+    b.push_back('I');
+    b.push_back('I');
+    put16(b, 42);
+    const uint32_t strip_off = 8, strip_bytes = (uint32_t)(data.size() * sizeof(float));
+    const uint32_t ifd_off = strip_off + strip_bytes + (strip_bytes & 1);
+    put32(b, ifd_off);
+    for (double v : data) {
+        const float f = (float)v;
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        put32(b, u);
+    }
+    if (strip_bytes & 1) b.push_back(0);
+    struct Tag { uint16_t id, type; uint32_t count, value; };
+    const Tag tags[] = {
+        {256, 4, 1, (uint32_t)width},   // ImageWidth
+        {257, 4, 1, (uint32_t)height},  // ImageLength
+        {258, 3, 1, 32},                // BitsPerSample
+        {259, 3, 1, 1},                 // Compression: none
+        {262, 3, 1, 1},                 // Photometric: min-is-black
+        {273, 4, 1, strip_off},         // StripOffsets
+        {274, 3, 1, 4},                 // Orientation: bottom-left
+        {277, 3, 1, 1},                 // SamplesPerPixel
+        {278, 4, 1, (uint32_t)height},  // RowsPerStrip
+        {279, 4, 1, strip_bytes},       // StripByteCounts
+        {284, 3, 1, 1},                 // PlanarConfiguration: contiguous
+        {339, 3, 1, 3},                 // SampleFormat: IEEE float
+    };
+    put16(b, (uint16_t)(sizeof(tags) / sizeof(tags[0])));
+    for (const Tag& t : tags) {
+        put16(b, t.id);
+        put16(b, t.type);
+        put32(b, t.count);
+        if (t.type == 3) {  // SHORT values sit left-justified in the value field
+            put16(b, (uint16_t)t.value);
+            put16(b, 0);
+        } else {
+            put32(b, t.value);
+        }
+    }
+    put32(b, 0);  // no further IFD
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("Could not open .tif file for saving: " + path);
+    f.write((const char*)b.data(), (std::streamsize)b.size());
+    if (!f) throw std::runtime_error("Unable to write data to .tif file");
+}
+
+static std::string settings_json(const SimulationManager& sm, const std::string& image) {
+    const mse_scenario& s = sm.scenario;
+    std::ostringstream js;
+    js.precision(17);
+    const char* modes[] = {"None", "CTEM", "STEM", "CBED"};
+    js << "{\n  \"image\": \"" << image << "\",\n  \"mode\": \"" << modes[std::min(std::max(s.mode, 0), 3)]
+       << "\",\n  \"resolution\": " << s.resolution << ",\n  \"slice thickness\": " << s.slice_thickness
+       << ",\n  \"slice offset\": " << s.slice_offset << ",\n  \"microscope\": {\"voltage\": " << s.voltage_kv
+       << ", \"beam tilt\": " << s.beam_tilt_mrad << ", \"beam azimuth\": " << s.beam_azimuth
+       << ", \"condenser aperture\": " << sm.probe.cond_ap_mrad << "},\n  \"full 3d\": {\"state\": "
+       << (s.full3d ? "true" : "false") << ", \"integrals\": " << s.full3d_integrals
+       << "},\n  \"potentials\": \"" << sm.params_file << "\",\n  \"frozen phonon\": {\"iterations\": "
+       << sm.phonon_configs << ", \"u_rms\": " << sm.u_rms << ", \"seed\": " << sm.seed
+       << "},\n  \"incoherence\": {\"chromatic\": {\"state\": " << (sm.chromatic.enabled ? "true" : "false")
+       << ", \"Cc\": " << sm.chromatic.cc_mm << ", \"sigma negative\": " << sm.chromatic.sigma_neg
+       << ", \"sigma positive\": " << sm.chromatic.sigma_pos << "}, \"source size\": {\"state\": "
+       << (sm.source.enabled ? "true" : "false") << ", \"sigma\": " << sm.source.sigma << "}}";
+    for (const auto& d : sm.detectors)
+        if (d.name == image)
+            js << ",\n  \"stem\": {\"detectors\": {\"" << d.name << "\": {\"inner\": " << d.d.inner_mrad
+               << ", \"outer\": " << d.d.outer_mrad << ", \"centre x\": " << d.d.xc_mrad << ", \"centre y\": "
+               << d.d.yc_mrad << "}}}";
+    js << ",\n  \"jobs\": " << sm.complete_jobs << "\n}\n";
+    return js.str();
+}
+
+void save_outputs(const SimulationManager& sm, const std::string& prefix) {
+    for (const auto& kv : sm.averaged()) {
+        const std::string& name = kv.first;
+        const Image& im = kv.second;
+        const size_t npix = (size_t)im.width * im.height;
+        auto write = [&](const std::string& base, auto value) {
+            std::ofstream(base + ".json") << settings_json(sm, name);
+            for (size_t j = 0; j < im.slices.size(); ++j) {
+                std::vector<double> d(npix);
+                for (size_t k = 0; k < npix; ++k) d[k] = value(im.slices[j], k);
+                const std::string fn = im.slices.size() > 1 ? base + "_" + std::to_string(j) + ".tif" : base + ".tif";
+                save_tiff(fn, d, im.width, im.height);
+            }
+        };
+        if (name == "EW") {  // complex: amplitude and phase
+            write(prefix + "_" + name + "_amplitude",
+                  [](const std::vector<double>& s, size_t k) { return std::abs(std::complex<double>(s[2 * k], s[2 * k + 1])); });
+            write(prefix + "_" + name + "_phase",
+                  [](const std::vector<double>& s, size_t k) { return std::arg(std::complex<double>(s[2 * k], s[2 * k + 1])); });
+        } else {
+            write(prefix + "_" + name, [](const std::vector<double>& s, size_t k) { return s[k]; });
+        }
+    }
 }
 
 void run_simulation(std::shared_ptr<SimulationManager> sm) {

@@ -167,6 +167,15 @@ class SimulationStem : public SimulationCbed {
     double getStemPixel(double inner, double outer, double xc, double yc, int parallel_ind);
 };
 
+// fileio::SaveTiff<float> (utilities/fileio.h:44-82): one uncompressed strip of 32-bit IEEE
+// floats, one sample per pixel, min-is-black, orientation bottom-left, rows in data order.
+void save_tiff(const std::string& path, const std::vector<double>& data, int width, int height);
+
+// The outputs of imageReturned (console/main.cpp:189-266): "EW" as <name>_amplitude and
+// <name>_phase, every other image as <name>; each a .tif (one per slice of a stack, suffixed
+// _<slice>) and a .json with the run's settings.
+void save_outputs(const SimulationManager& sm, const std::string& prefix);
+
 // Runs every job of a configured manager on one GPU (the role of SimulationRunner's
 // worker, threading/simulationrunner.cpp:52-131, for one device).
 void run_simulation(std::shared_ptr<SimulationManager> sm);

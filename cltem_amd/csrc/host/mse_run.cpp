@@ -5,14 +5,17 @@
 //           [--area x0,x1,y0,y1] [--stem-pixels N] [--det name,inner,outer[,xc,yc]]...
 //           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
 //           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
-//           [--device D] --out PREFIX
+//           [--device D] [--tiff] --out PREFIX
 //   mse_run ... --dump-atoms FILE [--dump-config C]   (writes the displaced atoms of phonon
 //           configuration C as x,y,z doubles and Z ints, and exits without using the GPU)
 //   mse_run ... --dump-incoherence N [--dump-config C]   (prints the focus changes of probes
 //           0..N-1 and the source offset of job C, one number per line, and exits)
+//   mse_run --write-test-tiff FILE W H   (writes the ramp x + 1000 y, W x H, and exits)
 //
 // Writes PREFIX_<image>.bin (doubles, averaged over the phonon configurations as
-// SimulationManager::updateImages's weighting does) and PREFIX.json describing them.
+// SimulationManager::updateImages's weighting does) and PREFIX.json describing them; with
+// --tiff also clTEM's outputs (32-bit float .tif + settings .json per image, EW as
+// amplitude and phase).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -51,6 +54,7 @@ int main(int argc, char** argv) {
         std::string dump;
         int dump_config = 0;
         int dump_incoherence = -1;
+        bool tiff = false;
         bool have_area = false;
         int stem_px = 64;
         for (int i = 1; i < argc; ++i) {
@@ -120,6 +124,16 @@ int main(int argc, char** argv) {
                 out = next();
             } else if (a == "--dump-atoms") {
                 dump = next();
+            } else if (a == "--write-test-tiff") {
+                const std::string fn = next();
+                const int w = std::stoi(next()), h = std::stoi(next());
+                std::vector<double> d((size_t)w * h);
+                for (int y = 0; y < h; ++y)
+                    for (int x = 0; x < w; ++x) d[(size_t)y * w + x] = x + 1000.0 * y;
+                save_tiff(fn, d, w, h);
+                return 0;
+            } else if (a == "--tiff") {
+                tiff = true;
             } else if (a == "--dump-incoherence") {
                 dump_incoherence = std::stoi(next());
             } else if (a == "--dump-config") {
@@ -177,6 +191,7 @@ int main(int argc, char** argv) {
             first = false;
         }
         js << ", \"jobs\": " << sm->complete_jobs << "}\n";
+        if (tiff) save_outputs(*sm, out);
         std::cout << "mse_run: " << sm->complete_jobs << " jobs done, results in " << out << ".json" << std::endl;
         return 0;
     } catch (const std::exception& ex) {

@@ -205,7 +205,7 @@ int mse_ctem_image(mse_engine* e, const mse_imaging* im, int accumulate, double*
  * microscope/simulationctem.cpp:174-263): image (N doubles, e.g. from mse_ctem_image) ->
  * FFT, x sqrt(DQE), IFFT (ccd_dqe, kernels/ccd_dqe_d.cl) -> conversion * Poisson(dose_per_pixel
  * * binning^2 * real part) -> FFT, x NTF, IFFT (ccd_ntf, kernels/ccd_ntf_d.cl) -> real part
- * into out.  dqe/ntf are n_table radial samples (clTEM's ccds/*.dat hold 725) at radius /
+ * into out.  dqe/ntf are n_table radial samples (the files of clTEM's ccds folder hold 725) at radius /
  * binning in pixels of the unshifted FFT, linearly interpolated, clamped to the last entry.
  * The noise comes from Philox4x32-10 keyed by seed (counter = pixel), so a seed reproduces
  * an image exactly; clTEM's generator is clock-seeded. */

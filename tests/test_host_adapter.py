@@ -202,3 +202,55 @@ def test_driver_cbed_chromatic_and_source_size_match_oracle(tmp_path, oracle):
                     os.path.join(PARAMS, "kirkland.dat"), "--out", plain], check=True)
     _, outp = _load(plain)
     assert np.linalg.norm(outp["Diff"] - out["Diff"]) / np.linalg.norm(out["Diff"]) > 1e-3
+
+
+def read_tiff(path):
+    """Minimal little-endian baseline TIFF reader: tags and the single float32 strip."""
+    import struct
+    b = open(path, "rb").read()
+    assert b[:4] == b"II*\x00"
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    n = struct.unpack_from("<H", b, ifd)[0]
+    tags = {}
+    for i in range(n):
+        tid, typ, cnt, val = struct.unpack_from("<HHII", b, ifd + 2 + 12 * i)
+        if typ == 3:
+            val &= 0xFFFF
+        tags[tid] = val
+    w, h = tags[256], tags[257]
+    data = np.frombuffer(b, dtype="<f4", count=w * h, offset=tags[273]).reshape(h, w)
+    return tags, data
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_tiff_writer(tmp_path):
+    f = str(tmp_path / "ramp.tif")
+    subprocess.run([RUN, "--write-test-tiff", f, "7", "5"], check=True)
+    tags, data = read_tiff(f)
+    # fileio::SaveTiff<float>: 32-bit IEEE float, one sample, one strip, no compression,
+    # min-is-black, bottom-left orientation, contiguous planes
+    assert tags[256] == 7 and tags[257] == 5 and tags[258] == 32 and tags[339] == 3
+    assert tags[277] == 1 and tags[278] == 5 and tags[279] == 7 * 5 * 4 and tags[259] == 1
+    assert tags[262] == 1 and tags[274] == 4 and tags[284] == 1
+    x = np.arange(7)[None, :]
+    y = np.arange(5)[:, None]
+    np.testing.assert_array_equal(data, (x + 1000.0 * y).astype(np.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_tiff_outputs(tmp_path):
+    """--tiff writes clTEM's outputs: EW as amplitude and phase, Diff, each with a settings
+    .json; the .tif values equal the FP64 results rounded to float32."""
+    prefix = str(tmp_path / "ctem")
+    subprocess.run([RUN, "--mode", "ctem", "--srtio3", "2,2,2", "--res", "256", "--dz", "1.9525", "--tiff",
+                    "--params", os.path.join(PARAMS, "kirkland.dat"), "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    ew = out["EW"].reshape(256, 256, 2)
+    amp = np.hypot(ew[..., 0], ew[..., 1])
+    ph = np.arctan2(ew[..., 1], ew[..., 0])
+    for name, want in (("EW_amplitude", amp), ("EW_phase", ph), ("Diff", out["Diff"].reshape(256, 256))):
+        tags, data = read_tiff(f"{prefix}_{name}.tif")
+        np.testing.assert_allclose(data, want.astype(np.float32), rtol=1e-6, atol=1e-6 * np.abs(want).max())
+        settings = json.load(open(f"{prefix}_{name}.json"))
+        assert settings["mode"] == "CTEM" and settings["resolution"] == 256
