@@ -93,8 +93,8 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         REQUIRE(e && dl_di && grad_xyz, MSE_ERR_ARG, "NULL argument");
         REQUIRE(e->adj_ready, MSE_ERR_STATE, "mse_forward_checkpointed has not been called");
         ConfigData& c = config(e, e->adj_cfg);
-        REQUIRE(c.version == e->adj_version, MSE_ERR_STATE,
-                "atoms or transmission functions changed since mse_forward_checkpointed");
+        REQUIRE(c.version == e->adj_version && c.sorted && c.built, MSE_ERR_STATE,
+                "atoms, geometry or transmission functions changed since mse_forward_checkpointed");
         REQUIRE(e->n_waves == e->adj_nw, MSE_ERR_STATE, "wave count changed since mse_forward_checkpointed");
         const mse_geometry& g = cfg_geom(e, c);
         REQUIRE(!g.full3d, MSE_ERR_UNSUPPORTED, "position gradients are implemented for projected potentials");

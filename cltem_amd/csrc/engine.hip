@@ -330,11 +330,17 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
             e->n_waves = 0;
             e->det_key.clear();
         }
-        for (auto& c : e->configs) {  // bins depend on the geometry
+        e->adj_ready = false;  // checkpoints belong to the previous geometry
+        for (auto& c : e->configs) {  // bins and transmission functions depend on the geometry
             c.sorted = false;
+            c.built = false;
+            ++c.version;
             if (c.own_geom) c.trans.release();
             c.own_geom = false;
         }
+        // the device table of transmission pointers and slice counts must not keep pointers
+        // released above or slice counts of the previous geometry
+        if (!e->configs.empty()) upload_cfg_ptrs(e);
         // k-grid exactly as the reference builds it (double)
         e->h_kx.assign(res, 0.0);
         e->h_ky.assign(res, 0.0);
@@ -526,9 +532,12 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         REQUIRE(first >= 0 && count >= 0 && first + count <= g.n_slices, MSE_ERR_ARG, "slice range out of bounds");
         if (count == 0) return;
         ++c.version;
-        const bool fresh = c.trans.p == nullptr;
+        // ensure() may move the buffer (a larger geometry than before): the device pointer
+        // table the sweeps read must then be refreshed, or they would read released memory
+        const float2* before = c.trans.p;
         c.trans.ensure((size_t)g.n_slices * e->N);
-        if (fresh) upload_cfg_ptrs(e);
+        c.built = true;
+        if (c.trans.p != before) upload_cfg_ptrs(e);
         const mse::PotArgs a = pot_args(e, c);
         float2* outp = c.trans.p + (size_t)first * e->N;
         // potential kernels chunk the slice range so one launch stays short
@@ -572,7 +581,7 @@ int mse_read_transmission(mse_engine* e, int cfg, int slice, float* out) {
         REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
         require_geom(e);
         ConfigData& c = config(e, cfg);
-        REQUIRE(c.trans.p, MSE_ERR_STATE, "no transmission functions for this configuration");
+        REQUIRE(c.trans.p && c.built, MSE_ERR_STATE, "no transmission functions for this configuration");
         REQUIRE(slice >= 0 && slice < cfg_geom(e, c).n_slices, MSE_ERR_ARG, "slice out of range");
         HIPX(hipMemcpyAsync(out, c.trans.p + (size_t)slice * e->N, e->N * sizeof(float2), hipMemcpyDeviceToHost,
                             e->stream));
@@ -632,8 +641,11 @@ void check_waves(mse_engine* e) {
 void check_trans(mse_engine* e) {
     for (int i = 0; i < e->n_waves; ++i) {
         const int c = e->h_wave_cfg[i];
-        REQUIRE(c >= 0 && c < (int)e->configs.size() && e->configs[c].trans.p, MSE_ERR_STATE,
-                "transmission functions of configuration " + std::to_string(c) + " are not built");
+        REQUIRE(c >= 0 && c < (int)e->configs.size(), MSE_ERR_STATE, "wave refers to an unknown configuration");
+        const ConfigData& cd = e->configs[c];
+        REQUIRE(cd.trans.p && cd.built && cd.trans.n >= (size_t)cfg_geom(e, cd).n_slices * e->N, MSE_ERR_STATE,
+                "transmission functions of configuration " + std::to_string(c) +
+                    " are not built for the current geometry");
     }
 }
 
@@ -776,6 +788,7 @@ void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_o
 struct StemGraphKey {
     int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices;
     const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos, *probe_df;
+    const void *cfg_ptrs, *cfg_nslices, *wave_cfg, *wave_map, *tw, *px, *py, *lim, *kx, *ky;
     mse_probe probe;
 };
 }  // namespace mse_rt
@@ -946,7 +959,8 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         REQUIRE(n_probes >= 1 && batch >= 1 && n_det >= 1, MSE_ERR_ARG, "counts must be positive");
         (void)use_graph;
         ConfigData& c = config(e, cfg);
-        REQUIRE(c.trans.p, MSE_ERR_STATE, "transmission functions of this configuration are not built");
+        REQUIRE(c.trans.p && c.built && c.trans.n >= (size_t)cfg_geom(e, c).n_slices * e->N, MSE_ERR_STATE,
+                "transmission functions of this configuration are not built for the current geometry");
         if (e->n_waves < batch || !e->waves.p) {
             HIPX(hipStreamSynchronize(e->stream));
             e->waves.ensure((size_t)batch * e->N);
@@ -988,6 +1002,16 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.det_bits = e->det_bits.p;
             k.trans = c.trans.p;
             k.probe_pos = e->probe_pos.p;
+            k.cfg_ptrs = e->cfg_ptrs.p;  // every device pointer the captured kernels read
+            k.cfg_nslices = e->cfg_nslices.p;
+            k.wave_cfg = e->wave_cfg.p;
+            k.wave_map = e->wave_map.p;
+            k.tw = e->tw.p;
+            k.px = e->px.p;
+            k.py = e->py.p;
+            k.lim = e->prop_lim.p;
+            k.kx = e->kx.p;
+            k.ky = e->ky.p;
             if (!e->h_focus.empty()) {  // the device array the captured probe kernel reads
                 if (e->probe_df.n < (size_t)batch) {
                     HIPX(hipStreamSynchronize(e->stream));
@@ -1077,7 +1101,9 @@ int mse_set_config_geometry(mse_engine* e, int cfg, const mse_geometry* g) {
         ConfigData& c = config(e, cfg);
         HIPX(hipStreamSynchronize(e->stream));
         c.trans.release();
+        c.built = false;
         c.sorted = false;
+        ++c.version;
         c.g = *g;
         c.own_geom = true;
         upload_cfg_ptrs(e);
@@ -1243,7 +1269,8 @@ int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, 
         require_geom(e);
         REQUIRE(n_waves >= 1 && reps >= 1, MSE_ERR_ARG, "n_waves and reps must be positive");
         ConfigData& c = config(e, cfg);
-        REQUIRE(c.trans.p, MSE_ERR_STATE, "transmission functions of this configuration are not built");
+        REQUIRE(c.trans.p && c.built && c.trans.n >= (size_t)cfg_geom(e, c).n_slices * e->N, MSE_ERR_STATE,
+                "transmission functions of this configuration are not built for the current geometry");
         if (e->n_waves < n_waves || !e->waves.p) {
             HIPX(hipStreamSynchronize(e->stream));
             e->waves.ensure((size_t)n_waves * e->N);

@@ -87,6 +87,7 @@ struct ConfigData {
     int n_in_range = 0;
     bool sorted = false;
     DevBuf<float2> trans;  // n_slices * N
+    bool built = false;     // trans holds the transmission functions of the current geometry
     bool own_geom = false;  // z-geometry of its own (a specimen of another thickness)
     mse_geometry g{};
 };

@@ -15,6 +15,36 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
 
+def test_rebuild_after_geometry_change(engine, oracle):
+    """A configuration rebuilt under a geometry with more slices gets a larger buffer; the
+    sweeps must read the new one (regression: the device pointer table kept the old one).
+    Waves of a configuration not rebuilt after a geometry change are refused."""
+    p = cltem_amd.load_params("kirkland")
+    for nz in (3, 6):  # 11 then 17 slices
+        w = wl.ctem_srtio3(3, 3, nz, resolution=256)
+        g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+        engine.set_geometry(g)
+        engine.set_parameterisation(p)
+        for c in (0, 1):
+            engine.sort_atoms(c, w["specimen"].x, w["specimen"].y, w["specimen"].z, w["specimen"].Z)
+            engine.build_transmission(c)
+        engine.set_wave_count(2)
+        engine.set_wave_configs([0, 1])
+        engine.init_plane_wave(1.0)
+        engine.multislice()
+        og = oracle.geometry(oracle.scenario(**w["scenario"]))
+        srt = oracle.sort_atoms(og, w["specimen"].x, w["specimen"].y, w["specimen"].z, w["specimen"].Z)
+        psi, _ = oracle.run_plane(oracle.transmission(og, srt, p.values, p.form, p.i_per_atom), oracle.propagator(og))
+        for i in range(2):
+            assert rel_l2(engine.exit_wave(i), psi) <= 1e-4, (nz, i)
+    w = wl.ctem_srtio3(3, 3, 8, resolution=256)
+    engine.set_geometry(cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"])))
+    engine.set_wave_count(1)
+    with pytest.raises(cltem_amd.MseError) as ex:
+        engine.multislice()
+    assert ex.value.code == -2
+
+
 @pytest.mark.parametrize("full3d", [False, True])
 def test_ctem_phonon_batch_mean_intensity(engine, oracle, full3d):
     w = wl.ctem_srtio3(3, 3, 4, resolution=256, params="lobato" if full3d else "kirkland")
