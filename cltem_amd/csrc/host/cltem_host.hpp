@@ -9,6 +9,8 @@
 // (clwrapper/clerror.h:17-24), and implement them on the C ABI of include/cltem_amd.h.
 #pragma once
 
+#include <array>
+#include <climits>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -38,6 +40,29 @@ int element_number(const std::string& symbol);
 // line 1 atom count, line 2 header/comment, then "A x y z" per atom; coordinates pass
 // through float like std::stof; a header containing "nm" scales by 10.
 AtomList read_xyz(const std::string& path);
+
+// A crystal read from a CIF file: cell geometry (Angstrom, degrees), symmetry operations as
+// rows (x, y, z coefficients, translation), and the sites with their symmetry copies in the
+// unit cell (fractional).
+struct CifStructure {
+    double a = 0, b = 0, c = 0, alpha = 0, beta = 0, gamma = 0;
+    struct Op { std::array<double, 4> row[3]; };
+    std::vector<Op> ops;
+    struct Site {
+        std::string label;
+        std::vector<int> Z;        // several elements may share a site
+        std::vector<double> occ;
+        double fx, fy, fz;
+        std::vector<std::array<double, 3>> positions;
+    };
+    std::vector<Site> sites;
+};
+CifStructure read_cif(const std::string& path);
+// Supercell of widths[3] (Angstrom) with zone axis [uvw] along z, optional [abc] along x and
+// tilts (degrees) about x, y, z; partial occupancies resolved per site with a seeded draw.
+// `occupancy` (optional) receives the occupancy of every atom before that selection.
+AtomList cif_supercell(const CifStructure& cif, const double uvw[3], const double abc[3], const double widths[3],
+                       const double tilts_deg[3], uint64_t seed, std::vector<double>* occupancy = nullptr);
 
 // SrTiO3 perovskite tiled nx*ny*nz along [001] (a = 3.905 A): Sr(0,0,0), Ti(1/2,1/2,1/2),
 // O(1/2,1/2,0), O(1/2,0,1/2), O(0,1/2,1/2); coordinates float32-exact.

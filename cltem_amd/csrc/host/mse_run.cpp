@@ -1,7 +1,8 @@
 // mse_run: command-line driver of the engine through the clTEM-shaped host classes.
 //
 //   mse_run --mode ctem|stem|cbed --params cltem_amd/params/kirkland.dat
-//           (--xyz file.xyz | --srtio3 NX,NY,NZ) --res 512 --dz 1.9525 --kv 200
+//           (--xyz file.xyz | --srtio3 NX,NY,NZ | --cif file.cif --widths X,Y,Z [--zone u,v,w]
+//            [--normal a,b,c] [--tilts a,b,g]) --res 512 --dz 1.9525 --kv 200
 //           [--area x0,x1,y0,y1] [--stem-pixels N] [--det name,inner,outer[,xc,yc]]...
 //           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
 //           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
@@ -55,6 +56,8 @@ int main(int argc, char** argv) {
         int dump_config = 0;
         int dump_incoherence = -1;
         bool tiff = false;
+        std::string cif_path;
+        double zone[3] = {0, 0, 1}, normal[3] = {0, 0, 0}, widths[3] = {0, 0, 0}, tilts[3] = {0, 0, 0};
         bool have_area = false;
         int stem_px = 64;
         for (int i = 1; i < argc; ++i) {
@@ -132,6 +135,12 @@ int main(int argc, char** argv) {
                     for (int x = 0; x < w; ++x) d[(size_t)y * w + x] = x + 1000.0 * y;
                 save_tiff(fn, d, w, h);
                 return 0;
+            } else if (a == "--cif") {
+                cif_path = next();
+            } else if (a == "--zone" || a == "--normal" || a == "--widths" || a == "--tilts") {
+                auto v = split_doubles(next());
+                double* dst = a == "--zone" ? zone : a == "--normal" ? normal : a == "--widths" ? widths : tilts;
+                for (int k = 0; k < 3; ++k) dst[k] = v.at(k);
             } else if (a == "--tiff") {
                 tiff = true;
             } else if (a == "--dump-incoherence") {
@@ -141,6 +150,11 @@ int main(int argc, char** argv) {
             } else {
                 throw std::runtime_error("unknown option " + a);
             }
+        }
+        if (!cif_path.empty()) {
+            if (widths[0] <= 0 || widths[1] <= 0 || widths[2] <= 0)
+                throw std::runtime_error("--cif needs --widths x,y,z (Angstrom)");
+            sm->atoms = cif_supercell(read_cif(cif_path), zone, normal, widths, tilts, sm->seed);
         }
         if (dump_incoherence >= 0) {
             std::cout.precision(17);

@@ -254,3 +254,111 @@ def test_driver_tiff_outputs(tmp_path):
         np.testing.assert_allclose(data, want.astype(np.float32), rtol=1e-6, atol=1e-6 * np.abs(want).max())
         settings = json.load(open(f"{prefix}_{name}.json"))
         assert settings["mode"] == "CTEM" and settings["resolution"] == 256
+
+
+def _pm3m_ops():
+    """The 48 operations of Pm-3m as CIF strings, generated from (z,x,y), (-y,x,z), (-x,-y,-z)."""
+    gens = [np.array(m) for m in ([[0, 0, 1], [1, 0, 0], [0, 1, 0]], [[0, -1, 0], [1, 0, 0], [0, 0, 1]],
+                                  [[-1, 0, 0], [0, -1, 0], [0, 0, -1]])]
+    ops = {tuple(np.eye(3, dtype=int).ravel())}
+    frontier = list(ops)
+    while frontier:
+        new = []
+        for o in frontier:
+            for g in gens:
+                m = tuple((g @ np.array(o).reshape(3, 3)).ravel())
+                if m not in ops:
+                    ops.add(m)
+                    new.append(m)
+        frontier = new
+    out = []
+    for o in sorted(ops):
+        m = np.array(o).reshape(3, 3)
+        terms = []
+        for r in range(3):
+            t = "".join(("+" if m[r, c] > 0 else "-") + "xyz"[c] for c in range(3) if m[r, c] != 0)
+            terms.append(t)
+        out.append(",".join(terms))
+    assert len(out) == 48
+    return out
+
+
+def _write_cif(path, a, sites, ops=None):
+    lines = ["data_test", f"_cell_length_a {a}(2)", f"_cell_length_b {a}", f"_cell_length_c {a}",
+             "_cell_angle_alpha 90.0", "_cell_angle_beta 90", "_cell_angle_gamma 90.000", "", "loop_",
+             "_symmetry_equiv_pos_as_xyz"]
+    lines += [f"'{o}'" for o in (ops or ["x,y,z"])]
+    lines += ["", "loop_", "_atom_site_label", "_atom_site_type_symbol", "_atom_site_fract_x", "_atom_site_fract_y",
+              "_atom_site_fract_z", "_atom_site_occupancy"]
+    lines += [f"{lab} {sym} {x} {y} {z} {occ}" for lab, sym, x, y, z, occ in sites]
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+def _cif_atoms(tmp_path, cif, *args):
+    f = tmp_path / "atoms.bin"
+    subprocess.run([RUN, "--cif", str(cif), *args, "--dump-atoms", str(f)], check=True)
+    raw = f.read_bytes()
+    n = len(raw) // 28
+    xyz = np.frombuffer(raw[:24 * n], dtype=np.float64).reshape(3, n)
+    Z = np.frombuffer(raw[24 * n:], dtype=np.int32)
+    return xyz, Z
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cif_supercell_matches_generator(tmp_path):
+    """SrTiO3 from its Pm-3m asymmetric unit (symmetry expansion) and from an explicit P1
+    cell: the [001] supercell equals the Python generator's tiling."""
+    a = 3.905
+    w = 3 * a - 1e-3
+    ref = sp.srtio3(3, 3, 3)
+    order_ref = np.lexsort((ref.z, ref.y, ref.x, ref.Z))
+    for name, sites, ops in (
+            ("sto_pm3m.cif", [("Sr1", "Sr2+", 0, 0, 0, 1), ("Ti1", "Ti4+", 0.5, 0.5, 0.5, 1), ("O1", "O2-", 0.5, 0.5, 0, 1)],
+             _pm3m_ops()),
+            ("sto_p1.cif", [("Sr", "Sr", 0, 0, 0, 1), ("Ti", "Ti", 0.5, 0.5, 0.5, 1), ("O1", "O", 0.5, 0.5, 0, 1),
+                            ("O2", "O", 0.5, 0, 0.5, 1), ("O3", "O", 0, 0.5, 0.5, 1)], None)):
+        cif = tmp_path / name
+        _write_cif(cif, a, sites, ops)
+        xyz, Z = _cif_atoms(tmp_path, cif, "--widths", f"{w},{w},{w}")
+        assert len(Z) == len(ref.Z), name
+        r = np.round(xyz, 6)  # the cell vectors carry cos(90 deg) ~ 6e-17 terms
+        o = np.lexsort((r[2], r[1], r[0], Z))
+        np.testing.assert_array_equal(Z[o], ref.Z[order_ref])
+        for got, exp in zip(xyz, (ref.x, ref.y, ref.z)):
+            np.testing.assert_allclose(got[o], exp[order_ref], atol=2e-6)
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cif_zone_axis_and_occupancy(tmp_path):
+    a = 3.905
+    cif = tmp_path / "sto.cif"
+    _write_cif(cif, a, [("Sr1", "Sr", 0, 0, 0, 1), ("Ti1", "Ti", 0.5, 0.5, 0.5, 1), ("O1", "O", 0.5, 0.5, 0, 1)],
+               _pm3m_ops())
+    # [110] along z: the cell is rotated, density and bond lengths are unchanged
+    W = 20.0
+    xyz, Z = _cif_atoms(tmp_path, cif, "--widths", f"{W},{W},{W}", "--zone", "1,1,0")
+    assert abs(len(Z) / W ** 3 - 5 / a ** 3) < 0.08 * 5 / a ** 3
+    pts = xyz.T
+    ti = pts[Z == 22]
+    o = pts[Z == 8]
+    d = np.sqrt(((ti[:, None, :] - o[None, :, :]) ** 2).sum(-1)).min(axis=1)
+    inner = np.all((ti > 2.5) & (ti < W - 2.5), axis=1)
+    np.testing.assert_allclose(d[inner], a / 2, atol=1e-5)
+    # rows of Sr along z repeat every a*sqrt(2) for [110]
+    sr = pts[Z == 38]
+    col = sr[np.all(np.abs(sr[:, :2] - sr[0, :2]) < 1e-6, axis=1)][:, 2]
+    assert np.allclose(np.diff(np.sort(col)), a * np.sqrt(2), atol=1e-5)
+    # partial occupancy: a half-occupied O site and a 70/30 Sr/Ca site; seeded, one atom per site
+    cif2 = tmp_path / "mixed.cif"
+    _write_cif(cif2, a, [("Sr1", "Sr", 0, 0, 0, 0.7), ("Ca1", "Ca", 0, 0, 0, 0.3), ("Ti1", "Ti", 0.5, 0.5, 0.5, 1),
+                         ("O1", "O", 0.5, 0.5, 0, 0.5)], _pm3m_ops())
+    w = 10 * a - 1e-3
+    xyz1, Z1 = _cif_atoms(tmp_path, cif2, "--widths", f"{w},{w},{w}", "--seed", "5")
+    xyz2, Z2 = _cif_atoms(tmp_path, cif2, "--widths", f"{w},{w},{w}", "--seed", "5")
+    np.testing.assert_array_equal(Z1, Z2)
+    n_a = np.sum(Z1 == 38) + np.sum(Z1 == 20)
+    assert n_a == 1000 and np.sum(Z1 == 22) == 1000
+    assert abs(np.sum(Z1 == 20) / 1000 - 0.3) < 0.05
+    assert abs(np.sum(Z1 == 8) / 3000 - 0.5) < 0.04
+    _, Z3 = _cif_atoms(tmp_path, cif2, "--widths", f"{w},{w},{w}", "--seed", "6")
+    assert not np.array_equal(Z1, Z3)
