@@ -102,23 +102,21 @@ def profiled_traffic(kernel):
 
 
 def roofline(engine, g, pos, batch, probe, dets):
+    """Dominant sweep's achieved algorithmic bandwidth: average launch durations of the row
+    and column passes timed back to back with hipEvents on the engine stream (a batch of B
+    probes of one configuration, current kernel variants)."""
     N = g.resolution * g.resolution
-    ns = g.n_slices
-    engine.profile_reset()
-    engine.profile(True)
-    engine.stem_batch(0, pos[:batch], batch, probe, dets)
-    engine.profile(False)
-    n_row, t_row = engine.profile_read(0)
-    n_col, t_col = engine.profile_read(1)
+    engine.stem_batch(0, pos[:batch], batch, probe, dets)  # B waves of configuration 0
+    row_us, col_us = engine.time_sweeps(g.n_slices // 2, reps=20)
     # algorithmic bytes per launch (8 bytes per complex64):
     #   row pass: read + write every wave (16N each) and the shared t slice once (8N)
-    #   column pass: read + write every wave; the last slice only reads (detector fused)
-    row_bytes = ns * (batch * 16 * N + 8 * N)
-    col_bytes = (ns - 1) * batch * 16 * N + batch * 8 * N
-    kern = {"row_pass": (n_row, t_row, row_bytes), "column_pass": (n_col, t_col, col_bytes)}
-    name = max(kern, key=lambda k: kern[k][1])
-    n, tms, b = kern[name]
-    achieved = b / (tms * 1e-3) / 1e9
+    #   column pass: read + write every wave
+    row_bytes = batch * 16 * N + 8 * N
+    col_bytes = batch * 16 * N
+    kern = {"row_pass": (row_us, row_bytes), "column_pass": (col_us, col_bytes)}
+    name = max(kern, key=lambda k: kern[k][0])
+    us, b = kern[name]
+    achieved = b / (us * 1e-6) / 1e9
     return {
         "kernel": f"{name} (512^2, B={batch})",
         "bound": "hbm",
@@ -129,13 +127,13 @@ def roofline(engine, g, pos, batch, probe, dets):
         "traffic": profiled_traffic(name) if (batch == 64 and g.resolution == 512) else None,
         "traffic_source": "profiles/slice_sweeps_512_B64.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                           "calibrated with a copy kernel)",
-        "launches": n,
-        "avg_launch_us": round(tms * 1e3 / n, 2),
-        "bytes_per_launch": int(b / n),
-        "row_pass_avg_us": round(t_row * 1e3 / max(n_row, 1), 2),
-        "column_pass_avg_us": round(t_col * 1e3 / max(n_col, 1), 2),
-        "row_pass_GBs": round(row_bytes / (t_row * 1e-3) / 1e9, 1),
-        "column_pass_GBs": round(col_bytes / (t_col * 1e-3) / 1e9, 1),
+        "avg_launch_us": round(us, 2),
+        "bytes_per_launch": int(b),
+        "row_pass_avg_us": round(row_us, 2),
+        "column_pass_avg_us": round(col_us, 2),
+        "row_pass_GBs": round(row_bytes / (row_us * 1e-6) / 1e9, 1),
+        "column_pass_GBs": round(col_bytes / (col_us * 1e-6) / 1e9, 1),
+        "timing": "hipEvents around 20 back-to-back launches of each sweep on the engine stream",
     }
 
 
@@ -169,29 +167,24 @@ def cpu_baseline(w, g, spec0, pos, gpu_det, n_probes, n_check):
 
 
 def kernel_roofline(engine, g, n_waves, label):
-    """Row/column pass times of one eager slice loop over all waves (each wave reading its
-    own transmission function), as achieved algorithmic GB/s against the HBM peak."""
+    """Row/column pass launch durations over all waves (each wave reading its own
+    transmission function), timed back to back with hipEvents, as achieved algorithmic GB/s."""
     N = g.resolution * g.resolution
-    ns = g.n_slices
-    engine.profile_reset()
-    engine.profile(True)
-    engine.multislice()
-    engine.synchronize()
-    engine.profile(False)
-    n_row, t_row = engine.profile_read(0)
-    n_col, t_col = engine.profile_read(1)
-    row_bytes = ns * n_waves * 24 * N   # read + write the wave, read its t
-    col_bytes = ns * n_waves * 16 * N
-    kern = {"row_pass": (n_row, t_row, row_bytes), "column_pass": (n_col, t_col, col_bytes)}
-    name = max(kern, key=lambda k: kern[k][1])
-    n, tms, b = kern[name]
-    achieved = b / (tms * 1e-3) / 1e9
+    row_us, col_us = engine.time_sweeps(g.n_slices // 2, reps=10)
+    row_bytes = n_waves * 24 * N   # read + write the wave, read its own t
+    col_bytes = n_waves * 16 * N
+    kern = {"row_pass": (row_us, row_bytes), "column_pass": (col_us, col_bytes)}
+    name = max(kern, key=lambda k: kern[k][0])
+    us, b = kern[name]
+    achieved = b / (us * 1e-6) / 1e9
     return {
         "kernel": f"{name} ({label})", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "launches": n, "avg_launch_us": round(tms * 1e3 / max(n, 1), 2), "bytes_per_launch": int(b / max(n, 1)),
-        "row_pass_GBs": round(row_bytes / (t_row * 1e-3) / 1e9, 1),
-        "column_pass_GBs": round(col_bytes / (t_col * 1e-3) / 1e9, 1),
+        "avg_launch_us": round(us, 2), "bytes_per_launch": int(b),
+        "row_pass_avg_us": round(row_us, 2), "column_pass_avg_us": round(col_us, 2),
+        "row_pass_GBs": round(row_bytes / (row_us * 1e-6) / 1e9, 1),
+        "column_pass_GBs": round(col_bytes / (col_us * 1e-6) / 1e9, 1),
+        "timing": "hipEvents around 10 back-to-back launches of each sweep on the engine stream",
     }
 
 
@@ -229,7 +222,6 @@ def bench_sio2(args):
     t_total = time.perf_counter() - t0
     value = args.steps * n_cfg * ns / t_total
     log(f"timed {args.steps} steps in {t_total:.3f} s -> {value:.1f} configuration.slices/s")
-    engine.init_plane_wave(1.0)
     roof = kernel_roofline(engine, g, n_cfg, f"1024^2, {n_cfg} configurations")
     cpu = None
     if not args.no_cpu:

@@ -337,9 +337,17 @@ class Engine:
         _check(self.lib.mse_autotune(self.h, config, n_waves, reps, _dp(row), _dp(col), C.byref(br),
                                      C.byref(bc)))
         self.n_waves = max(self.n_waves, n_waves)
-        nr, nc = 6, 9  # row / column kernel variants
+        nr, nc = 7, 11  # row / column kernel variants
         return dict(row_us=row[:nr].tolist(), col_us=col[:nc].tolist(), best_row=br.value,
                     best_col=bc.value)
+
+    def time_sweeps(self, slice_: int = 0, reps: int = 10):
+        """Average microseconds per launch of the row and column passes (current variants,
+        every wave with its own configuration); the waves' contents become undefined."""
+        r = C.c_double(0)
+        c = C.c_double(0)
+        _check(self.lib.mse_time_sweeps(self.h, slice_, reps, C.byref(r), C.byref(c)))
+        return r.value, c.value
 
     def bandwidth_probe(self, pattern: int, n_waves: int, reps: int = 5, variant: int = 0) -> float:
         us = C.c_double(0)

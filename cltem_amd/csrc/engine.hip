@@ -1329,6 +1329,42 @@ int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, 
     });
 }
 
+int mse_time_sweeps(mse_engine* e, int slice, int reps, double* row_us, double* col_us) {
+    return run_guarded("mse_time_sweeps", e, [&]() {
+        REQUIRE(e && row_us && col_us, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        check_trans(e);
+        REQUIRE(reps >= 1 && slice >= 0 && slice < e->g.n_slices, MSE_ERR_ARG, "bad slice or repetition count");
+        const bool prof = e->prof;
+        e->prof = false;
+        hipEvent_t a, b, c;
+        HIPX(hipEventCreate(&a));
+        HIPX(hipEventCreate(&b));
+        HIPX(hipEventCreate(&c));
+        mse::RowArgs ra = base_row_args(e);
+        ra.slice = slice;
+        const mse::ColArgs ca = base_col_args(e);
+        const int nw = e->n_waves;
+        row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nw, e->N, ra);  // warm
+        col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nw, e->N, ca);
+        HIPX(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; ++r) row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nw, e->N, ra);
+        HIPX(hipEventRecord(b, e->stream));
+        for (int r = 0; r < reps; ++r) col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nw, e->N, ca);
+        HIPX(hipEventRecord(c, e->stream));
+        HIPX(hipEventSynchronize(c));
+        float ms_r = 0, ms_c = 0;
+        HIPX(hipEventElapsedTime(&ms_r, a, b));
+        HIPX(hipEventElapsedTime(&ms_c, b, c));
+        *row_us = 1e3 * ms_r / reps;
+        *col_us = 1e3 * ms_c / reps;
+        e->prof = prof;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipEventDestroy(c);
+    });
+}
+
 int mse_bandwidth_probe(mse_engine* e, int pattern, int variant, int n_waves, int reps, double* us_per_launch) {
     return run_guarded("mse_bandwidth_probe", e, [&]() {
         REQUIRE(e && us_per_launch, MSE_ERR_ARG, "NULL argument");

@@ -221,10 +221,13 @@ void slice_step(mse_engine* e, float2* waves, int nw, int slice);
 //          6: 32 columns (256-byte row runs), half lanes per column
 //   row    4: persistent lines with the next row prefetched into registers
 //          5: as 4 with half lanes per line (res <= 512; otherwise as 4)
+//          6: as 4 with twiddles read at use
 //   column 7: persistent blocks with the next tile prefetched into registers
 //          8: as 7 with half lanes per column, two blocks per CU (res <= 512; otherwise as 7)
-constexpr int kRowVariants = 6;
-constexpr int kColVariants = 9;
+//          9: as 7 with twiddles read at use
+//         10: as 7 with 8-column tiles (512 threads: no register spills at 1024 points)
+constexpr int kRowVariants = 7;
+constexpr int kColVariants = 11;
 
 // Sweep launchers, one translation unit per resolution (sweeps_<N>.hip).
 template <int N>

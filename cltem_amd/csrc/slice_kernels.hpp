@@ -443,7 +443,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
 // loads of item i+1 are already in flight into registers, and the stores of item i drain
 // during the next item's transforms.  Same arithmetic as k_col<..., COL_PROP>.
 // --------------------------------------------------------------------------------------
-template <class PL, int TC, class SYNC = WaveSync>
+template <class PL, int TC, class SYNC = WaveSync, bool TWREG = true>
 __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict__ data, size_t wave_stride,
                                                               const float2* __restrict__ tw, ColArgs a,
                                                               int nsets) {
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
     const int t = tid % L;
     const int nitems = nsets * ntiles;
 
-    PlanTw<PL, true> tws;
+    PlanTw<PL, TWREG> tws;
     tws.load(tw, t);
     const SYNC sync{};
 
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
 // (wave, row) items independently; the next item's wave row and transmission row are
 // loaded into registers while the current row is transformed.
 // --------------------------------------------------------------------------------------
-template <class PL, class SYNC = WaveSync>
+template <class PL, class SYNC = WaveSync, bool TWREG = true>
 __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data, size_t wave_stride,
                                                        const float2* __restrict__ tw, RowArgs a, int nsets) {
     constexpr int N = PL::N, L = PL::L, E = PL::E;
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
     const int nitems = nsets * N;
     const int stride = gridDim.x * LINES;
 
-    PlanTw<PL, true> tws;
+    PlanTw<PL, TWREG> tws;
     tws.load(tw, t);
     const SYNC sync{};
 

@@ -60,6 +60,15 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
                 return;
             }
         }
+        if (mode == mse::ROW_IFFT_T_FFT && variant == 6 && in == out) {  // twiddles read at use
+            constexpr int LINES = 256 / P0::L;
+            const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+            launch(e, PK_ROW, [&]() {
+                mse::k_row_step_pipe<P0, mse::WaveSync, false><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p,
+                                                                                            args, nlines_sets);
+            });
+            return;
+        }
         if (mode == mse::ROW_IFFT_T_FFT && variant >= 4 && in == out) {
             constexpr int LINES = 256 / P0::L;
             const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
@@ -134,6 +143,26 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
                 });
                 return tiles;
             }
+        }
+        if (variant == 10 && mode == mse::COL_PROP && in == out) {  // 8-column tiles (no spills at 1024)
+            constexpr int NT = 8 * P0::L;
+            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 8) * nsets));
+            tiles = N / 8;
+            launch(e, kind, [&]() {
+                mse::k_col_prop_pipe<P0, 8, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride, e->tw.p,
+                                                                                        args, nsets);
+            });
+            return tiles;
+        }
+        if (variant == 9 && mode == mse::COL_PROP && in == out) {  // twiddles read at use
+            constexpr int NT = 16 * P0::L;
+            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
+            tiles = N / 16;
+            launch(e, kind, [&]() {
+                mse::k_col_prop_pipe<P0, 16, mse::WaveSync, false><<<blocks, NT, 0, e->stream>>>(out, wave_stride,
+                                                                                                e->tw.p, args, nsets);
+            });
+            return tiles;
         }
         if (variant >= 7 && mode == mse::COL_PROP && in == out) {
             constexpr int NT = 16 * P0::L;

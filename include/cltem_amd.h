@@ -228,12 +228,14 @@ int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes
                    const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
                    int use_graph, double* accum);
 
-/* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..5):
+/* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..6):
  *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at
- *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes.
- *      kernel 1 = column pass (variants 0..8): 0 16-column tiles, 1 half lanes, 2 8-column
+ *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes,
+ *      6 as 4 with twiddles read at use.
+ *      kernel 1 = column pass (variants 0..10): 0 16-column tiles, 1 half lanes, 2 8-column
  *      tiles, 3 block barriers, 4/5 twiddles read at use (16/8 columns), 6 32-column tiles,
- *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes.
+ *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes, 9 as 7
+ *      with twiddles read at use, 10 as 7 with 8-column tiles.
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
  *      (reps launches each; times in microseconds per launch), selects the fastest and
  *      leaves the waves' contents undefined. ---- */
@@ -260,6 +262,12 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
  * only (MSE_ERR_UNSUPPORTED for full 3D); there dL/dz = 0 because an atom's potential does
  * not depend on its depth within its slice. */
 int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz);
+/* ---- sweep timing: `reps` back-to-back row passes (slice `slice`, every wave with its own
+ *      configuration, current variant) and then `reps` column passes, timed with hipEvents on
+ *      the engine stream; average microseconds per launch.  Leaves the waves' contents
+ *      undefined. ---- */
+int mse_time_sweeps(mse_engine* e, int slice, int reps, double* row_us, double* col_us);
+
 /* ---- bandwidth reference: copy n_waves waves in place with the access pattern of the row
  *      sweep (pattern 0) or of the column sweep's LDS tiles (pattern 1, column variant
  *      `variant`), no arithmetic; average microseconds per launch.  Used to calibrate HBM
