@@ -155,6 +155,17 @@ def test_errors_are_reported(engine):
     bad = cltem_amd.plan_geometry(cltem_amd.make_scenario(**dict(scen, resolution=640)))
     with pytest.raises(cltem_amd.MseError):
         engine.set_geometry(bad)
+    # device allocation failure is an error, not a crash, and the engine stays usable
+    big = cltem_amd.plan_geometry(cltem_amd.make_scenario(**dict(scen, resolution=1024)))
+    engine.set_geometry(big)
+    with pytest.raises(cltem_amd.MseError) as ex:
+        engine.set_wave_count(40000)  # 40000 x 8 MiB = 313 GiB > HBM
+    assert ex.value.code == -3
+    with pytest.raises(cltem_amd.MseError):
+        engine.multislice()  # no waves
+    engine.set_wave_count(2)
+    engine.init_plane_wave(1.0)
+    engine.synchronize()
 
 
 def test_ctem_image_formation(engine, oracle):
