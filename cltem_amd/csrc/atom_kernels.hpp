@@ -187,34 +187,35 @@ __device__ __forceinline__ float vproj(const float* __restrict__ p, int ic, floa
     }
 }
 
-// Full-3D potentials (kernels/transmission_potentials_full_3d_f.cl:57-134)
+// Full-3D potentials (kernels/transmission_potentials_full_3d_f.cl:57-134), evaluated from
+// per-element constants precomputed on the host in double: the square roots, reciprocals and
+// b^-3/2 factors of the parameters are hoisted out of the per-point sums, and the exponent
+// scales carry log2(e) so that every term is one hardware exp2 (v_exp_f32, about 1 ulp).
+// r2 is the squared distance, already clamped to r_min^2.  Table layout per Z
+// (kPot3dStride floats):
+//   Kirkland  a_i | 2 pi log2e sqrt(b_i) | c_i d_i^-3/2 | pi^2 log2e / d_i
+//   Peng      a_i b_i^-3/2 | pi^2 log2e / b_i
+//   Lobato    a_i b_i^-3/2 | 2 pi log2e / sqrt(b_i) | sqrt(b_i) / pi
+constexpr int kPot3dStride = 16;
+
 template <int FORM>
-__device__ __forceinline__ float v3d(const float* __restrict__ p, int ic, float r) {
+__device__ __forceinline__ float v3d_tab(const float* __restrict__ t, int ic, float r2) {
     if constexpr (FORM == 0) {
-        const float x = 2.0f * (float)M_PI * r;
+        const float r = __builtin_amdgcn_sqrtf(r2);
         float suml = 0.f, sumg = 0.f;
-        for (int i = 0; i < ic * 2; i += 2) suml += p[i] * expf(-x * sqrtf(p[i + 1]));
-        const float xg = ((float)M_PI * r) * ((float)M_PI * r);
-        for (int i = ic * 2; i < ic * 4; i += 2) {
-            const float dir = rsqrtf(p[i + 1]);
-            sumg += p[i] * (dir * dir * dir) * expf(-xg / p[i + 1]);
-        }
-        return 150.4121417f / r * suml + 266.5157269f * sumg;
+        for (int i = 0; i < ic; ++i) suml += t[i] * __builtin_amdgcn_exp2f(-r * t[3 + i]);
+        for (int i = 0; i < ic; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
+        return 150.4121417f * __builtin_amdgcn_rcpf(r) * suml + 266.5157269f * sumg;
     } else if constexpr (FORM == 1) {
-        const float x = ((float)M_PI * r) * ((float)M_PI * r);
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) {
-            const float bir = rsqrtf(p[i + 5]);
-            sum += p[i] * (bir * bir * bir) * expf(-x / p[i + 5]);
-        }
+        for (int i = 0; i < ic; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
         return 266.5157269f * sum;
     } else {
-        const float x = (float)M_PI * r;
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const float inv_r = __builtin_amdgcn_rcpf(r);
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) {
-            const float bir = rsqrtf(p[i + 5]);
-            sum += p[i] * (bir * bir * bir) * expf(-2.0f * x * bir) * (sqrtf(p[i + 5]) / x + 1.0f);
-        }
+        for (int i = 0; i < ic; ++i)
+            sum += t[i] * __builtin_amdgcn_exp2f(-r * t[5 + i]) * (t[10 + i] * inv_r + 1.0f);
         return 472.545072199968f * sum;
     }
 }
@@ -350,7 +351,7 @@ template <int FORM>
 __global__ __launch_bounds__(256) void k_potential_full3d(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
-    const float* __restrict__ params, PotArgs a, int first) {
+    const float* __restrict__ tab3d, PotArgs a, int first) {
 #pragma clang fp contract(off)
     __shared__ double sx[256], sy[256], sz[256];
     __shared__ int sZ[256];
@@ -368,6 +369,9 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     const double im_pos_x = a.startx + xid * a.pixelscale;
     const double im_pos_y = a.starty + yid * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
+    // distances below r_min are evaluated at r_min (squared, as the potentials take r^2)
+    const double r_min2 = r_min * r_min;
+    const float r_min2f = (float)r_min * (float)r_min;
     const bool no_shift = (a.shift_x == 0.0 && a.shift_y == 0.0);
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
@@ -389,18 +393,30 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
                 for (int l = 0; l < cnt; ++l) {
                     double rad_x = im_pos_x - sx[l];
                     double rad_y = im_pos_y - sy[l];
-                    if (no_shift && rad_x * rad_x + rad_y * rad_y > 64.0) continue;
-                    const float* pz = params + (size_t)(sZ[l] - 1) * a.per_atom;
+                    const float* pz = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+                    if (no_shift) {
+                        // the xy distance is the same at every integration point
+                        const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
+                        if (xyrad2 > 64.0) continue;
+                        for (int h = 0; h <= a.integrals; ++h) {
+                            const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
+                            if (rad_z <= 3.0) {
+                                const double r2 = xyrad2 + rad_z * rad_z;
+                                const double p1 = (double)v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
+                                sumz += (h != 0) * (p1 + p2) * 0.5;
+                                p2 = p1;
+                            }
+                        }
+                        continue;
+                    }
                     for (int h = 0; h <= a.integrals; ++h) {
                         rad_x -= a.shift_x;
                         rad_y -= a.shift_y;
                         const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
-                        const double im_pos_z = current_z - h * a.dz * int_r;
-                        const double rad_z = im_pos_z - sz[l];
+                        const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
                         if (xyrad2 <= 64.0 && rad_z <= 3.0) {
-                            double rad = sqrt(xyrad2 + rad_z * rad_z);
-                            if (rad < r_min) rad = r_min;
-                            const double p1 = (double)v3d<FORM>(pz, a.i_count, (float)rad);
+                            const double r2 = xyrad2 + rad_z * rad_z;
+                            const double p1 = (double)v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
                             sumz += (h != 0) * (p1 + p2) * 0.5;
                             p2 = p1;
                         }

@@ -425,6 +425,35 @@ int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const doub
         std::vector<float> f(params, params + n_params);
         e->params.ensure(n_params);
         HIPX(hipMemcpyAsync(e->params.p, f.data(), n_params * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        // full-3D constants per element, in double then rounded (v3d_tab's table layout)
+        const int nz = n_params / per;
+        const double log2e = 1.4426950408889634, pi = 3.141592653589793;
+        std::vector<float> t3((size_t)nz * mse::kPot3dStride, 0.f);
+        for (int z = 0; z < nz; ++z) {
+            const double* p = params + (size_t)z * per;
+            float* t = t3.data() + (size_t)z * mse::kPot3dStride;
+            for (int i = 0; i < i_per_atom; ++i) {
+                if (form == 0) {  // a_i, b_i pairs then c_i, d_i pairs
+                    const double a = p[2 * i], b = p[2 * i + 1], c = p[2 * i_per_atom + 2 * i],
+                                 d = p[2 * i_per_atom + 2 * i + 1];
+                    t[i] = (float)a;
+                    t[3 + i] = (float)(2.0 * pi * log2e * std::sqrt(b));
+                    t[6 + i] = (float)(c / (d * std::sqrt(d)));
+                    t[9 + i] = (float)(pi * pi * log2e / d);
+                } else {  // a_i then b_i
+                    const double a = p[i], b = p[i_per_atom + i];
+                    t[i] = (float)(a / (b * std::sqrt(b)));
+                    if (form == 1) {
+                        t[5 + i] = (float)(pi * pi * log2e / b);
+                    } else {
+                        t[5 + i] = (float)(2.0 * pi * log2e / std::sqrt(b));
+                        t[10 + i] = (float)(std::sqrt(b) / pi);
+                    }
+                }
+            }
+        }
+        e->params3d.ensure(t3.size());
+        HIPX(hipMemcpyAsync(e->params3d.p, t3.data(), t3.size() * sizeof(float), hipMemcpyHostToDevice, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
         e->form = form;
         e->i_count = i_per_atom;
@@ -549,11 +578,11 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
             launch(e, PK_POT, [&]() {
                 if (g.full3d) {
                     if (e->form == 0)
-                        mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
                     else if (e->form == 1)
-                        mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
                     else
-                        mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
                 } else {
                     if (e->form == 0)
                         mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);

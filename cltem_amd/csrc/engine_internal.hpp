@@ -71,7 +71,14 @@ struct DevBuf {
         if (count <= n && p) return;
         release();
         if (count == 0) return;
-        HIPX(hipMalloc((void**)&p, count * sizeof(T)));
+        const hipError_t rc = hipMalloc((void**)&p, count * sizeof(T));
+        if (rc != hipSuccess) {
+            // a failed allocation also leaves its code as the runtime's last error, which the
+            // launch check of the next kernel would report: clear it, the error is thrown here
+            (void)hipGetLastError();
+            p = nullptr;
+            throw Error(MSE_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(rc));
+        }
         n = count;
     }
 };
@@ -112,6 +119,7 @@ struct mse_engine {
     DevBuf<double> kx, ky;
     std::vector<double> h_kx, h_ky;
     DevBuf<float> params;
+    DevBuf<float> params3d;  // per-element constants of the full-3D potentials (kPot3dStride per Z)
     int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
     std::vector<ConfigData> configs;
     DevBuf<const float2*> cfg_ptrs;
