@@ -205,6 +205,20 @@ class Engine:
         _check(self.lib.mse_set_config_geometry(self.h, config, C.byref(g)))
         self._cfg_geom[config] = g
 
+    def set_config_beam(self, config: int, k_vec=None):
+        """Beam wave vector of a configuration (plasmon tilt, modifyBeamTilt); None restores
+        the geometry's.  Rebuild its transmission functions for the slices still to come."""
+        if k_vec is None:
+            _check(self.lib.mse_set_config_beam(self.h, config, None))
+        else:
+            k = np.ascontiguousarray(k_vec, dtype=np.float64).reshape(3)
+            _check(self.lib.mse_set_config_beam(self.h, config, _dp(k)))
+
+    def set_mixed_potentials(self, first_config: int = 0, n_sets: int = 0, seed: int = 0):
+        """Mixed static potentials: every wave draws one of configurations first_config ..
+        first_config + n_sets - 1 per slice (keyed by its probe index); n_sets < 2 turns it off."""
+        _check(self.lib.mse_set_mixed_potentials(self.h, first_config, n_sets, seed & 0xFFFFFFFFFFFFFFFF))
+
     # -- waves --------------------------------------------------------------------------
     def set_wave_count(self, n: int):
         _check(self.lib.mse_set_wave_count(self.h, n))

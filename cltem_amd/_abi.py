@@ -144,6 +144,8 @@ SIGNATURES = [
     ("mse_build_transmission", C.c_int, [_eng, C.c_int, C.c_int, C.c_int]),
     ("mse_read_transmission", C.c_int, [_eng, C.c_int, C.c_int, _fp]),
     ("mse_set_config_geometry", C.c_int, [_eng, C.c_int, C.POINTER(MseGeometry)]),
+    ("mse_set_config_beam", C.c_int, [_eng, C.c_int, C.POINTER(C.c_double)]),
+    ("mse_set_mixed_potentials", C.c_int, [_eng, C.c_int, C.c_int, C.c_ulonglong]),
     ("mse_set_wave_count", C.c_int, [_eng, C.c_int]),
     ("mse_set_wave_configs", C.c_int, [_eng, _ip, C.c_int]),
     ("mse_init_plane_wave", C.c_int, [_eng, C.c_double]),

@@ -24,6 +24,7 @@ int ceil_sqrt(int n) {
 }
 
 void require_cfg_waves(mse_engine* e, int cfg) {
+    REQUIRE(e->mix_sets < 2, MSE_ERR_UNSUPPORTED, "position gradients are not defined under mixed static potentials");
     for (int w = 0; w < e->n_waves; ++w)
         REQUIRE(e->h_wave_cfg[w] == cfg, MSE_ERR_ARG,
                 "every wave must refer to configuration " + std::to_string(cfg));

@@ -104,6 +104,7 @@ mse::ColArgs base_col_args(mse_engine* e) {
     mse::ColArgs a{};
     a.px = e->px.p;
     a.py = e->py.p;
+    a.wave_prop = e->any_beam ? e->wave_prop.p : nullptr;
     a.lim = e->prop_lim.p;
     a.kx = e->kx.p;
     a.ky = e->ky.p;
@@ -122,6 +123,10 @@ mse::RowArgs base_row_args(mse_engine* e) {
     a.slice = 0;
     a.plane_value = 0.f;
     a.scale = 1.0f;
+    a.mix_key = e->mix_key.p;
+    a.mix_seed = make_uint2((unsigned)(e->mix_seed & 0xffffffffu), (unsigned)(e->mix_seed >> 32));
+    a.mix_first = e->mix_first;
+    a.mix_sets = e->mix_sets >= 2 ? e->mix_sets : 0;
     return a;
 }
 
@@ -177,10 +182,67 @@ mse::PotArgs pot_args(mse_engine* e, const ConfigData& c) {
     a.z_min = g.z_lims[0];
     a.shift_x = (g.k_vec[0] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;  // simulationgeneral.cpp:562
     a.shift_y = (g.k_vec[1] / g.k_vec[2]) * g.slice_thickness / g.full3d_integrals;
+    if (c.own_beam) {
+        // modifyBeamTilt (simulationgeneral.cpp:649-679): tilt and azimuth of the configuration's
+        // beam vector; the tilt is passed in mrad as the kernel expects it, and the full-3D y
+        // shift uses ky
+        const double kx = c.k_vec[0], ky = c.k_vec[1], kz = c.k_vec[2];
+        const double theta = M_PI_2 - std::atan(std::sqrt(kx * kx + ky * ky) / kz);
+        const double phi = std::atan2(ky, kx);
+        a.cos_phi = std::cos(phi);
+        a.sin_phi = std::sin(phi);
+        a.sin_2theta = std::sin(2.0 * theta);
+        a.tan_theta = std::tan(theta);
+        a.shift_x = (kx / kz) * g.slice_thickness / g.full3d_integrals;
+        a.shift_y = (ky / kz) * g.slice_thickness / g.full3d_integrals;
+    }
     a.integrals = g.full3d_integrals;
     a.per_atom = e->per_atom;
     a.i_count = e->i_count;
     return a;
+}
+
+// Propagator (kernels/propagator_d.cl:49-83) for beam vector kv, split as
+// f e^{i pi dz (k^2-kz^2)/kz} e^{-i pi dz (bx+kx)^2/kz} e^{-i pi dz (by+ky)^2/kz}
+// (separable for any tilt), with the two unitary FFT scales 1/res^2 folded into px.
+// Needs the engine's k-grid (h_kx, h_ky) and geometry.
+void propagator_tables(mse_engine* e, const double* kv, std::vector<float2>& hpx, std::vector<float2>& hpy) {
+    const int res = e->res;
+    const double dz = e->g.slice_thickness;
+    const double k = e->g.wavenumber, bkx = kv[0], bky = kv[1], bkz = kv[2];
+    const double f = bkz / k;
+    const double c0 = M_PI * dz * (k * k - bkz * bkz) / bkz;
+    const double inv_n = 1.0 / ((double)res * (double)res);
+    hpx.resize(res);
+    hpy.resize(res);
+    for (int i = 0; i < res; ++i) {
+        double ax = std::remainder(c0 - M_PI * dz * (bkx + e->h_kx[i]) * (bkx + e->h_kx[i]) / bkz, 2.0 * M_PI);
+        hpx[i] = make_float2((float)(f * inv_n * std::cos(ax)), (float)(f * inv_n * std::sin(ax)));
+        double ay = std::remainder(-M_PI * dz * (bky + e->h_ky[i]) * (bky + e->h_ky[i]) / bkz, 2.0 * M_PI);
+        hpy[i] = make_float2((float)std::cos(ay), (float)std::sin(ay));
+    }
+}
+
+void sync_wave_prop(mse_engine* e) {
+    const int nw = std::min<int>(e->n_waves, (int)e->h_wave_cfg.size());
+    bool any = false;
+    for (int w = 0; w < nw; ++w) {
+        const int c = e->h_wave_cfg[w];
+        if (c >= 0 && c < (int)e->configs.size() && e->configs[c].own_beam) any = true;
+    }
+    e->any_beam = any;
+    if (!any) return;
+    e->h_wave_prop.assign(2 * (size_t)nw, nullptr);
+    for (int w = 0; w < nw; ++w) {
+        const ConfigData& c = e->configs[e->h_wave_cfg[w]];
+        e->h_wave_prop[2 * w] = c.own_beam ? c.px.p : e->px.p;
+        e->h_wave_prop[2 * w + 1] = c.own_beam ? c.py.p : e->py.p;
+    }
+    HIPX(hipStreamSynchronize(e->stream));  // kernels in flight may read the table
+    e->wave_prop.ensure(2 * (size_t)nw);
+    HIPX(hipMemcpyAsync(e->wave_prop.p, e->h_wave_prop.data(), 2 * (size_t)nw * sizeof(const float2*),
+                        hipMemcpyHostToDevice, e->stream));
+    HIPX(hipStreamSynchronize(e->stream));
 }
 
 ConfigData& config(mse_engine* e, int c) {
@@ -337,6 +399,7 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
             ++c.version;
             if (c.own_geom) c.trans.release();
             c.own_geom = false;
+            c.own_beam = false;  // its propagator tables belong to the previous k-grid
         }
         // the device table of transmission pointers and slice counts must not keep pointers
         // released above or slice counts of the previous geometry
@@ -352,21 +415,8 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
             const double ang = -2.0 * M_PI * (double)q / (double)res;
             tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
         }
-        // propagator (kernels/propagator_d.cl:49-83) split as
-        // f e^{i pi dz (k^2-kz^2)/kz} e^{-i pi dz (bx+kx)^2/kz} e^{-i pi dz (by+ky)^2/kz},
-        // with the two unitary FFT scales 1/res^2 folded into px
-        const double dz = g->slice_thickness;
-        const double k = g->wavenumber, bkx = g->k_vec[0], bky = g->k_vec[1], bkz = g->k_vec[2];
-        const double f = bkz / k;
-        const double c0 = M_PI * dz * (k * k - bkz * bkz) / bkz;
-        const double inv_n = 1.0 / ((double)res * (double)res);
-        std::vector<float2> hpx(res), hpy(res);
-        for (int i = 0; i < res; ++i) {
-            double ax = std::remainder(c0 - M_PI * dz * (bkx + e->h_kx[i]) * (bkx + e->h_kx[i]) / bkz, 2.0 * M_PI);
-            hpx[i] = make_float2((float)(f * inv_n * std::cos(ax)), (float)(f * inv_n * std::sin(ax)));
-            double ay = std::remainder(-M_PI * dz * (bky + e->h_ky[i]) * (bky + e->h_ky[i]) / bkz, 2.0 * M_PI);
-            hpy[i] = make_float2((float)std::cos(ay), (float)std::sin(ay));
-        }
+        std::vector<float2> hpx, hpy;
+        propagator_tables(e, g->k_vec, hpx, hpy);
         // masks, decided in double with the reference formulas, stored as per-row |kx index| limits
         const double k_max = bw * g->limit_factor;           // simulationgeneral.cpp:633
         const double k_max_2 = k_max * k_max;                 // propagator_d.cl:69-71 (strict <)
@@ -410,6 +460,7 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
         HIPX(hipStreamSynchronize(e->stream));
         e->have_geom = true;
         upload_cfg_ptrs(e);
+        sync_wave_prop(e);
     });
 }
 
@@ -627,6 +678,13 @@ int mse_set_wave_count(mse_engine* e, int n_waves) {
         e->waves.ensure((size_t)n_waves * e->N);
         e->wave_cfg.ensure(n_waves);
         e->probe_pos.ensure(2 * (size_t)n_waves);
+        e->mix_key.ensure(n_waves);
+        {
+            std::vector<unsigned> keys(n_waves);
+            for (int i = 0; i < n_waves; ++i) keys[i] = (unsigned)i;
+            HIPX(hipMemcpyAsync(e->mix_key.p, keys.data(), n_waves * sizeof(unsigned), hipMemcpyHostToDevice,
+                                e->stream));
+        }
         if ((int)e->h_wave_cfg.size() != n_waves) {
             e->h_wave_cfg.assign(n_waves, 0);
             HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
@@ -634,6 +692,7 @@ int mse_set_wave_count(mse_engine* e, int n_waves) {
         }
         e->n_waves = n_waves;
         HIPX(hipStreamSynchronize(e->stream));
+        sync_wave_prop(e);
     });
 }
 
@@ -648,6 +707,7 @@ int mse_set_wave_configs(mse_engine* e, const int* configs, int n_waves) {
         HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
                             e->stream));
         HIPX(hipStreamSynchronize(e->stream));
+        sync_wave_prop(e);
     });
 }
 
@@ -662,12 +722,137 @@ __global__ void k_fill_plane(float2* __restrict__ w, int res, size_t N, int n_wa
     w[i] = make_float2((p % res) == 0 ? v : 0.f, 0.f);
 }
 
+// bilinear_translate (kernels/bilinear_translate_d.cl:21-72) of each wave's fft-shifted
+// diffraction image by (ix + sx, iy + sy) pixels: the pixel moved from (x - ix, y - iy),
+// wrapped once (an image moved further is zero there), blended with its left, upper and
+// upper-left neighbours (wrapped).  sh holds on, ix, iy, sx, sy per wave; on = 0 copies.
+__global__ void k_translate_images(const double* __restrict__ in, double* __restrict__ out, int res, size_t N,
+                                   const double* __restrict__ sh) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int w = blockIdx.y;
+    const double* img = in + (size_t)w * N;
+    const double* s = sh + 5 * (size_t)w;
+    double v;
+    if (s[0] == 0.0) {
+        v = img[p];
+    } else {
+        const int W = res, H = res;
+        int nx = (int)(p % W) - (int)s[1];
+        int ny = (int)(p / W) - (int)s[2];
+        if (nx < 0) nx += W; else if (nx >= W) nx -= W;
+        if (ny < 0) ny += H; else if (ny >= H) ny -= H;
+        if (nx < 0 || nx >= W || ny < 0 || ny >= H) {
+            v = 0.0;
+        } else {
+            const int xl = nx == 0 ? W - 1 : nx - 1;
+            const int yu = ny == 0 ? H - 1 : ny - 1;
+            const double sx = s[3], sy = s[4];
+            v = (1.0 - sx) * (1.0 - sy) * img[(size_t)ny * W + nx] + sx * (1.0 - sy) * img[(size_t)ny * W + xl] +
+                (1.0 - sx) * sy * img[(size_t)yu * W + nx] + sx * sy * img[(size_t)yu * W + xl];
+        }
+    }
+    out[(size_t)w * N + p] = v;
+}
+
+// Each wave's image summed over each detector's bitmask in double (band_pass + sum_reduction
+// of getStemPixel, simulationstem.cpp:100-131): out[w * ndet + d] = (or +=) scale * sum.
+__global__ __launch_bounds__(256) void k_masked_sums(const double* __restrict__ img, size_t N,
+                                                     const unsigned* __restrict__ bits, int ndet, double scale,
+                                                     int accumulate, double* __restrict__ out) {
+    const int d = blockIdx.x, w = blockIdx.y;
+    const size_t words = N / 32;
+    const unsigned* b = bits + (size_t)d * words;
+    const double* im = img + (size_t)w * N;
+    double acc = 0.0;
+    for (size_t i = threadIdx.x; i < words; i += 256) {
+        unsigned m = b[i];
+        while (m) {
+            acc += im[i * 32 + __builtin_ctz(m)];
+            m &= m - 1;
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const size_t o = (size_t)w * ndet + d;
+        const double r = scale * red[0];
+        out[o] = accumulate ? out[o] + r : r;
+    }
+}
+
+// translateDiffImage's shift (simulationgeneral.cpp:832-854) of waves [w0, w0 + nw): the
+// difference of the wave's beam vector from the geometry's over inverseScale, split into
+// whole pixels and a fraction; uploaded to wave_shift.  Returns whether any wave moves.
+bool upload_wave_shifts(mse_engine* e, int w0, int nw) {
+    std::vector<double> sh(5 * (size_t)nw, 0.0);
+    bool any = false;
+    for (int i = 0; i < nw; ++i) {
+        const int c = e->h_wave_cfg[w0 + i];
+        if (c < 0 || c >= (int)e->configs.size() || !e->configs[c].own_beam) continue;
+        const double dkx = e->configs[c].k_vec[0] - e->g.k_vec[0];
+        const double dky = e->configs[c].k_vec[1] - e->g.k_vec[1];
+        if (dkx == 0.0 && dky == 0.0) continue;
+        const double shx = dkx / e->g.inverse_scale, shy = dky / e->g.inverse_scale;
+        const double ix = std::floor(shx), iy = std::floor(shy);
+        REQUIRE(std::fabs(ix) < 1e9 && std::fabs(iy) < 1e9, MSE_ERR_ARG, "diffraction translation out of range");
+        double* o = sh.data() + 5 * (size_t)i;
+        o[0] = 1.0;
+        o[1] = ix;
+        o[2] = iy;
+        o[3] = shx - ix;
+        o[4] = shy - iy;
+        any = true;
+    }
+    if (!any) return false;
+    e->wave_shift.ensure(sh.size());
+    HIPX(hipMemcpyAsync(e->wave_shift.p, sh.data(), sh.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPX(hipStreamSynchronize(e->stream));
+    return true;
+}
+
+// Diffraction images of waves [w0, w0 + nw) (COL_FFT_DIFF into diff_img), translated into
+// diff_moved where their beam has moved; returns the buffer that holds the final images.
+const double* diffraction_images(mse_engine* e, int w0, int nw) {
+    e->diff_img.ensure((size_t)nw * e->N);
+    mse::ColArgs ca = base_col_args(e);
+    ca.scale = (float)(1.0 / e->res);
+    ca.diff = e->diff_img.p;
+    ca.diff_accumulate = 0;
+    col_pass(e, mse::COL_FFT_DIFF, e->waves.p + (size_t)w0 * e->N, nullptr, nw, e->N, ca);
+    if (!upload_wave_shifts(e, w0, nw)) return e->diff_img.p;
+    e->diff_moved.ensure((size_t)nw * e->N);
+    launch(e, PK_OTHER, [&]() {
+        k_translate_images<<<dim3((unsigned)((e->N + 255) / 256), nw), 256, 0, e->stream>>>(
+            e->diff_img.p, e->diff_moved.p, e->res, e->N, e->wave_shift.p);
+    });
+    return e->diff_moved.p;
+}
+
 void check_waves(mse_engine* e) {
     require_geom(e);
     REQUIRE(e->n_waves > 0 && e->waves.p, MSE_ERR_STATE, "mse_set_wave_count has not been called");
 }
 
+void check_mix(mse_engine* e) {
+    if (e->mix_sets < 2) return;
+    for (int c = e->mix_first; c < e->mix_first + e->mix_sets; ++c) {
+        REQUIRE(c < (int)e->configs.size(), MSE_ERR_STATE, "mixed potential set " + std::to_string(c) + " does not exist");
+        const ConfigData& cd = e->configs[c];
+        REQUIRE(!cd.own_geom && !cd.own_beam, MSE_ERR_STATE,
+                "mixed potential sets must use the engine's geometry and beam");
+        REQUIRE(cd.trans.p && cd.built && cd.trans.n >= (size_t)e->g.n_slices * e->N, MSE_ERR_STATE,
+                "transmission functions of mixed potential set " + std::to_string(c) + " are not built");
+    }
+}
+
 void check_trans(mse_engine* e) {
+    check_mix(e);
     for (int i = 0; i < e->n_waves; ++i) {
         const int c = e->h_wave_cfg[i];
         REQUIRE(c >= 0 && c < (int)e->configs.size(), MSE_ERR_STATE, "wave refers to an unknown configuration");
@@ -713,6 +898,11 @@ void upload_positions(mse_engine* e, const double* pos_xy, int n, int first_wave
     }
     HIPX(hipMemcpyAsync(e->probe_pos.p + 2 * (size_t)first_wave, rel.data(), rel.size() * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));
+    // keys of the mixed-potential draws: the probe indices
+    std::vector<unsigned> keys(n);
+    for (int i = 0; i < n; ++i) keys[i] = (unsigned)(first_probe + i);
+    HIPX(hipMemcpyAsync(e->mix_key.p + first_wave, keys.data(), n * sizeof(unsigned), hipMemcpyHostToDevice,
+                        e->stream));
     if (!e->h_focus.empty()) {
         REQUIRE((size_t)first_probe + n <= e->h_focus.size(), MSE_ERR_ARG,
                 "fewer focus changes than probes (mse_set_focus_spread)");
@@ -739,7 +929,9 @@ void probes_into_waves(mse_engine* e, int n, const mse_probe* p) {
 void active_waves(mse_engine* e, int nw) {
     std::vector<int> key(e->h_wave_cfg.begin(), e->h_wave_cfg.begin() + nw);
     std::vector<int> ns(e->configs.size());
-    for (size_t c = 0; c < e->configs.size(); ++c) ns[c] = cfg_geom(e, e->configs[c]).n_slices;
+    // under mixed static potentials every wave reads sets of the engine's thickness
+    for (size_t c = 0; c < e->configs.size(); ++c)
+        ns[c] = e->mix_sets >= 2 ? e->g.n_slices : cfg_geom(e, e->configs[c]).n_slices;
     key.push_back(-1);
     key.insert(key.end(), ns.begin(), ns.end());
     key.push_back(e->g.n_slices);
@@ -815,9 +1007,10 @@ void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_o
 // of `nb` probes, replayed per batch.  Positions and the output offset live in device
 // memory and are updated (outside the graph) before each replay.
 struct StemGraphKey {
-    int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices;
-    const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos, *probe_df;
-    const void *cfg_ptrs, *cfg_nslices, *wave_cfg, *wave_map, *tw, *px, *py, *lim, *kx, *ky;
+    int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices, mix_first, mix_sets;
+    unsigned long long mix_seed;
+    const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos, *probe_df, *mix_key;
+    const void *cfg_ptrs, *cfg_nslices, *wave_cfg, *wave_map, *tw, *px, *py, *wave_prop, *lim, *kx, *ky;
     mse_probe probe;
 };
 }  // namespace mse_rt
@@ -859,8 +1052,9 @@ int mse_multislice(mse_engine* e, int first, int count) {
 // Everything a captured slice loop depends on: launch shapes, kernel variants and every
 // device pointer baked into its kernel arguments.
 struct LoopGraphKey {
-    int res, nw, first, count, row_var, col_var, num_cus;
-    const void *waves, *cfg_ptrs, *cfg_nslices, *wave_cfg, *tw, *px, *py, *lim;
+    int res, nw, first, count, row_var, col_var, num_cus, mix_first, mix_sets;
+    unsigned long long mix_seed;
+    const void *waves, *cfg_ptrs, *cfg_nslices, *wave_cfg, *tw, *px, *py, *wave_prop, *lim, *mix_key;
 };
 
 int mse_multislice_graph(mse_engine* e, int first, int count) {
@@ -890,6 +1084,11 @@ int mse_multislice_graph(mse_engine* e, int first, int count) {
         k.tw = e->tw.p;
         k.px = e->px.p;
         k.py = e->py.p;
+        k.wave_prop = e->any_beam ? e->wave_prop.p : nullptr;
+        k.mix_first = e->mix_first;
+        k.mix_sets = e->mix_sets;
+        k.mix_seed = e->mix_seed;
+        k.mix_key = e->mix_key.p;
         k.lim = e->prop_lim.p;
         std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
         active_waves(e, e->n_waves);  // the per-slice wave sets are baked into the graph
@@ -929,16 +1128,28 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
         check_waves(e);
         ensure_detectors(e, dets, n_det);
         const int nw = e->n_waves;
-        e->partials.ensure((size_t)nw * n_det * (e->res / 8));
         e->det_out.ensure((size_t)nw * n_det);
-        mse::ColArgs ca = base_col_args(e);
-        ca.scale = (float)(1.0 / e->res);
-        ca.det_scale = 1.0;
-        ca.det_bits = e->det_bits.p;
-        ca.ndet = n_det;
-        ca.partials = e->partials.p;
-        e->det_tiles = col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
-        reduce_detectors(e, nw, n_det, sqrt2, e->det_out.p, 0);
+        if (e->any_beam) {
+            // plasmon-tilted waves: detectors over the translated diffraction images
+            // (getStemPixel with d_kx, d_ky: simulationstem.cpp:80-131)
+            const double* im = diffraction_images(e, 0, nw);
+            if (n_det > 0)
+                launch(e, PK_OTHER, [&]() {
+                    k_masked_sums<<<dim3(n_det, nw), 256, 0, e->stream>>>(im, e->N, e->det_bits.p, n_det,
+                                                                         sqrt2 ? std::sqrt(2.0) : 1.0, 0,
+                                                                         e->det_out.p);
+                });
+        } else {
+            e->partials.ensure((size_t)nw * n_det * (e->res / 8));
+            mse::ColArgs ca = base_col_args(e);
+            ca.scale = (float)(1.0 / e->res);
+            ca.det_scale = 1.0;
+            ca.det_bits = e->det_bits.p;
+            ca.ndet = n_det;
+            ca.partials = e->partials.p;
+            e->det_tiles = col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
+            reduce_detectors(e, nw, n_det, sqrt2, e->det_out.p, 0);
+        }
         HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)nw * n_det * sizeof(double), hipMemcpyDeviceToHost, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
     });
@@ -949,14 +1160,8 @@ int mse_diffraction(mse_engine* e, int wave, double* out) {
         REQUIRE(e && out, MSE_ERR_ARG, "NULL argument");
         check_waves(e);
         REQUIRE(wave >= 0 && wave < e->n_waves, MSE_ERR_ARG, "wave index out of range");
-        e->diff_img.ensure(e->N);
-        mse::ColArgs ca = base_col_args(e);
-        ca.scale = (float)(1.0 / e->res);
-        ca.diff = e->diff_img.p;
-        ca.diff_accumulate = 0;
-        const float2* w = e->waves.p + (size_t)wave * e->N;
-        col_pass(e, mse::COL_FFT_DIFF, w, nullptr, 1, e->N, ca);
-        HIPX(hipMemcpyAsync(out, e->diff_img.p, e->N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        const double* im = diffraction_images(e, wave, 1);  // translated when the beam moved
+        HIPX(hipMemcpyAsync(out, im, e->N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
     });
 }
@@ -990,11 +1195,16 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         ConfigData& c = config(e, cfg);
         REQUIRE(c.trans.p && c.built && c.trans.n >= (size_t)cfg_geom(e, c).n_slices * e->N, MSE_ERR_STATE,
                 "transmission functions of this configuration are not built for the current geometry");
+        check_mix(e);
+        REQUIRE(!c.own_beam, MSE_ERR_UNSUPPORTED,
+                "a configuration with its own beam vector runs through mse_multislice and mse_stem_detect "
+                "(its detector images are translated)");
         if (e->n_waves < batch || !e->waves.p) {
             HIPX(hipStreamSynchronize(e->stream));
             e->waves.ensure((size_t)batch * e->N);
             e->wave_cfg.ensure(batch);
             e->probe_pos.ensure(2 * (size_t)batch);
+            e->mix_key.ensure(batch);
             e->n_waves = batch;
         }
         std::vector<int> wc(batch, cfg);
@@ -1002,6 +1212,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             e->h_wave_cfg = wc;
             HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
         }
+        sync_wave_prop(e);
         ensure_detectors(e, dets, n_det);
         e->partials.ensure((size_t)batch * n_det * (e->res / 8));
         e->det_out.ensure((size_t)n_probes * n_det);
@@ -1038,6 +1249,15 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.tw = e->tw.p;
             k.px = e->px.p;
             k.py = e->py.p;
+            k.wave_prop = e->any_beam ? e->wave_prop.p : nullptr;
+            k.mix_first = e->mix_first;
+            k.mix_sets = e->mix_sets;
+            k.mix_seed = e->mix_seed;
+            k.mix_key = e->mix_key.p;
+        k.mix_first = e->mix_first;
+        k.mix_sets = e->mix_sets;
+        k.mix_seed = e->mix_seed;
+        k.mix_key = e->mix_key.p;
             k.lim = e->prop_lim.p;
             k.kx = e->kx.p;
             k.ky = e->ky.p;
@@ -1104,6 +1324,51 @@ void destroy_stem_graph(mse_engine* e) {
 }  // namespace mse_rt
 
 extern "C" {
+
+int mse_set_mixed_potentials(mse_engine* e, int first_config, int n_sets, unsigned long long seed) {
+    return run_guarded("mse_set_mixed_potentials", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(n_sets >= 0 && first_config >= 0 && (size_t)first_config + n_sets <= 4096, MSE_ERR_ARG,
+                "mixed potential sets out of range");
+        HIPX(hipStreamSynchronize(e->stream));
+        e->mix_first = n_sets >= 2 ? first_config : 0;
+        e->mix_sets = n_sets >= 2 ? n_sets : 0;
+        e->mix_seed = seed;
+        e->map_key.clear();  // the active wave sets depend on the mixing
+    });
+}
+
+int mse_set_config_beam(mse_engine* e, int cfg, const double* k_vec) {
+    return run_guarded("mse_set_config_beam", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        require_geom(e);
+        ConfigData& c = config(e, cfg);
+        HIPX(hipStreamSynchronize(e->stream));  // kernels in flight may read its tables
+        if (!k_vec) {
+            c.own_beam = false;
+        } else {
+            const double kx = k_vec[0], ky = k_vec[1], kz = k_vec[2];
+            REQUIRE(std::isfinite(kx) && std::isfinite(ky) && std::isfinite(kz) && kz > 0.0, MSE_ERR_ARG,
+                    "beam vector must be finite with kz > 0");
+            const double kk = std::sqrt(kx * kx + ky * ky + kz * kz);
+            REQUIRE(std::fabs(kk - e->g.wavenumber) <= 1e-9 * e->g.wavenumber, MSE_ERR_ARG,
+                    "beam vector length must equal the wavenumber");
+            c.k_vec[0] = kx;
+            c.k_vec[1] = ky;
+            c.k_vec[2] = kz;
+            std::vector<float2> hpx, hpy;
+            propagator_tables(e, c.k_vec, hpx, hpy);
+            c.px.ensure(e->res);
+            c.py.ensure(e->res);
+            HIPX(hipMemcpyAsync(c.px.p, hpx.data(), e->res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(c.py.p, hpy.data(), e->res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipStreamSynchronize(e->stream));
+            c.own_beam = true;
+        }
+        ++c.version;  // transmission functions built before no longer match its potentials
+        sync_wave_prop(e);
+    });
+}
 
 int mse_set_config_geometry(mse_engine* e, int cfg, const mse_geometry* g) {
     return run_guarded("mse_set_config_geometry", e, [&]() {
@@ -1263,14 +1528,10 @@ int mse_diffraction_sum(mse_engine* e, const int* wave_group, int n_groups, int 
         if (accumulate)
             HIPX(hipMemcpyAsync(e->det_out.p, out, (size_t)n_groups * e->N * sizeof(double), hipMemcpyHostToDevice,
                                 e->stream));
-        mse::ColArgs ca = base_col_args(e);
-        ca.scale = (float)(1.0 / e->res);
-        ca.diff = e->diff_img.p;
-        ca.diff_accumulate = 0;
-        col_pass(e, mse::COL_FFT_DIFF, e->waves.p, nullptr, nw, e->N, ca);
+        const double* im = diffraction_images(e, 0, nw);  // translated where a beam moved
         launch(e, PK_OTHER, [&]() {
             mse::k_sum_groups<<<(unsigned)((e->N + 255) / 256), 256, 0, e->stream>>>(
-                e->diff_img.p, nw, e->N, e->tmp_bid.p, n_groups, e->det_out.p, accumulate);
+                im, nw, e->N, e->tmp_bid.p, n_groups, e->det_out.p, accumulate);
         });
         HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)n_groups * e->N * sizeof(double), hipMemcpyDeviceToHost,
                             e->stream));

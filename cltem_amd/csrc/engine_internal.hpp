@@ -97,6 +97,10 @@ struct ConfigData {
     bool built = false;     // trans holds the transmission functions of the current geometry
     bool own_geom = false;  // z-geometry of its own (a specimen of another thickness)
     mse_geometry g{};
+    // beam wave vector of its own (plasmon tilt): tilted potentials and propagator tables
+    bool own_beam = false;
+    double k_vec[3] = {0.0, 0.0, 0.0};
+    DevBuf<float2> px, py;
 };
 
 enum ProfKind : int { PK_ROW = 0, PK_COL = 1, PK_POT = 2, PK_PROBE = 3, PK_OTHER = 4, PK_COUNT = 5 };
@@ -136,6 +140,17 @@ struct mse_engine {
     DevBuf<double> partials;
     DevBuf<double> det_out;
     DevBuf<double> diff_img;
+    DevBuf<double> diff_moved;   // bilinearly translated diffraction images (plasmon tilt)
+    DevBuf<double> wave_shift;   // per wave: on, ix, iy, sx, sy of the translation
+    // per wave: the px, py tables of its configuration's propagator, uploaded when some wave
+    // uses a configuration with a beam of its own (then any_beam; otherwise px, py for all)
+    DevBuf<const float2*> wave_prop;
+    std::vector<const float2*> h_wave_prop;
+    bool any_beam = false;
+    // mixed static potentials (mse_set_mixed_potentials): per wave, the key of its set draws
+    DevBuf<unsigned> mix_key;
+    int mix_first = 0, mix_sets = 0;
+    unsigned long long mix_seed = 0;
     DevBuf<float2> scratch;
     DevBuf<float2> img_tmp;  // CTEM image formation work copy of the waves
     DevBuf<float> ccd_filter;  // sqrt(DQE) and NTF factors per pixel (2 res*res)
@@ -217,6 +232,8 @@ int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, 
              const mse::ColArgs& args, int kind = PK_COL, int variant = -1);
 // one forward slice (row pass with t, column pass with the propagator) on nw waves in place
 void slice_step(mse_engine* e, float2* waves, int nw, int slice);
+// refresh wave_prop / any_beam after the waves' configurations or a configuration's beam changed
+void sync_wave_prop(mse_engine* e);
 
 // ---- sweeps ------------------------------------------------------------------------------
 // Kernel variants (chosen by mse_autotune or mse_set_kernel_variant):

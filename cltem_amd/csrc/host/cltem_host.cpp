@@ -155,6 +155,80 @@ void ChromaticSpread::setHalfWidthHalfMaxs(double hwhm_neg, double hwhm_pos) {
 
 void SourceSize::setFullWidthHalfMax(double fwhm) { sigma = fwhm / (2.0 * std::sqrt(2.0 * std::log(2.0))); }
 
+std::vector<double> PlasmonScattering::depths(uint64_t seed, int job, double thickness) const {
+    const int attempts = individual > 0 ? (1 << 20) : 1;
+    for (int a = 0; a < attempts; ++a) {
+        std::vector<double> d;
+        double t = 0.0;
+        for (uint32_t k = 0;; ++k) {
+            double u[4];
+            uniforms4(seed, (uint32_t)job, 0x9Au + 256u * (uint32_t)a, k, 4u, u);
+            t += -mean_free_path * std::log(u[0]);
+            if (t > thickness) break;
+            d.push_back(t);
+            if (individual > 0 && (int)d.size() > individual) break;
+        }
+        if (individual <= 0 || (int)d.size() == individual) return d;
+    }
+    throw std::runtime_error("Could not generate valid plasmon configuration.");
+}
+
+void PlasmonScattering::angles(uint64_t seed, int job, int n, double* polar_mrad, double* azimuth) const {
+    double u[4];
+    uniforms4(seed, (uint32_t)job, 0xA5u, (uint32_t)n, 5u, u);
+    const double ce2 = characteristic_angle * characteristic_angle;
+    const double cc2 = critical_angle * critical_angle;
+    *polar_mrad = std::sqrt(ce2 * (std::pow((cc2 + ce2) / ce2, u[0]) - 1.0));
+    *azimuth = 2.0 * M_PI * u[1];
+}
+
+static void rotation_about(const double* ax, double angle, double R[3][3]) {
+    const double n = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    const double a[3] = {ax[0] / n, ax[1] / n, ax[2] / n};
+    const double C[3][3] = {{0.0, -a[2], a[1]}, {a[2], 0.0, -a[0]}, {-a[1], a[0], 0.0}};
+    const double s = std::sin(angle), c1 = 1.0 - std::cos(angle);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double cc = 0.0;
+            for (int k = 0; k < 3; ++k) cc += C[i][k] * C[k][j];
+            R[i][j] = (i == j ? 1.0 : 0.0) + C[i][j] * s + cc * c1;
+        }
+}
+
+void rotate_vector_spherical(double z[3], double y[3], double theta, double phi) {
+    double rt[3][3], rp[3][3];
+    rotation_about(y, theta, rt);
+    rotation_about(z, phi, rp);
+    auto apply = [](const double R[3][3], double* v) {
+        const double r0 = R[0][0] * v[0] + R[0][1] * v[1] + R[0][2] * v[2];
+        const double r1 = R[1][0] * v[0] + R[1][1] * v[1] + R[1][2] * v[2];
+        const double r2 = R[2][0] * v[0] + R[2][1] * v[1] + R[2][2] * v[2];
+        v[0] = r0;
+        v[1] = r1;
+        v[2] = r2;
+    };
+    apply(rt, z);
+    apply(rp, z);
+    apply(rp, y);
+}
+
+int SimulationManager::mixedSets() const {
+    // clTEM needs frozen phonons, precalculated transmission functions (no plasmons) and a
+    // count below the number of iterations (simulationmanager.h:218-231)
+    const bool phonons = phonon_configs > 1 && u_rms > 0.0;
+    if (!phonons || plasmon.enabled || parallel_potentials < 2) return 0;
+    if (scenario.mode != 2 && parallel_potentials >= phonon_configs) return 0;
+    return parallel_potentials;
+}
+
+uint64_t SimulationManager::mixedSeed(int job) const {
+    const uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)job, 0x3Bu, 0u, 7u};
+    uint32_t r[4];
+    philox4x32_10(ctr, key, r);
+    return (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+}
+
 double SimulationManager::focusChange(int job, int probe) const {
     if (!chromatic.enabled) return 0.0;
     double u[4];
@@ -270,12 +344,35 @@ void SimulationGeneral::sortAtoms() {
 }
 
 bool SimulationGeneral::initialiseSimulation() {
+    const int sets = sm->mixedSets();
+    if (sets >= 2 && sets_for == sm.get() && !sm->source.enabled) {
+        // the shared sets of this manager are built: only the job's draws change
+        // (simulationgeneral.cpp:358-362 reuses them the same way)
+        check(mse_set_mixed_potentials(eng, 0, sets, sm->mixedSeed(config)));
+        return true;
+    }
     geom = sm->geometry();
-    check(mse_set_geometry(eng, &geom));
+    check(mse_set_geometry(eng, &geom));  // every configuration back to the geometry's beam
+    beam_moved = false;
+    sets_for = nullptr;
     int form = 0, per = 0;
     std::vector<double> values;
     load_params(sm->params_file, &form, &per, &values);
     check(mse_set_parameterisation(eng, form, per, values.data(), (int)values.size()));
+    if (sets >= 2) {
+        // one phonon configuration per set (simulationgeneral.cpp:576-612), in configurations 0..sets-1
+        for (int j = 0; j < sets; ++j) {
+            const AtomList a = displaced(sm->atoms, sm->u_rms, sm->seed, j);
+            check(mse_sort_atoms(eng, j, a.x.data(), a.y.data(), a.z.data(), a.Z.data(), (int)a.size(), nullptr,
+                                 nullptr, nullptr, nullptr));
+            check(mse_set_reference_offset(eng, j, reference_perturb_x, reference_perturb_y));
+            check(mse_build_transmission(eng, j, 0, geom.n_slices));
+        }
+        check(mse_set_mixed_potentials(eng, 0, sets, sm->mixedSeed(config)));
+        sets_for = sm.get();
+        return true;
+    }
+    check(mse_set_mixed_potentials(eng, 0, 0, 0));
     sortAtoms();
     // simulationgeneral.cpp:559-560: the potential grid origin carries the source-size offset
     check(mse_set_reference_offset(eng, 0, reference_perturb_x, reference_perturb_y));
@@ -284,6 +381,50 @@ bool SimulationGeneral::initialiseSimulation() {
 }
 
 void SimulationGeneral::doMultiSliceStep(int slice) { check(mse_multislice(eng, slice, 1)); }
+
+void SimulationGeneral::resetBeam() {
+    if (!beam_moved) return;
+    check(mse_set_config_beam(eng, 0, nullptr));
+    check(mse_build_transmission(eng, 0, 0, geom.n_slices));
+    beam_moved = false;
+}
+
+int SimulationGeneral::runSlices(int job) {
+    const int n = geom.n_slices;
+    if (!sm->plasmon.enabled) {
+        check(mse_multislice(eng, 0, n));
+        return 0;
+    }
+    // the structure's own thickness (console/main.cpp:711-717)
+    double zmin = 0.0, zmax = 0.0;
+    if (sm->atoms.size() > 0) {
+        zmin = *std::min_element(sm->atoms.z.begin(), sm->atoms.z.end());
+        zmax = *std::max_element(sm->atoms.z.begin(), sm->atoms.z.end());
+    }
+    const std::vector<double> depths = sm->plasmon.depths(sm->seed, job, zmax - zmin);
+    // beam vector and the azimuth axis at the top (simulationcbed.cpp:152-162)
+    double k[3] = {0.0, 0.0, geom.wavenumber}, yax[3] = {0.0, 1.0, 0.0};
+    rotate_vector_spherical(k, yax, geom.beam_tilt_mrad / 1000.0, geom.beam_azimuth);
+    size_t next = 0;
+    int first = 0, events = 0;
+    for (int i = 0; i < n; ++i) {
+        const double depth = (i + 1 - geom.pre_slices) * geom.slice_thickness;
+        if (next < depths.size() && depth >= depths[next]) {
+            check(mse_multislice(eng, first, i + 1 - first));
+            first = i + 1;
+            double polar = 0.0, azimuth = 0.0;
+            sm->plasmon.angles(sm->seed, job, (int)next, &polar, &azimuth);
+            rotate_vector_spherical(k, yax, polar / 1000.0, azimuth);
+            check(mse_set_config_beam(eng, 0, k));
+            beam_moved = true;
+            if (i + 1 < n) check(mse_build_transmission(eng, 0, i + 1, n - i - 1));
+            ++next;
+            ++events;
+        }
+    }
+    if (first < n) check(mse_multislice(eng, first, n - first));
+    return events;
+}
 
 std::vector<double> SimulationGeneral::getDiffractionImage(int parallel_ind) {
     std::vector<double> out((size_t)geom.resolution * geom.resolution);
@@ -324,7 +465,7 @@ bool SimulationCtem::initialiseSimulation() {
 
 void SimulationCtem::simulate() {
     initialiseSimulation();
-    for (int i = 0; i < geom.n_slices; ++i) doMultiSliceStep(i);
+    runSlices(config);
     const int res = geom.resolution;
     std::map<std::string, Image> ims;
     Image ew(res, res, 1), diff(res, res, 1);
@@ -359,7 +500,7 @@ void SimulationCbed::simulate() {
     n_waves = 1;
     // CbedPosition: the mode's area is the point (x, x, y, y)
     initialiseProbeWave(sm->scenario.area[0], sm->scenario.area[2]);
-    for (int i = 0; i < geom.n_slices; ++i) doMultiSliceStep(i);
+    runSlices(config);
     const int res = geom.resolution;
     std::map<std::string, Image> ims;
     Image diff(res, res, 1);
@@ -399,9 +540,33 @@ void SimulationStem::simulate() {
     std::vector<double> df(sm->chromatic.enabled ? pixels.size() : 0);
     for (size_t i = 0; i < df.size(); ++i) df[i] = sm->focusChange(config, pixels[i]);
     check(mse_set_focus_spread(eng, df.data(), (int)df.size()));
-    check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
-                         sm->sqrt2 ? 1 : 0, 1, acc.data()));
-    n_waves = batch;
+    if (!sm->plasmon.enabled) {
+        check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
+                             sm->sqrt2 ? 1 : 0, 1, acc.data()));
+        n_waves = batch;
+    } else {
+        // every batch of pixels is one job with its own scattering history, as each of
+        // clTEM's STEM jobs is (simulationstem.cpp:190-232); detectors read the diffraction
+        // pattern moved back by the beam change (getStemPixel with d_kx, d_ky)
+        const int n_batches = ((int)pixels.size() + batch - 1) / batch;
+        std::vector<double> out;
+        for (int b = 0; b < n_batches; ++b) {
+            const int p0 = b * batch, nb = std::min(batch, (int)pixels.size() - p0);
+            resetBeam();
+            if (n_waves != nb) {
+                check(mse_set_wave_count(eng, nb));
+                std::vector<int> wc(nb, 0);
+                check(mse_set_wave_configs(eng, wc.data(), nb));
+                n_waves = nb;
+            }
+            if (!df.empty()) check(mse_set_focus_spread(eng, df.data() + p0, nb));
+            check(mse_init_probes(eng, pos.data() + 2 * (size_t)p0, nb, &sm->probe));
+            runSlices(config * n_batches + b);
+            out.assign((size_t)nb * nd, 0.0);
+            check(mse_stem_detect(eng, dets.data(), nd, sm->sqrt2 ? 1 : 0, out.data()));
+            std::copy(out.begin(), out.end(), acc.begin() + (size_t)p0 * nd);
+        }
+    }
     std::map<std::string, Image> ims;
     for (int d = 0; d < nd; ++d) {
         Image im(nx, ny, 1);

@@ -104,6 +104,23 @@ struct SourceSize {
     void setFullWidthHalfMax(double fwhm);   // probesourcesize.cpp:17-19
 };
 
+// PlasmonScattering (incoherence/inelastic/plasmon.h:25-100) with Philox draws keyed by the
+// manager's seed in place of its clock-seeded mt19937: counter (job, 0x9A + 256 attempt, k, 4)
+// for the k-th free path of an attempt, (job, 0xA5, n, 5) for the angles of scattering n.
+struct PlasmonScattering {
+    bool enabled = false;
+    int individual = 0;              // 0: PlasmonType::Full; n > 0: exactly n scatterings
+    double mean_free_path = 100.0;   // Angstrom (clTEM's settings hold nm, times 10)
+    double characteristic_angle = 1.0, critical_angle = 0.1;  // mrad
+    // generateScatteringDepths (plasmon.cpp:37-127): depths below the specimen top
+    std::vector<double> depths(uint64_t seed, int job, double thickness) const;
+    // getScatteringPolar (mrad) and getScatteringAzimuth of scattering n (plasmon.cpp:14-29)
+    void angles(uint64_t seed, int job, int n, double* polar_mrad, double* azimuth) const;
+};
+
+// Utils::rotateVectorSpherical (utilities/vectorutils.cpp:8-31)
+void rotate_vector_spherical(double z[3], double y[3], double theta, double phi);
+
 // The inputs clTEM's SimulationManager holds for this path, and its result merging.
 struct SimulationManager {
     mse_scenario scenario{};   // mode, resolution, voltage, slicing, padding, areas, tilt
@@ -119,12 +136,20 @@ struct SimulationManager {
     int device = 0;
     ChromaticSpread chromatic;
     SourceSize source;
+    PlasmonScattering plasmon;
+    // mixed static potentials (setParallelPotentialsCount): with phonons and >= 2, jobs share
+    // this many phonon configurations and draw one per probe and slice
+    int parallel_potentials = 0;
 
     // ChromaticAberration::getFocusChange for probe `probe` of job `job` (Angstrom): an
     // asymmetric Gaussian in energy, 10000 * Cc[mm] * dE / kV (chromaticaberration.cpp:43-58, 99-106)
     double focusChange(int job, int probe) const;
     // ProbeSourceSize::getOffset for x and y of job `job` (Angstrom)
     void sourceOffset(int job, double* dx, double* dy) const;
+    // parallelPotentialsCount (simulationmanager.h:218-231): the number of shared sets, or 0
+    int mixedSets() const;
+    // seed of job `job`'s set draws: Philox4x32-10 words 0, 1 at (job, 0x3B, 0, 7)
+    uint64_t mixedSeed(int job) const;
 
     // SimulationManager::updateImages: FP64 sum of every job's images and weights
     void updateImages(std::map<std::string, Image>& ims, int jobCount);
@@ -155,6 +180,12 @@ class SimulationGeneral {
     void sortAtoms();
     bool initialiseSimulation();
     void doMultiSliceStep(int slice);
+    // the slice loop of simulate() (simulationcbed.cpp:142-199): with plasmons, after the slice
+    // whose end reaches the next scattering depth the beam vector is rotated and the job's
+    // propagator and remaining transmission functions follow it (modifyBeamTilt).  `job` keys
+    // the scattering draws.  Returns the number of scatterings applied.
+    int runSlices(int job);
+    void resetBeam();  // the geometry's beam again, with its transmission functions rebuilt
     std::vector<double> getDiffractionImage(int parallel_ind);
     std::vector<double> getExitWaveImage(unsigned t = 0, unsigned l = 0, unsigned b = 0, unsigned r = 0);
 
@@ -165,6 +196,8 @@ class SimulationGeneral {
     std::vector<int> pixels;
     mse_geometry geom{};
     int n_waves = 0;
+    bool beam_moved = false;  // a plasmon scattering changed configuration 0's beam
+    const SimulationManager* sets_for = nullptr;  // manager whose mixed potential sets are built
 };
 
 class SimulationCtem : public SimulationGeneral {

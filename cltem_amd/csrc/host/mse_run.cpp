@@ -6,11 +6,16 @@
 //           [--area x0,x1,y0,y1] [--stem-pixels N] [--det name,inner,outer[,xc,yc]]...
 //           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
 //           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
+//           [--plasmon mfp_nm,characteristic_mrad,critical_mrad[,n_individual]] [--parallel-potentials N]
 //           [--device D] [--tiff] --out PREFIX
 //   mse_run ... --dump-atoms FILE [--dump-config C]   (writes the displaced atoms of phonon
 //           configuration C as x,y,z doubles and Z ints, and exits without using the GPU)
 //   mse_run ... --dump-incoherence N [--dump-config C]   (prints the focus changes of probes
 //           0..N-1 and the source offset of job C, one number per line, and exits)
+//   mse_run ... --dump-plasmons THICKNESS [--dump-config C]   (prints the scattering count,
+//           depths, then polar (mrad) and azimuth of each scattering of job C, and exits)
+//   mse_run ... --dump-mixed-seed [--dump-config C]   (prints the seed of job C's mixed
+//           potential draws and exits)
 //   mse_run --write-test-tiff FILE W H   (writes the ramp x + 1000 y, W x H, and exits)
 //
 // Writes PREFIX_<image>.bin (doubles, averaged over the phonon configurations as
@@ -55,6 +60,8 @@ int main(int argc, char** argv) {
         std::string dump;
         int dump_config = 0;
         int dump_incoherence = -1;
+        double dump_plasmons = -1.0;
+        bool dump_mixed = false;
         bool tiff = false;
         std::string cif_path;
         double zone[3] = {0, 0, 1}, normal[3] = {0, 0, 0}, widths[3] = {0, 0, 0}, tilts[3] = {0, 0, 0};
@@ -118,6 +125,19 @@ int main(int argc, char** argv) {
                 sm->chromatic.enabled = true;
                 sm->chromatic.cc_mm = v.at(0);
                 sm->chromatic.setHalfWidthHalfMaxs(v.at(1), v.at(2));
+            } else if (a == "--plasmon") {
+                auto v = split_doubles(next());
+                sm->plasmon.enabled = true;
+                sm->plasmon.mean_free_path = v.at(0) * 10.0;  // nm -> Angstrom (jsonutils.cpp:350)
+                sm->plasmon.characteristic_angle = v.at(1);
+                sm->plasmon.critical_angle = v.at(2);
+                sm->plasmon.individual = v.size() > 3 ? (int)v[3] : 0;
+            } else if (a == "--parallel-potentials") {
+                sm->parallel_potentials = std::stoi(next());
+            } else if (a == "--dump-mixed-seed") {
+                dump_mixed = true;
+            } else if (a == "--dump-plasmons") {
+                dump_plasmons = std::stod(next());
             } else if (a == "--source-size") {
                 sm->source.enabled = true;
                 sm->source.setFullWidthHalfMax(std::stod(next()));
@@ -155,6 +175,23 @@ int main(int argc, char** argv) {
             if (widths[0] <= 0 || widths[1] <= 0 || widths[2] <= 0)
                 throw std::runtime_error("--cif needs --widths x,y,z (Angstrom)");
             sm->atoms = cif_supercell(read_cif(cif_path), zone, normal, widths, tilts, sm->seed);
+        }
+        if (dump_mixed) {
+            std::cout << sm->mixedSeed(dump_config) << std::endl;
+            return 0;
+        }
+        if (dump_plasmons >= 0.0) {
+            std::cout.precision(17);
+            const std::vector<double> d = sm->plasmon.depths(sm->seed, dump_config, dump_plasmons);
+            std::cout << d.size() << "\n";
+            for (double v : d) std::cout << v << "\n";
+            for (size_t i = 0; i < d.size(); ++i) {
+                double polar, azimuth;
+                sm->plasmon.angles(sm->seed, dump_config, (int)i, &polar, &azimuth);
+                std::cout << polar << "\n" << azimuth << "\n";
+            }
+            std::cout << std::flush;
+            return 0;
         }
         if (dump_incoherence >= 0) {
             std::cout.precision(17);

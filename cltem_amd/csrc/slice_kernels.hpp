@@ -15,6 +15,7 @@
 // sums are taken from it in the last slice's column pass.
 #pragma once
 #include "fft_wave.hpp"
+#include "philox.hpp"
 
 namespace mse {
 
@@ -57,7 +58,23 @@ struct RowArgs {
     int slice;
     float plane_value;               // ROW_PLANE_T_FFT: psi amplitude times res
     float scale;                     // ROW_FFT / ROW_IFFT output scale
+    // mixed static potentials: with mix_sets >= 2 a wave reads configuration mix_first + a
+    // set drawn per (mix_key[wave], slice) instead of its own (row_config)
+    const unsigned* mix_key;
+    uint2 mix_seed;
+    int mix_first, mix_sets;
 };
+
+// The configuration whose transmission function `wave` multiplies at slice a.slice: its own,
+// or under mixed static potentials floor(n r / 2^32) past mix_first with r the first word of
+// Philox4x32-10 (key mix_seed) at counter (mix_key[wave], slice, 0x3A, 6).
+__device__ __forceinline__ int row_config(const RowArgs& a, int wave) {
+    if (a.mix_sets > 1) {
+        const uint4 r = philox4x32_10(make_uint4(a.mix_key[wave], (uint32_t)a.slice, 0x3Au, 6u), a.mix_seed);
+        return a.mix_first + (int)(((uint64_t)r.x * (uint64_t)a.mix_sets) >> 32);
+    }
+    return a.wave_cfg[wave];
+}
 
 struct ProbeArgs {
     const double* pos;   // per wave: (x, y) relative to the padded start, Angstrom
@@ -74,6 +91,7 @@ struct ColArgs {
     const int* wave_map;     // set -> wave index (NULL: identity); skips finished waves
     const float2* px;        // propagator factors (COL_PROP*)
     const float2* py;
+    const float2* const* wave_prop;  // per wave: its own px, py (beam tilt); NULL: px, py for all
     const int* lim;          // per ky row: largest |kx index| kept (mask), -1 for none
     const float* filter;     // COL_FILTER: res*res real factors, row-major [ky][kx] (unshifted)
     const double* kx;        // k-grid (COL_PROBE)
@@ -143,8 +161,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             fft_line<PL, true>(ps, ln, t, tws, sync);
             sync();
             float2* gt = a.aux_out + (size_t)wave * wave_stride + (size_t)row * N;
-            const float2* tr = a.cfg_trans[a.wave_cfg[wave]] + (size_t)a.slice * N * N + (size_t)row * N;
-            const bool has_t = a.slice < a.cfg_nslices[a.wave_cfg[wave]];
+            const int cfg = row_config(a, wave);
+            const float2* tr = a.cfg_trans[cfg] + (size_t)a.slice * N * N + (size_t)row * N;
+            const bool has_t = a.slice < a.cfg_nslices[cfg];
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 const float2 u = make_float2(v[m].x * inv, v[m].y * inv);
@@ -175,7 +194,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         return;
     }
     if constexpr (MODE == ROW_IFFT_T_FFT || MODE == ROW_PLANE_T_FFT) {
-        const int cfg = a.wave_cfg[wave];
+        const int cfg = row_config(a, wave);
         if (a.slice < a.cfg_nslices[cfg]) {  // below a thinner specimen the slice is vacuum
             const float2* tr = a.cfg_trans[cfg] + (size_t)a.slice * N * N + (size_t)row * N;
 #pragma unroll
@@ -358,13 +377,21 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
                   MODE == COL_BAND || MODE == COL_PROP_CONJ) {
         const int xs = (x < N / 2) ? x : N - x;  // |signed kx index|
         float2 pxv = make_float2(a.scale, 0.f);
-        if constexpr (MODE != COL_BAND) pxv = a.px[x];
+        const float2* pyt = a.py;
+        if constexpr (MODE != COL_BAND) {
+            const float2* pxt = a.px;
+            if (a.wave_prop) {
+                pxt = a.wave_prop[2 * wave];
+                pyt = a.wave_prop[2 * wave + 1];
+            }
+            pxv = pxt[x];
+        }
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + L * m;
             const bool keep = xs <= a.lim[y];
             float2 f = pxv;
-            if constexpr (MODE != COL_BAND) f = cmul(pxv, a.py[y]);
+            if constexpr (MODE != COL_BAND) f = cmul(pxv, pyt[y]);
             if constexpr (MODE == COL_PROP_CONJ) f.y = -f.y;
             v[m] = keep ? cmul(v[m], f) : make_float2(0.f, 0.f);
         }
@@ -497,13 +524,19 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
         fft_line<PL, false>(v, ln, t, tws, sync);
         {
             const int xs = (x < N / 2) ? x : N - x;
-            const float2 pxv = a.px[x];
+            const float2* pxt = a.px;
+            const float2* pyt = a.py;
+            if (a.wave_prop) {
+                pxt = a.wave_prop[2 * wv];
+                pyt = a.wave_prop[2 * wv + 1];
+            }
+            const float2 pxv = pxt[x];
             const int lim = a.lim[xs];  // the aperture mask is symmetric in kx and ky
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 const int y = t + L * m;
                 const int ys = (y < N / 2) ? y : N - y;
-                v[m] = ys <= lim ? cmul(v[m], cmul(pxv, a.py[y])) : make_float2(0.f, 0.f);
+                v[m] = ys <= lim ? cmul(v[m], cmul(pxv, pyt[y])) : make_float2(0.f, 0.f);
             }
         }
         sync();
@@ -550,7 +583,7 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
     auto fetch = [&](int item) {
         const int wv = a.wave_map ? a.wave_map[item / N] : item / N, row = item % N;
         const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
-        const int cfg = a.wave_cfg[wv];
+        const int cfg = row_config(a, wv);
 #pragma unroll
         for (int m = 0; m < E; ++m) qv[m] = src[t + L * m];
         if (a.slice < a.cfg_nslices[cfg]) {

@@ -47,3 +47,104 @@ def source_offset(seed: int, job: int, sigma: float):
 
 def source_sigma_from_fwhm(fwhm: float) -> float:
     return fwhm / _FWHM
+
+
+# ---- plasmon scattering (incoherence/inelastic/plasmon.cpp) --------------------------------
+# Counters: (job, 0x9A + 256 * attempt, k, 4) for the k-th free path of an attempt; (job, 0xA5,
+# n, 5) for the polar (word 0) and azimuthal (word 1) angles of the n-th scattering.
+_DEPTH_ATTEMPTS = 1 << 20
+
+
+def plasmon_depths(seed: int, job: int, thickness: float, mean_free_path: float, individual: int = 0):
+    """generateScatteringDepths: depths (Angstrom below the specimen top) of one job's plasmon
+    scatterings, each free path -mfp ln(u) (getScatteringDistance, plasmon.cpp:9-12), while the
+    running depth stays <= thickness (getDistancesforCombined, plasmon.cpp:100-127).  With
+    individual = n > 0 the draw is repeated until exactly n scatterings fall inside the specimen
+    (getDistancesForIndividual); raises ValueError when no attempt gives that."""
+    for attempt in range(_DEPTH_ATTEMPTS if individual > 0 else 1):
+        depths = []
+        k = 0
+        t = 0.0
+        while True:
+            u = _uniforms(seed, [[job, 0x9A + 256 * attempt, k, 4]])[0][0]
+            t += -mean_free_path * np.log(u)
+            k += 1
+            if t > thickness:
+                break
+            depths.append(t)
+            if individual > 0 and len(depths) > individual:
+                break
+        if individual <= 0 or len(depths) == individual:
+            return depths
+    raise ValueError("could not generate a plasmon configuration with the requested scattering count")
+
+
+def plasmon_angles(seed: int, job: int, n: int, characteristic_mrad: float, critical_mrad: float):
+    """Polar angle (mrad) and azimuth (rad) of scatterings 0..n-1 of a job: getScatteringPolar
+    theta^2 = theta_E^2 (((theta_c^2 + theta_E^2) / theta_E^2)^u - 1) and getScatteringAzimuth
+    2 pi u (plasmon.cpp:14-29)."""
+    if n <= 0:
+        return np.zeros(0), np.zeros(0)
+    u = _uniforms(seed, np.stack([np.full(n, job), np.full(n, 0xA5), np.arange(n), np.full(n, 5)], axis=1))
+    ce2 = characteristic_mrad ** 2
+    cc2 = critical_mrad ** 2
+    polar = np.sqrt(ce2 * (np.power((cc2 + ce2) / ce2, u[:, 0]) - 1.0))
+    return polar, 2.0 * np.pi * u[:, 1]
+
+
+def rotation_about(axis, angle):
+    """generateRotationAroundVector (utilities/vectorutils.cpp:8-18)."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    c = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    return np.eye(3) + c * np.sin(angle) + c @ c * (1.0 - np.cos(angle))
+
+
+def rotate_vector_spherical(z, y, theta, phi):
+    """rotateVectorSpherical (utilities/vectorutils.cpp:21-31): returns the new (z, y)."""
+    r_theta = rotation_about(y, theta)
+    r_phi = rotation_about(z, phi)
+    z = r_phi @ (r_theta @ np.asarray(z, dtype=np.float64))
+    y = r_phi @ np.asarray(y, dtype=np.float64)
+    return z, y
+
+
+def initial_beam(wavenumber: float, tilt_mrad: float, azimuth: float):
+    """(k_vec, y axis) at the top of the specimen (simulationcbed.cpp:152-162)."""
+    return rotate_vector_spherical([0.0, 0.0, wavenumber], [0.0, 1.0, 0.0], tilt_mrad / 1000.0, azimuth)
+
+
+def plasmon_events(depths, pre_slices: int, dz: float, n_slices: int):
+    """Slice after which each scattering is applied: the first slice i whose end, (i + 1 -
+    pre_slices) dz below the top, reaches the scattering depth, at most one per slice
+    (simulationcbed.cpp:171-188).  Scatterings past the last slice are dropped."""
+    out = []
+    j = 0
+    for i in range(n_slices):
+        if j < len(depths) and (i + 1 - pre_slices) * dz >= depths[j]:
+            out.append(i)
+            j += 1
+    return out
+
+
+# ---- mixed static potentials (microscope/simulationgeneral.cpp:731-745) ---------------------
+
+def _key(seed: int) -> np.ndarray:
+    return np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF], dtype=np.uint64)
+
+
+def mixed_potential_picks(seed: int, keys, n_slices: int, n_sets: int) -> np.ndarray:
+    """Set drawn by each wave key at each slice, as the engine's row pass draws it:
+    floor(n_sets r / 2^32), r the first word of Philox4x32-10 at (key, slice, 0x3A, 6)."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    k, s = np.meshgrid(keys, np.arange(n_slices, dtype=np.uint64), indexing="ij")
+    ctr = np.stack([k.ravel(), s.ravel(), np.full(k.size, 0x3A, np.uint64), np.full(k.size, 6, np.uint64)], axis=1)
+    r = philox4x32(ctr, _key(seed))[:, 0].astype(np.uint64)
+    return ((r * np.uint64(n_sets)) >> np.uint64(32)).astype(np.int64).reshape(len(keys), n_slices)
+
+
+def mixed_seed(seed: int, job: int) -> int:
+    """Seed of one job's set draws (the C++ host's SimulationManager::mixedSeed): words 0 and 1
+    of Philox4x32-10 keyed by the manager's seed at (job, 0x3B, 0, 7)."""
+    r = philox4x32(np.array([[job, 0x3B, 0, 7]], dtype=np.uint64), _key(seed))[0]
+    return int(r[0]) | (int(r[1]) << 32)

@@ -157,6 +157,32 @@ int mse_read_transmission(mse_engine* e, int config, int slice, float* out);
  *      SimulationGeneral for every specimen (simulationgeneral.cpp:345-391). ---- */
 int mse_set_config_geometry(mse_engine* e, int config, const mse_geometry* g);
 
+/* ---- plasmon beam tilt: SimulationGeneral::modifyBeamTilt (microscope/simulationgeneral.cpp:649-679)
+ *      as a per-configuration state.  k_vec (1/Angstrom, |k_vec| = the wavenumber, kz > 0) becomes
+ *      the configuration's beam vector; NULL restores the geometry's.  Transmission functions
+ *      built afterwards use its tilt and azimuth (projected: beam_theta/beam_phi of
+ *      transmission_potentials_projected; full 3D: the per-integral shifts (kx, ky)/kz * dz/integrals),
+ *      and every wave of the configuration propagates with propagator_d at k_vec.  Diffraction
+ *      images, their sums and STEM detector values of such waves are bilinearly translated by
+ *      (k_vec - geometry k_vec) / inverse_scale pixels (translateDiffImage, simulationgeneral.cpp:832-854,
+ *      kernels/bilinear_translate_d.cl).  Rebuild the configuration's transmission functions
+ *      for the slices still to come (mse_build_transmission).  Not reproduced: clTEM passes the
+ *      projected tilt as a 30th argument of a 29-argument kernel, and its full-3D y shift uses kx.
+ *      mse_stem_batch refuses such a configuration (use mse_multislice + mse_stem_detect). ---- */
+int mse_set_config_beam(mse_engine* e, int config, const double* k_vec);
+
+/* ---- mixed static potentials: doMultiSliceStep with useParallelPotentials
+ *      (microscope/simulationgeneral.cpp:731-745; the sets are built per phonon configuration
+ *      at simulationgeneral.cpp:576-612).  With n_sets >= 2, at every slice every wave
+ *      multiplies the transmission function of configuration first_config + floor(n_sets r / 2^32),
+ *      r the first word of Philox4x32-10 keyed by seed at counter (wave key, slice, 0x3A, 6);
+ *      a wave's key is the probe index of its last probe set-up (mse_init_probes*: the index
+ *      into pos_xy; mse_stem_batch: the probe's index over all batches), else its wave index.
+ *      clTEM draws the set from a clock-seeded generator; this draw is reproducible.  The sets
+ *      must use the engine's geometry and beam.  n_sets 0 or 1: each wave uses its own
+ *      configuration again.  Position gradients refuse mixed sets. ---- */
+int mse_set_mixed_potentials(mse_engine* e, int first_config, int n_sets, unsigned long long seed);
+
 /* ---- waves: clWaveFunctionReal/Recip[parallel_pixels] (simulationgeneral.h:85-86) ---- */
 int mse_set_wave_count(mse_engine* e, int n_waves);
 /* wave -> frozen-phonon configuration whose transmission functions it uses (default 0) */

@@ -989,6 +989,81 @@ int orc_diffraction(int res, const double* recip, double* out) {
     return 0;
 }
 
+// translateDiffImage (simulationgeneral.cpp:832-854): shift = d_k / inverseScale split into a
+// floor and a fraction; bilinear_translate_d (kernels/bilinear_translate_d.cl:21-72): the pixel
+// taken from (x - ix, y - iy) wrapped once (further: 0), blended with its left, upper and
+// upper-left neighbours with the factors of the opposite quadrants.
+int orc_translate_image(int res, const double* in, double* out, double dkx, double dky, double inverse_scale) {
+    const double shift_x = dkx / inverse_scale, shift_y = dky / inverse_scale;
+    const int ix = (int)std::floor(shift_x), iy = (int)std::floor(shift_y);
+    const double sx = shift_x - ix, sy = shift_y - iy;
+    const int W = res, H = res;
+    for (int yid = 0; yid < H; ++yid)
+        for (int xid = 0; xid < W; ++xid) {
+            const int id = xid + W * yid;
+            int nx = xid - ix, ny = yid - iy;
+            if (nx < 0)
+                nx = W + nx;
+            else if (nx >= W)
+                nx = nx - W;
+            if (ny < 0)
+                ny = H + ny;
+            else if (ny >= H)
+                ny = ny - H;
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) {
+                out[id] = 0.0;
+                continue;
+            }
+            const int nid = nx + W * ny;
+            int br = nid, bl = nid - 1, tr = nid - W, tl = nid - W - 1;
+            if (nx == 0) {
+                bl += W;
+                tl += W;
+            }
+            if (ny == 0) {
+                tr += W * H;
+                tl += W * H;
+            }
+            const double f_br = sx * sy, f_bl = (1.0 - sx) * sy, f_tr = sx * (1.0 - sy),
+                         f_tl = (1.0 - sx) * (1.0 - sy);
+            out[id] = f_tl * in[br] + f_tr * in[bl] + f_bl * in[tr] + f_br * in[tl];
+        }
+    return 0;
+}
+
+// generateRotationAroundVector (utilities/vectorutils.cpp:8-18): I + sin(a) C + (1 - cos(a)) C^2
+// with C the cross-product matrix of the normalised axis
+static void rotation_about(const double* ax_in, double angle, double R[3][3]) {
+    const double n = std::sqrt(ax_in[0] * ax_in[0] + ax_in[1] * ax_in[1] + ax_in[2] * ax_in[2]);
+    const double a0 = ax_in[0] / n, a1 = ax_in[1] / n, a2 = ax_in[2] / n;
+    const double C[3][3] = {{0.0, -a2, a1}, {a2, 0.0, -a0}, {-a1, a0, 0.0}};
+    const double s = std::sin(angle), c1 = 1.0 - std::cos(angle);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double cc = 0.0;
+            for (int k = 0; k < 3; ++k) cc += C[i][k] * C[k][j];
+            R[i][j] = (i == j ? 1.0 : 0.0) + C[i][j] * s + cc * c1;
+        }
+}
+
+static void apply(const double R[3][3], double* v) {
+    const double r0 = R[0][0] * v[0] + R[0][1] * v[1] + R[0][2] * v[2];
+    const double r1 = R[1][0] * v[0] + R[1][1] * v[1] + R[1][2] * v[2];
+    const double r2 = R[2][0] * v[0] + R[2][1] * v[1] + R[2][2] * v[2];
+    v[0] = r0;
+    v[1] = r1;
+    v[2] = r2;
+}
+
+void orc_rotate_vector_spherical(double* z, double* y, double theta, double phi) {
+    double r_theta[3][3], r_phi[3][3];
+    rotation_about(y, theta, r_theta);
+    rotation_about(z, phi, r_phi);
+    apply(r_theta, z);
+    apply(r_phi, z);
+    apply(r_phi, y);
+}
+
 // CTEM: init_plane_wave (value 1) then the slice loop (simulationctem.cpp:77-82, 340-341)
 int orc_run_plane(int res, int n_slices, const double* trans, const double* P, double* psi_out,
                   double* recip_out) {

@@ -25,6 +25,10 @@
  *   orc_ctem_intensity .............. orc_run_plane over orc_transmission of the sorted atoms
  *   orc_ctem_loss_fd ................ brute-force central differences (every slice rebuilt,
  *                                     perturbed atoms moved in numpy), O(h^2) convergence
+ *   orc_translate_image ............. whole-pixel moves = numpy.roll; fractional moves = the
+ *                                     linear blend of the two neighbouring rolls; sum kept
+ *   orc_rotate_vector_spherical ..... MicroscopeParameters::Wavevector from (0, 0, k); lengths
+ *                                     and the y . z = 0 frame kept; scipy Rotation composition
  */
 #ifndef MSE_ORACLE_H
 #define MSE_ORACLE_H
@@ -122,6 +126,13 @@ int orc_slice_step(int res, double* psi, double* recip, const double* t, const d
 double orc_stem_pixel(int res, const double* recip, double inner_mrad, double outer_mrad,
                       double xc_mrad, double yc_mrad, double angle_scale, int sqrt2);
 int orc_diffraction(int res, const double* recip, double* out);
+/* translateDiffImage (simulationgeneral.cpp:832-854) + bilinear_translate_d
+ * (kernels/bilinear_translate_d.cl:21-72): an fft-shifted image moved by (dkx, dky) / inverse_scale
+ * pixels (plasmon tilt), with one wrap-around */
+int orc_translate_image(int res, const double* in, double* out, double dkx, double dky, double inverse_scale);
+/* Utils::rotateVectorSpherical (utilities/vectorutils.cpp:8-31): z rotated by theta about y, then
+ * z and y rotated by phi about the original z (Rodrigues matrices of the normalised axes) */
+void orc_rotate_vector_spherical(double* z, double* y, double theta, double phi);
 
 /* Whole-path drivers.  trans = n_slices * 2N (already band limited). */
 int orc_run_plane(int res, int n_slices, const double* trans, const double* P,

@@ -122,6 +122,9 @@ def lib():
         L.orc_stem_pixel.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_int]
         L.orc_diffraction.argtypes = [C.c_int, dp, dp]
+        L.orc_translate_image.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double]
+        L.orc_rotate_vector_spherical.argtypes = [dp, dp, C.c_double, C.c_double]
+        L.orc_rotate_vector_spherical.restype = None
         L.orc_run_plane.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
         L.orc_ctem_image.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp]
         L.orc_ctem_intensity.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, C.c_int,
@@ -333,6 +336,20 @@ def diffraction(recip):
     out = np.zeros((res, res))
     lib().orc_diffraction(res, _dp(recip.view(np.float64)), _dp(out))
     return out
+
+
+def translate_image(img, dkx, dky, inverse_scale):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.zeros_like(img)
+    lib().orc_translate_image(img.shape[0], _dp(img), _dp(out), dkx, dky, inverse_scale)
+    return out
+
+
+def rotate_vector_spherical(z, y, theta, phi):
+    z = _f64(z).copy()
+    y = _f64(y).copy()
+    lib().orc_rotate_vector_spherical(_dp(z), _dp(y), theta, phi)
+    return z, y
 
 
 def run_plane(trans, P):

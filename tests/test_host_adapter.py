@@ -60,6 +60,34 @@ def test_cpp_incoherence_draws_match_python():
 
 
 @pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cpp_plasmon_draws_match_python():
+    """The C++ host's PlasmonScattering draws (depths, then polar and azimuthal angles) equal
+    cltem_amd.incoherence's, for the combined and the individual-count modes."""
+    from cltem_amd import incoherence as inc
+    for job, ind in ((0, 0), (5, 0), (2, 3)):
+        spec = "8,1.2,15" + (f",{ind}" if ind else "")
+        r = subprocess.run([RUN, "--srtio3", "2,2,2", "--seed", "77", "--plasmon", spec, "--dump-plasmons", "250",
+                            "--dump-config", str(job)], check=True, capture_output=True, text=True)
+        vals = np.array(r.stdout.split(), dtype=np.float64)
+        n = int(vals[0])
+        ref = inc.plasmon_depths(77, job, 250.0, 80.0, individual=ind)
+        assert n == len(ref) and (ind == 0 or n == ind)
+        np.testing.assert_allclose(vals[1:n + 1], ref, rtol=1e-13)
+        polar, az = inc.plasmon_angles(77, job, n, 1.2, 15.0)
+        np.testing.assert_allclose(vals[n + 1::2], polar, rtol=1e-12)
+        np.testing.assert_allclose(vals[n + 2::2], az, rtol=1e-13)
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_cpp_mixed_seed_matches_python():
+    from cltem_amd import incoherence as inc
+    for job in (0, 1, 7):
+        r = subprocess.run([RUN, "--seed", "20261202", "--dump-mixed-seed", "--dump-config", str(job)], check=True,
+                           capture_output=True, text=True)
+        assert int(r.stdout.strip()) == inc.mixed_seed(20261202, job)
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
 def test_driver_rejects_bad_input(tmp_path):
     r = subprocess.run([RUN, "--mode", "ctem"], capture_output=True, text=True)
     assert r.returncode == 1 and "no structure" in r.stderr
