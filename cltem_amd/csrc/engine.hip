@@ -1207,10 +1207,13 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             e->mix_key.ensure(batch);
             e->n_waves = batch;
         }
-        std::vector<int> wc(batch, cfg);
+        // every wave (also those past the batch) refers to the configuration, so the host and
+        // device tables keep one entry per wave
+        std::vector<int> wc(e->n_waves, cfg);
         if (e->h_wave_cfg != wc) {
             e->h_wave_cfg = wc;
-            HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), wc.size() * sizeof(int), hipMemcpyHostToDevice,
+                                e->stream));
         }
         sync_wave_prop(e);
         ensure_detectors(e, dets, n_det);

@@ -249,7 +249,8 @@ int mse_diffraction_sum(mse_engine* e, const int* wave_group, int n_groups, int 
  *      for n_probes positions of one configuration, in batches of `batch` waves, with the
  *      detector reduction fused into the last slice.  accum[p*n_det + d] += intensity
  *      (FP64, so repeated calls over configurations accumulate like updateImages,
- *      simulationmanager.cpp:274-350).  use_graph=1 replays a captured hipGraph per batch. ---- */
+ *      simulationmanager.cpp:274-350).  use_graph=1 replays a captured hipGraph per batch.
+ *      Afterwards every wave refers to `config` and waves 0..batch-1 hold the last batch. ---- */
 int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes, int batch,
                    const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
                    int use_graph, double* accum);

@@ -272,3 +272,24 @@ def test_mixed_size_cbed_driver(oracle):
         err = rel_l2(patterns[i], ref)
         print(f"instance {i}: {og.resolution}^2, {og.n_slices} slices, alpha {it.alpha_mrad:.1f} mrad: rel-L2 {err:.3e}")
         assert err <= 1e-3
+
+
+def test_waves_past_a_stem_batch_stay_consistent(engine):
+    """mse_stem_batch with fewer waves than the engine holds: every wave then refers to the
+    batch's configuration, and per-wave calls on the waves past the batch stay valid."""
+    w = wl.stem_srtio3(nx=3, ny=3, nz=2, resolution=256, pixels=2, window_cells=(1, 2))
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+    engine.set_geometry(g)
+    engine.set_parameterisation(cltem_amd.load_params("kirkland"))
+    s = w["specimen"]
+    engine.sort_atoms(0, s.x, s.y, s.z, s.Z)
+    engine.build_transmission(0)
+    engine.set_wave_count(6)
+    engine.set_wave_configs([0] * 6)
+    engine.init_plane_wave(1.0)
+    got = engine.stem_batch(0, wl.stem_positions(g), 2, cltem_amd.make_probe(20.0),
+                            cltem_amd.make_detectors([(60.0, 200.0, 0.0, 0.0)]))
+    assert np.all(got > 0)
+    d = engine.diffraction(5)  # a plane wave untouched by the batches: one bright pixel
+    assert np.isclose(d.sum(), d.max(), rtol=1e-6) and d.max() > 0
+    engine.multislice(0, 1)
