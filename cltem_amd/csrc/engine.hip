@@ -234,9 +234,10 @@ void sync_wave_prop(mse_engine* e) {
     if (!any) return;
     e->h_wave_prop.assign(2 * (size_t)nw, nullptr);
     for (int w = 0; w < nw; ++w) {
-        const ConfigData& c = e->configs[e->h_wave_cfg[w]];
-        e->h_wave_prop[2 * w] = c.own_beam ? c.px.p : e->px.p;
-        e->h_wave_prop[2 * w + 1] = c.own_beam ? c.py.p : e->py.p;
+        const int ci = e->h_wave_cfg[w];
+        const bool own = ci >= 0 && ci < (int)e->configs.size() && e->configs[ci].own_beam;
+        e->h_wave_prop[2 * w] = own ? e->configs[ci].px.p : e->px.p;
+        e->h_wave_prop[2 * w + 1] = own ? e->configs[ci].py.p : e->py.p;
     }
     HIPX(hipStreamSynchronize(e->stream));  // kernels in flight may read the table
     e->wave_prop.ensure(2 * (size_t)nw);
