@@ -223,6 +223,22 @@ void propagator_tables(mse_engine* e, const double* kv, std::vector<float2>& hpx
     }
 }
 
+// Per-wave buffers for n waves: the waves, their configuration table, probe positions and
+// mixed-potential keys (identity keys when the key buffer is new).  Kernels that may read the
+// old buffers must have finished (callers synchronise first).
+void size_wave_buffers(mse_engine* e, int n) {
+    e->waves.ensure((size_t)n * e->N);
+    e->wave_cfg.ensure(n);
+    e->probe_pos.ensure(2 * (size_t)n);
+    if (e->mix_key.n < (size_t)n || !e->mix_key.p) {
+        e->mix_key.ensure(n);
+        std::vector<unsigned> keys(n);
+        for (int i = 0; i < n; ++i) keys[i] = (unsigned)i;
+        HIPX(hipMemcpyAsync(e->mix_key.p, keys.data(), n * sizeof(unsigned), hipMemcpyHostToDevice, e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    }
+}
+
 void sync_wave_prop(mse_engine* e) {
     const int nw = std::min<int>(e->n_waves, (int)e->h_wave_cfg.size());
     bool any = false;
@@ -676,11 +692,8 @@ int mse_set_wave_count(mse_engine* e, int n_waves) {
         require_geom(e);
         REQUIRE(n_waves >= 1 && n_waves <= 65535, MSE_ERR_ARG, "wave count out of range");
         HIPX(hipStreamSynchronize(e->stream));
-        e->waves.ensure((size_t)n_waves * e->N);
-        e->wave_cfg.ensure(n_waves);
-        e->probe_pos.ensure(2 * (size_t)n_waves);
-        e->mix_key.ensure(n_waves);
-        {
+        size_wave_buffers(e, n_waves);
+        {  // a new wave set starts with identity keys
             std::vector<unsigned> keys(n_waves);
             for (int i = 0; i < n_waves; ++i) keys[i] = (unsigned)i;
             HIPX(hipMemcpyAsync(e->mix_key.p, keys.data(), n_waves * sizeof(unsigned), hipMemcpyHostToDevice,
@@ -900,6 +913,7 @@ void upload_positions(mse_engine* e, const double* pos_xy, int n, int first_wave
     HIPX(hipMemcpyAsync(e->probe_pos.p + 2 * (size_t)first_wave, rel.data(), rel.size() * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));
     // keys of the mixed-potential draws: the probe indices
+    REQUIRE(e->mix_key.p && e->mix_key.n >= (size_t)first_wave + n, MSE_ERR_STATE, "wave buffers are not sized");
     std::vector<unsigned> keys(n);
     for (int i = 0; i < n; ++i) keys[i] = (unsigned)(first_probe + i);
     HIPX(hipMemcpyAsync(e->mix_key.p + first_wave, keys.data(), n * sizeof(unsigned), hipMemcpyHostToDevice,
@@ -1200,13 +1214,10 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         REQUIRE(!c.own_beam, MSE_ERR_UNSUPPORTED,
                 "a configuration with its own beam vector runs through mse_multislice and mse_stem_detect "
                 "(its detector images are translated)");
-        if (e->n_waves < batch || !e->waves.p) {
+        if (e->n_waves < batch || !e->waves.p || e->mix_key.n < (size_t)e->n_waves) {
             HIPX(hipStreamSynchronize(e->stream));
-            e->waves.ensure((size_t)batch * e->N);
-            e->wave_cfg.ensure(batch);
-            e->probe_pos.ensure(2 * (size_t)batch);
-            e->mix_key.ensure(batch);
-            e->n_waves = batch;
+            size_wave_buffers(e, std::max(batch, e->n_waves));
+            e->n_waves = std::max(batch, e->n_waves);
         }
         // every wave (also those past the batch) refers to the configuration, so the host and
         // device tables keep one entry per wave
@@ -1567,9 +1578,7 @@ int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, 
                 "transmission functions of this configuration are not built for the current geometry");
         if (e->n_waves < n_waves || !e->waves.p) {
             HIPX(hipStreamSynchronize(e->stream));
-            e->waves.ensure((size_t)n_waves * e->N);
-            e->wave_cfg.ensure(n_waves);
-            e->probe_pos.ensure(2 * (size_t)n_waves);
+            size_wave_buffers(e, n_waves);
             e->n_waves = n_waves;
         }
         e->h_wave_cfg.assign(e->n_waves, cfg);
@@ -1668,9 +1677,7 @@ int mse_bandwidth_probe(mse_engine* e, int pattern, int variant, int n_waves, in
         REQUIRE(variant >= 0 && variant < kColVariants, MSE_ERR_ARG, "variant out of range");
         if (e->n_waves < n_waves || !e->waves.p) {
             HIPX(hipStreamSynchronize(e->stream));
-            e->waves.ensure((size_t)n_waves * e->N);
-            e->wave_cfg.ensure(n_waves);
-            e->probe_pos.ensure(2 * (size_t)n_waves);
+            size_wave_buffers(e, n_waves);
             e->h_wave_cfg.assign(n_waves, 0);
             HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
                                 e->stream));

@@ -219,11 +219,19 @@ struct BlockSync {
     __device__ __forceinline__ void operator()() const { __syncthreads(); }
 };
 
-// padded LDS index: one extra complex every 16 entries
+// padded LDS index: one extra complex every 16 entries.  The LDS serves 16 lanes of an
+// 8-byte access per cycle over 16 bank pairs, so 16 lanes at consecutive indices, or at
+// indices 8 apart (the first pass's radix-8 writes), hit 16 different pairs.
 __device__ __forceinline__ int lpad(int q) { return q + (q >> 4); }
+// Index of the second pass's exchange: lpad plus 4 more per block of 64 (monotone, so
+// one-to-one).  With a first radix of 8 the second pass's 16-lane groups write two runs of 8
+// entries 64 apart, which lpad puts 4 pairs apart (a 2-way conflict); the extra 4 moves the
+// second run onto the other 8 pairs.  Consecutive reads stay conflict-free (the shift is the
+// same within an aligned block of 16).
+__device__ __forceinline__ int lpad2(int q) { return q + (q >> 4) + ((q >> 6) << 2); }
 template <int N>
 struct LdsLine {
-    static constexpr int size = N + N / 16;  // lpad(N-1) + 1 <= size
+    static constexpr int size = N + N / 8;  // lpad2(N-1) + 1 <= size (and lpad(N-1) + 1)
     // column tiles use a stride that is 1 mod 16 complex so 16 columns hit 16 bank pairs
     static constexpr int col_stride = size + ((17 - size % 16) % 16);
 };
@@ -275,7 +283,7 @@ struct PlanTw {
 
 // One Stockham pass.  v holds the line in register layout; line is this line's LDS.
 // P_ = product of radices already done (p); LAST = this pass ends the transform.
-template <int N, int L, int R, int P_, bool INV, bool LAST, class TW>
+template <int N, int L, int R, int P_, bool INV, bool LAST, class TW, bool PAD2 = false>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[N / L], float2* line, int t,
                                               const TW* tw) {
     constexpr int E = N / L;
@@ -301,16 +309,16 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[N / L], float2* line, 
         } else {
             const int j = (i - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[lpad(j + r * P_)] = a[r];
+            for (int r = 0; r < R; ++r) line[PAD2 ? lpad2(j + r * P_) : lpad(j + r * P_)] = a[r];
         }
     }
 }
 
-template <int N, int L>
+template <int N, int L, bool PAD2 = false>
 __device__ __forceinline__ void line_gather(float2 (&v)[N / L], const float2* line, int t) {
     constexpr int E = N / L;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = line[lpad(t + L * m)];
+    for (int m = 0; m < E; ++m) v[m] = line[PAD2 ? lpad2(t + L * m) : lpad(t + L * m)];
 }
 
 struct NoTw {
@@ -327,9 +335,9 @@ __device__ __forceinline__ void fft_line(float2 (&v)[PL::E], float2* line, int t
     sync();
     line_gather<N, L>(v, line, t);
     sync();
-    stockham_pass<N, L, PL::R1, PL::R0, INV, false>(v, line, t, &tw.p1);
+    stockham_pass<N, L, PL::R1, PL::R0, INV, false, typename TWS::T1, true>(v, line, t, &tw.p1);
     sync();
-    line_gather<N, L>(v, line, t);
+    line_gather<N, L, true>(v, line, t);
     sync();
     stockham_pass<N, L, PL::R2, PL::R0 * PL::R1, INV, true>(v, line, t, &tw.p2);
 }

@@ -293,3 +293,26 @@ def test_waves_past_a_stem_batch_stay_consistent(engine):
     d = engine.diffraction(5)  # a plane wave untouched by the batches: one bright pixel
     assert np.isclose(d.sum(), d.max(), rtol=1e-6) and d.max() > 0
     engine.multislice(0, 1)
+
+
+def test_stem_batch_on_a_fresh_engine_after_autotune(oracle):
+    """The benchmark's order on a new engine: autotune (which sizes the waves itself), then
+    the batched STEM path, whose probe set-ups also write the per-wave draw keys."""
+    w = wl.stem_srtio3(nx=3, ny=3, nz=2, resolution=256, pixels=2, window_cells=(1, 2))
+    with cltem_amd.Engine(0) as eng:
+        g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**w["scenario"]))
+        eng.set_geometry(g)
+        p = cltem_amd.load_params("kirkland")
+        eng.set_parameterisation(p)
+        s = w["specimen"]
+        eng.sort_atoms(0, s.x, s.y, s.z, s.Z)
+        eng.build_transmission(0)
+        eng.autotune(0, 4, reps=1)
+        pos = wl.stem_positions(g)
+        got = eng.stem_batch(0, pos, 2, cltem_amd.make_probe(20.0),
+                             cltem_amd.make_detectors([(60.0, 200.0, 0.0, 0.0)]), use_graph=True)
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
+    ref = oracle.run_stem(og, oracle.transmission(og, srt, p.values, p.form, p.i_per_atom), oracle.propagator(og),
+                          pos, np.array([[60.0, 200.0, 0.0, 0.0]]), cond_ap_mrad=20.0)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-3
