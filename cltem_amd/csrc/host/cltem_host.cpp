@@ -642,30 +642,272 @@ void save_tiff(const std::string& path, const std::vector<double>& data, int wid
     if (!f) throw std::runtime_error("Unable to write data to .tif file");
 }
 
-static std::string settings_json(const SimulationManager& sm, const std::string& image) {
-    const mse_scenario& s = sm.scenario;
-    std::ostringstream js;
-    js.precision(17);
-    const char* modes[] = {"None", "CTEM", "STEM", "CBED"};
-    js << "{\n  \"image\": \"" << image << "\",\n  \"mode\": \"" << modes[std::min(std::max(s.mode, 0), 3)]
-       << "\",\n  \"resolution\": " << s.resolution << ",\n  \"slice thickness\": " << s.slice_thickness
-       << ",\n  \"slice offset\": " << s.slice_offset << ",\n  \"microscope\": {\"voltage\": " << s.voltage_kv
-       << ", \"beam tilt\": " << s.beam_tilt_mrad << ", \"beam azimuth\": " << s.beam_azimuth
-       << ", \"condenser aperture\": " << sm.probe.cond_ap_mrad << "},\n  \"full 3d\": {\"state\": "
-       << (s.full3d ? "true" : "false") << ", \"integrals\": " << s.full3d_integrals
-       << "},\n  \"potentials\": \"" << sm.params_file << "\",\n  \"frozen phonon\": {\"iterations\": "
-       << sm.phonon_configs << ", \"u_rms\": " << sm.u_rms << ", \"seed\": " << sm.seed
-       << "},\n  \"incoherence\": {\"chromatic\": {\"state\": " << (sm.chromatic.enabled ? "true" : "false")
-       << ", \"Cc\": " << sm.chromatic.cc_mm << ", \"sigma negative\": " << sm.chromatic.sigma_neg
-       << ", \"sigma positive\": " << sm.chromatic.sigma_pos << "}, \"source size\": {\"state\": "
-       << (sm.source.enabled ? "true" : "false") << ", \"sigma\": " << sm.source.sigma << "}}";
-    for (const auto& d : sm.detectors)
-        if (d.name == image)
-            js << ",\n  \"stem\": {\"detectors\": {\"" << d.name << "\": {\"inner\": " << d.d.inner_mrad
-               << ", \"outer\": " << d.d.outer_mrad << ", \"centre x\": " << d.d.xc_mrad << ", \"centre y\": "
-               << d.d.yc_mrad << "}}}";
-    js << ",\n  \"jobs\": " << sm.complete_jobs << "\n}\n";
-    return js.str();
+namespace {
+// A JSON value with the object ordering of nlohmann::json's default type (keys sorted) and
+// its setw(4) text, enough for the settings files.
+struct Json {
+    enum Kind { Null, Bool, Int, Num, Str, Obj } kind = Null;
+    bool b = false;
+    long long i = 0;
+    double d = 0.0;
+    std::string s;
+    std::map<std::string, Json> o;
+    Json& operator[](const std::string& k) {
+        kind = Obj;
+        return o[k];
+    }
+    Json& operator=(bool v) { kind = Bool; b = v; return *this; }
+    Json& operator=(int v) { kind = Int; i = v; return *this; }
+    Json& operator=(unsigned v) { kind = Int; i = v; return *this; }
+    Json& operator=(long long v) { kind = Int; i = v; return *this; }
+    Json& operator=(double v) { kind = Num; d = v; return *this; }
+    Json& operator=(const char* v) { kind = Str; s = v; return *this; }
+    Json& operator=(const std::string& v) { kind = Str; s = v; return *this; }
+    void erase(const std::string& k) { o.erase(k); }
+
+    static std::string number(double v) {
+        if (!std::isfinite(v)) return "null";
+        char buf[40];
+        for (int p = 1; p <= 17; ++p) {  // shortest text that reads back as the same double
+            std::snprintf(buf, sizeof(buf), "%.*g", p, v);
+            if (std::strtod(buf, nullptr) == v) break;
+        }
+        std::string t(buf);
+        if (t.find_first_of(".e") == std::string::npos) t += ".0";
+        return t;
+    }
+    static std::string quoted(const std::string& t) {
+        std::string r = "\"";
+        for (unsigned char c : t) {
+            if (c == '"' || c == '\\') {
+                r += '\\';
+                r += (char)c;
+            } else if (c < 0x20) {
+                char u[8];
+                std::snprintf(u, sizeof(u), "\\u%04x", c);
+                r += u;
+            } else {
+                r += (char)c;
+            }
+        }
+        return r + "\"";
+    }
+    void dump(std::ostream& os, int depth) const {
+        switch (kind) {
+            case Null: os << "null"; break;
+            case Bool: os << (b ? "true" : "false"); break;
+            case Int: os << i; break;
+            case Num: os << number(d); break;
+            case Str: os << quoted(s); break;
+            case Obj: {
+                if (o.empty()) {
+                    os << "{}";
+                    break;
+                }
+                os << "{\n";
+                size_t n = 0;
+                for (const auto& kv : o) {
+                    os << std::string(4 * (depth + 1), ' ') << quoted(kv.first) << ": ";
+                    kv.second.dump(os, depth + 1);
+                    os << (++n < o.size() ? ",\n" : "\n");
+                }
+                os << std::string(4 * depth, ' ') << "}";
+                break;
+            }
+        }
+    }
+};
+
+const double kDeg = 180.0 / M_PI;
+
+// (magnitude, angle in degrees) of a complex aberration held as (re, im)
+void polar_aberration(Json& j, const double* c, double scale, const char* units) {
+    j["mag"] = std::hypot(c[0], c[1]) / scale;
+    j["ang"] = std::atan2(c[1], c[0]) * kDeg;
+    j["units"] = units;
+}
+
+std::string base_name(const std::string& path) {
+    std::string b = path.substr(path.find_last_of("/\\") + 1);
+    const size_t dot = b.find_last_of('.');
+    return dot == std::string::npos ? b : b.substr(0, dot);
+}
+}  // namespace
+
+std::string settings_json(const SimulationManager& sm, const std::string& image) {
+    const mse_scenario& sc = sm.scenario;
+    const mse_geometry g = sm.geometry();
+    const int mode = sc.mode;
+    const bool probe_mode = mode == 2 || mode == 3;
+    static const char* kModes[] = {"None", "CTEM", "STEM", "CBED"};
+    Json j;
+    j["double precision"] = false;
+    j["mode"]["id"] = mode;
+    j["mode"]["name"] = kModes[std::min(std::max(mode, 0), 3)];
+    j["potentials"] = base_name(sm.params_file);
+    j["resolution"] = sc.resolution;
+    j["slice thickness"]["val"] = sc.slice_thickness;
+    j["slice thickness"]["units"] = "Å";
+    j["slice offset"]["val"] = sc.slice_offset;
+    j["slice offset"]["units"] = "Å";
+    j["slice count"] = g.n_slices;
+    j["intermediate output"]["slice interval"] = 0;  // only the exit surface is returned
+    j["intermediate output"]["enabled"] = false;
+    j["maintain areas"] = false;
+    const char* axes[] = {"x", "y", "z"};
+    const double starts[] = {g.sim_x[0], g.sim_y[0], g.z_lims[0]}, ends[] = {g.sim_x[1], g.sim_y[1], g.z_lims[1]};
+    const double pads[] = {sc.pad_xy[1], sc.pad_xy[1], g.pad_z[1]};
+    for (int k = 0; k < 3; ++k) {
+        Json& ax = j["simulation area"][axes[k]];
+        ax["start"] = starts[k];
+        ax["finish"] = ends[k];
+        ax["padding"] = pads[k];
+        ax["units"] = "Å";
+    }
+    j["default padding"]["z"]["val"] = sc.pad_z[1];
+    j["default padding"]["z"]["units"] = "Å";
+    j["default padding"]["xy"]["val"] = sc.pad_xy[1];
+    j["default padding"]["xy"]["units"] = "Å";
+    j["full 3d"]["state"] = sc.full3d != 0;
+    if (sc.full3d) j["full 3d"]["integrals"] = sc.full3d_integrals;
+    j["precalculate transmission"] = !sm.plasmon.enabled;
+
+    Json& mic = j["microscope"];
+    mic["voltage"]["val"] = sc.voltage_kv;
+    mic["voltage"]["units"] = "kV";
+    if (mode == 1) {  // the objective lens of CTEM image formation (not run by this driver)
+        mic["objective aperture"]["semi-angle"] = 20.0;
+        mic["objective aperture"]["smoothing"] = 0.0;
+        mic["objective aperture"]["units"] = "mrad";
+        mic["alpha"]["val"] = 0.0;
+        mic["alpha"]["units"] = "mrad";
+        mic["delta"]["val"] = 0.0;
+        mic["delta"]["units"] = "nm";
+    } else {
+        mic["condenser aperture"]["semi-angle"] = sm.probe.cond_ap_mrad;
+        mic["condenser aperture"]["smoothing"] = sm.probe.smooth_mrad;
+        mic["condenser aperture"]["units"] = "mrad";
+    }
+    mic["beam tilt"]["inclination"]["val"] = sc.beam_tilt_mrad;
+    mic["beam tilt"]["inclination"]["units"] = "mrad";
+    mic["beam tilt"]["azimuth"]["val"] = sc.beam_azimuth * kDeg;
+    mic["beam tilt"]["azimuth"]["units"] = "°";
+    const mse_probe& p = sm.probe;
+    Json& ab = mic["aberrations"];
+    ab["C10"]["val"] = p.C10 / 10.0;
+    ab["C10"]["units"] = "nm";
+    polar_aberration(ab["C12"], p.C12, 10.0, "nm, °");
+    polar_aberration(ab["C21"], p.C21, 10.0, "nm, °");
+    polar_aberration(ab["C23"], p.C23, 10.0, "nm, °");
+    ab["C30"]["val"] = p.C30 / 10000.0;
+    ab["C30"]["units"] = "μm";
+    polar_aberration(ab["C32"], p.C32, 10000.0, "μm, °");
+    polar_aberration(ab["C34"], p.C34, 10000.0, "μm, °");
+    polar_aberration(ab["C41"], p.C41, 10000.0, "μm, °");
+    polar_aberration(ab["C43"], p.C43, 10000.0, "μm, °");
+    polar_aberration(ab["C45"], p.C45, 10000.0, "μm, °");
+    ab["C50"]["val"] = p.C50 / 10000.0;
+    ab["C50"]["units"] = "μm";
+    polar_aberration(ab["C52"], p.C52, 10000.0, "μm, °");
+    polar_aberration(ab["C54"], p.C54, 10000.0, "μm, °");
+    polar_aberration(ab["C56"], p.C56, 10000.0, "μm, °");
+
+    if (mode == 1) {
+        j["ctem"]["simulate image"] = false;
+        j["ctem"]["area"]["x"]["start"] = sc.area[0];
+        j["ctem"]["area"]["x"]["finish"] = sc.area[1];
+        j["ctem"]["area"]["y"]["start"] = sc.area[2];
+        j["ctem"]["area"]["y"]["finish"] = sc.area[3];
+    }
+    if (mode == 2) {
+        Json& st = j["stem"];
+        st["area"]["x"]["start"] = sc.area[0];
+        st["area"]["x"]["finish"] = sc.area[1];
+        st["area"]["x"]["units"] = "Å";
+        st["area"]["y"]["start"] = sc.area[2];
+        st["area"]["y"]["finish"] = sc.area[3];
+        st["area"]["y"]["units"] = "Å";
+        st["scan"]["x"]["pixels"] = sc.stem_pixels[0];
+        st["scan"]["y"]["pixels"] = sc.stem_pixels[1];
+        st["area"]["padding"]["val"] = sc.area_padding;
+        st["area"]["padding"]["units"] = "Å";
+        st["static area"]["concurrent pixels"] = sm.parallel_pixels;
+        st["static area"]["enabled"] = true;
+    }
+    if (mode == 3) {
+        j["cbed"]["position"]["x"] = sc.area[0];
+        j["cbed"]["position"]["y"] = sc.area[2];
+        j["cbed"]["position"]["padding"] = sc.area_padding;
+        j["cbed"]["position"]["units"] = "Å";
+    }
+
+    // incoherent effects (jsonutils.cpp:657-778)
+    const bool phonons = sm.phonon_configs > 1 && sm.u_rms > 0.0;
+    const bool probe_effects = probe_mode && (sm.chromatic.enabled || sm.source.enabled);
+    if (phonons || probe_effects || sm.plasmon.enabled)
+        j["incoherence"]["iterations"] = std::max(1, sm.phonon_configs);
+    if (probe_mode && (phonons || probe_effects || sm.plasmon.enabled)) {
+        Json& pr = j["incoherence"]["probe"];
+        const double fwhm = 2.0 * std::sqrt(2.0 * std::log(2.0));
+        pr["chromatic"]["enabled"] = sm.chromatic.enabled;
+        if (sm.chromatic.enabled) {
+            pr["chromatic"]["Cc"]["val"] = sm.chromatic.cc_mm;
+            pr["chromatic"]["Cc"]["units"] = "mm";
+            pr["chromatic"]["dE"]["HWHM +"] = sm.chromatic.sigma_pos * fwhm / 2.0;
+            pr["chromatic"]["dE"]["HWHM -"] = sm.chromatic.sigma_neg * fwhm / 2.0;
+            pr["chromatic"]["dE"]["units"] = "eV";
+        }
+        pr["source size"]["enabled"] = sm.source.enabled;
+        if (sm.source.enabled) {
+            pr["source size"]["FWHM"]["val"] = sm.source.sigma * fwhm;
+            pr["source size"]["FWHM"]["units"] = "Å";
+        }
+    }
+    if (phonons) {
+        Json& ph = j["incoherence"]["inelastic scattering"]["phonon"];
+        if (sm.mixedSets() >= 2) {
+            ph["mixed static potentials"]["count"] = sm.mixedSets();
+            ph["mixed static potentials"]["enabled"] = true;
+        }
+        ph["force atom resort"] = false;
+    }
+    if (sm.plasmon.enabled) {
+        Json& pl = j["incoherence"]["inelastic scattering"]["plasmon"];
+        pl["enabled"] = true;
+        pl["type"] = sm.plasmon.individual > 0 ? "individual" : "full";
+        if (sm.plasmon.individual > 0) pl["individual"] = sm.plasmon.individual;
+        pl["mean free path"]["value"] = sm.plasmon.mean_free_path / 10.0;
+        pl["mean free path"]["units"] = "nm";
+        pl["characteristic angle"]["value"] = sm.plasmon.characteristic_angle;
+        pl["characteristic angle"]["units"] = "mrad";
+        pl["critical angle"]["value"] = sm.plasmon.critical_angle;
+        pl["critical angle"]["units"] = "mrad";
+    }
+
+    // imageReturned (console/main.cpp:189-228)
+    j["filename"] = sm.structure_file;
+    if (image == "EW" || image == "Diff") {
+        mic.erase("aberrations");
+        mic.erase("alpha");
+        mic.erase("delta");
+        mic.erase("objective aperture");
+    } else if (image != "Image") {
+        for (const auto& d : sm.detectors)
+            if (d.name == image) {
+                Json& dj = j["stem"]["detectors"][d.name];
+                dj["radius"]["inner"] = d.d.inner_mrad;
+                dj["radius"]["outer"] = d.d.outer_mrad;
+                dj["radius"]["units"] = "mrad";
+                dj["centre"]["x"] = d.d.xc_mrad;
+                dj["centre"]["y"] = d.d.yc_mrad;
+                dj["centre"]["units"] = "mrad";
+            }
+        mic.erase("alpha");
+        mic.erase("delta");
+    }
+    std::ostringstream os;
+    j.dump(os, 0);
+    return os.str();
 }
 
 void save_outputs(const SimulationManager& sm, const std::string& prefix) {

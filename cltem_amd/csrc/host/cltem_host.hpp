@@ -125,6 +125,7 @@ void rotate_vector_spherical(double z[3], double y[3], double theta, double phi)
 struct SimulationManager {
     mse_scenario scenario{};   // mode, resolution, voltage, slicing, padding, areas, tilt
     std::string params_file;   // params/*.dat
+    std::string structure_file;  // CrystalStructure::fileName (the settings' "filename")
     mse_probe probe{};         // STEM/CBED condenser and aberrations
     std::vector<Detector> detectors;
     AtomList atoms;            // undisplaced structure
@@ -228,6 +229,11 @@ class SimulationStem : public SimulationCbed {
 // fileio::SaveTiff<float> (utilities/fileio.h:44-82): one uncompressed strip of 32-bit IEEE
 // floats, one sample per pixel, min-is-black, orientation bottom-left, rows in data order.
 void save_tiff(const std::string& path, const std::vector<double>& data, int width, int height);
+
+// JSONUtils::BasicManagerToJson(man, false, true) (utilities/jsonutils.cpp:432-781) with the
+// structure file name and the per-image edits of imageReturned (console/main.cpp:189-228),
+// written as SaveSettingsJson does (keys sorted, 4-space indents).
+std::string settings_json(const SimulationManager& sm, const std::string& image);
 
 // The outputs of imageReturned (console/main.cpp:189-266): "EW" as <name>_amplitude and
 // <name>_phase, every other image as <name>; each a .tif (one per slice of a stack, suffixed

@@ -14,6 +14,8 @@
 //           0..N-1 and the source offset of job C, one number per line, and exits)
 //   mse_run ... --dump-plasmons THICKNESS [--dump-config C]   (prints the scattering count,
 //           depths, then polar (mrad) and azimuth of each scattering of job C, and exits)
+//   mse_run ... --dump-settings IMAGE   (prints the settings .json written beside image IMAGE
+//           -- EW, Diff or a detector name -- and exits without using the GPU)
 //   mse_run ... --dump-mixed-seed [--dump-config C]   (prints the seed of job C's mixed
 //           potential draws and exits)
 //   mse_run --write-test-tiff FILE W H   (writes the ramp x + 1000 y, W x H, and exits)
@@ -62,6 +64,7 @@ int main(int argc, char** argv) {
         int dump_incoherence = -1;
         double dump_plasmons = -1.0;
         bool dump_mixed = false;
+        std::string dump_settings;
         bool tiff = false;
         std::string cif_path;
         double zone[3] = {0, 0, 1}, normal[3] = {0, 0, 0}, widths[3] = {0, 0, 0}, tilts[3] = {0, 0, 0};
@@ -79,7 +82,8 @@ int main(int argc, char** argv) {
             } else if (a == "--params") {
                 sm->params_file = next();
             } else if (a == "--xyz") {
-                sm->atoms = read_xyz(next());
+                sm->structure_file = next();
+                sm->atoms = read_xyz(sm->structure_file);
             } else if (a == "--srtio3") {
                 auto v = split_doubles(next());
                 sm->atoms = srtio3((int)v.at(0), (int)v.at(1), (int)v.at(2));
@@ -134,6 +138,8 @@ int main(int argc, char** argv) {
                 sm->plasmon.individual = v.size() > 3 ? (int)v[3] : 0;
             } else if (a == "--parallel-potentials") {
                 sm->parallel_potentials = std::stoi(next());
+            } else if (a == "--dump-settings") {
+                dump_settings = next();
             } else if (a == "--dump-mixed-seed") {
                 dump_mixed = true;
             } else if (a == "--dump-plasmons") {
@@ -157,6 +163,7 @@ int main(int argc, char** argv) {
                 return 0;
             } else if (a == "--cif") {
                 cif_path = next();
+                sm->structure_file = cif_path;
             } else if (a == "--zone" || a == "--normal" || a == "--widths" || a == "--tilts") {
                 auto v = split_doubles(next());
                 double* dst = a == "--zone" ? zone : a == "--normal" ? normal : a == "--widths" ? widths : tilts;
@@ -225,6 +232,10 @@ int main(int argc, char** argv) {
         if (s.mode == 2) {
             s.stem_pixels[0] = s.stem_pixels[1] = stem_px;
             if (sm->detectors.empty()) sm->detectors.push_back({"ADF", mse_detector{60.0, 200.0, 0.0, 0.0}});
+        }
+        if (!dump_settings.empty()) {
+            std::cout << settings_json(*sm, dump_settings) << std::endl;
+            return 0;
         }
         run_simulation(sm);
         std::ofstream js(out + ".json");

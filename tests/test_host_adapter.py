@@ -87,6 +87,60 @@ def test_cpp_mixed_seed_matches_python():
         assert int(r.stdout.strip()) == inc.mixed_seed(20261202, job)
 
 
+def _settings(*args):
+    r = subprocess.run([RUN, "--params", os.path.join(PARAMS, "kirkland.dat"), *args], check=True,
+                       capture_output=True, text=True)
+    return json.loads(r.stdout)
+
+
+def _at(j, *path):
+    for k in path:
+        j = j[k]
+    return j
+
+
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_settings_json_follows_cltem_schema():
+    """The settings written beside each image carry the fields clTEM's JsonToManager reads
+    (utilities/jsonutils.cpp:15-360) at the same paths, with clTEM's units."""
+    st = _settings("--mode", "stem", "--srtio3", "3,3,2", "--res", "256", "--dz", "1.9525", "--kv", "300",
+                   "--det", "ADF,60,200", "--stem-pixels", "8", "--batch", "16", "--phonons", "3", "--urms", "0.08",
+                   "--parallel-potentials", "2", "--chromatic", "1.2,0.5,0.8", "--source-size", "0.7",
+                   "--tilt", "2.5,0.3", "--dump-settings", "ADF")
+    for path, want in [(("mode", "id"), 2), (("mode", "name"), "STEM"), (("resolution",), 256),
+                       (("slice thickness", "val"), 1.9525), (("slice offset", "val"), 0.0),
+                       (("microscope", "voltage", "val"), 300.0),
+                       (("microscope", "condenser aperture", "semi-angle"), 20.0),
+                       (("microscope", "beam tilt", "inclination", "val"), 2.5),
+                       (("stem", "scan", "x", "pixels"), 8), (("stem", "static area", "concurrent pixels"), 16),
+                       (("stem", "detectors", "ADF", "radius", "inner"), 60.0),
+                       (("stem", "detectors", "ADF", "radius", "outer"), 200.0),
+                       (("incoherence", "iterations"), 3), (("incoherence", "probe", "chromatic", "enabled"), True),
+                       (("incoherence", "probe", "chromatic", "Cc", "val"), 1.2),
+                       (("incoherence", "inelastic scattering", "phonon", "mixed static potentials", "count"), 2),
+                       (("default padding", "xy", "val"), 8.0), (("full 3d", "state"), False),
+                       (("precalculate transmission",), True), (("intermediate output", "enabled"), False)]:
+        assert _at(st, *path) == pytest.approx(want) if isinstance(want, float) else _at(st, *path) == want, path
+    assert _at(st, "microscope", "beam tilt", "azimuth", "val") == pytest.approx(0.3 * 180 / np.pi)
+    assert _at(st, "incoherence", "probe", "chromatic", "dE", "HWHM -") == pytest.approx(0.5)
+    assert _at(st, "incoherence", "probe", "source size", "FWHM", "val") == pytest.approx(0.7)
+    assert _at(st, "slice count") > 0 and "aberrations" in st["microscope"]
+    for k in ("C10", "C30", "C50"):
+        assert "val" in st["microscope"]["aberrations"][k]
+    for k in ("C12", "C21", "C23", "C32", "C34", "C41", "C43", "C45", "C52", "C54", "C56"):
+        assert set(st["microscope"]["aberrations"][k]) == {"mag", "ang", "units"}
+    # CBED diffraction: imageReturned drops the objective-lens entries and the aberrations
+    cb = _settings("--mode", "cbed", "--srtio3", "2,2,2", "--res", "256", "--area", "3,3,4,4",
+                   "--plasmon", "50,1.5,12,2", "--dump-settings", "Diff")
+    assert cb["cbed"]["position"]["x"] == 3.0 and cb["cbed"]["position"]["y"] == 4.0
+    assert "aberrations" not in cb["microscope"] and "alpha" not in cb["microscope"]
+    pl = cb["incoherence"]["inelastic scattering"]["plasmon"]
+    assert pl["type"] == "individual" and pl["individual"] == 2 and pl["mean free path"]["value"] == 50.0
+    assert cb["precalculate transmission"] is False
+    ct = _settings("--mode", "ctem", "--srtio3", "2,2,2", "--res", "256", "--dump-settings", "EW")
+    assert ct["mode"]["name"] == "CTEM" and "ctem" in ct and "condenser aperture" not in ct["microscope"]
+
+
 @pytest.mark.skipif(not have_driver(), reason="mse_run not built")
 def test_driver_rejects_bad_input(tmp_path):
     r = subprocess.run([RUN, "--mode", "ctem"], capture_output=True, text=True)
@@ -281,7 +335,8 @@ def test_driver_tiff_outputs(tmp_path):
         tags, data = read_tiff(f"{prefix}_{name}.tif")
         np.testing.assert_allclose(data, want.astype(np.float32), rtol=1e-6, atol=1e-6 * np.abs(want).max())
         settings = json.load(open(f"{prefix}_{name}.json"))
-        assert settings["mode"] == "CTEM" and settings["resolution"] == 256
+        assert settings["mode"]["name"] == "CTEM" and settings["resolution"] == 256
+        assert "aberrations" not in settings["microscope"]  # imageReturned drops them for EW and Diff
 
 
 def _pm3m_ops():
