@@ -221,9 +221,9 @@ int SimulationManager::mixedSets() const {
     return parallel_potentials;
 }
 
-uint64_t SimulationManager::mixedSeed(int job) const {
+uint64_t SimulationManager::mixedSeed(int job, int batch) const {
     const uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
-    const uint32_t ctr[4] = {(uint32_t)job, 0x3Bu, 0u, 7u};
+    const uint32_t ctr[4] = {(uint32_t)job, 0x3Bu, (uint32_t)batch, 7u};
     uint32_t r[4];
     philox4x32_10(ctr, key, r);
     return (uint64_t)r[0] | ((uint64_t)r[1] << 32);
@@ -389,29 +389,34 @@ void SimulationGeneral::resetBeam() {
     beam_moved = false;
 }
 
-int SimulationGeneral::runSlices(int job) {
+int SimulationGeneral::runSlices(int job, const std::function<void(int)>& output) {
     const int n = geom.n_slices;
-    if (!sm->plasmon.enabled) {
-        check(mse_multislice(eng, 0, n));
-        return 0;
-    }
-    // the structure's own thickness (console/main.cpp:711-717)
-    double zmin = 0.0, zmax = 0.0;
-    if (sm->atoms.size() > 0) {
-        zmin = *std::min_element(sm->atoms.z.begin(), sm->atoms.z.end());
-        zmax = *std::max_element(sm->atoms.z.begin(), sm->atoms.z.end());
-    }
-    const std::vector<double> depths = sm->plasmon.depths(sm->seed, job, zmax - zmin);
+    const int step = sm->slice_step;
+    const int n_out = sm->outputCount(n);
+    const bool plasmons = sm->plasmon.enabled;
+    std::vector<double> depths;
     // beam vector and the azimuth axis at the top (simulationcbed.cpp:152-162)
     double k[3] = {0.0, 0.0, geom.wavenumber}, yax[3] = {0.0, 1.0, 0.0};
-    rotate_vector_spherical(k, yax, geom.beam_tilt_mrad / 1000.0, geom.beam_azimuth);
+    if (plasmons) {
+        // the structure's own thickness (console/main.cpp:711-717)
+        double zmin = 0.0, zmax = 0.0;
+        if (sm->atoms.size() > 0) {
+            zmin = *std::min_element(sm->atoms.z.begin(), sm->atoms.z.end());
+            zmax = *std::max_element(sm->atoms.z.begin(), sm->atoms.z.end());
+        }
+        depths = sm->plasmon.depths(sm->seed, job, zmax - zmin);
+        rotate_vector_spherical(k, yax, geom.beam_tilt_mrad / 1000.0, geom.beam_azimuth);
+    }
     size_t next = 0;
-    int first = 0, events = 0;
+    int first = 0, events = 0, outs = 0;
     for (int i = 0; i < n; ++i) {
         const double depth = (i + 1 - geom.pre_slices) * geom.slice_thickness;
-        if (next < depths.size() && depth >= depths[next]) {
-            check(mse_multislice(eng, first, i + 1 - first));
-            first = i + 1;
+        const bool scatter = plasmons && next < depths.size() && depth >= depths[next];
+        const bool out = step > 0 && (i + 1) % step == 0;
+        if (!scatter && !out) continue;
+        check(mse_multislice(eng, first, i + 1 - first));
+        first = i + 1;
+        if (scatter) {
             double polar = 0.0, azimuth = 0.0;
             sm->plasmon.angles(sm->seed, job, (int)next, &polar, &azimuth);
             rotate_vector_spherical(k, yax, polar / 1000.0, azimuth);
@@ -421,8 +426,10 @@ int SimulationGeneral::runSlices(int job) {
             ++next;
             ++events;
         }
+        if (out) output(outs++);
     }
     if (first < n) check(mse_multislice(eng, first, n - first));
+    if (outs < n_out) output(outs);
     return events;
 }
 
@@ -465,14 +472,17 @@ bool SimulationCtem::initialiseSimulation() {
 
 void SimulationCtem::simulate() {
     initialiseSimulation();
-    runSlices(config);
     const int res = geom.resolution;
-    std::map<std::string, Image> ims;
-    Image ew(res, res, 1), diff(res, res, 1);
-    ew.slices[0] = getExitWaveImage();  // interleaved re/im (complex averaged, as clTEM)
+    const int n_out = sm->outputCount(geom.n_slices);
+    Image ew(res, res, n_out), diff(res, res, n_out);
+    for (auto& sl : ew.slices) sl.resize(2 * (size_t)res * res);  // interleaved re/im (averaged as complex)
+    runSlices(config, [&](int k) {
+        ew.slices[k] = getExitWaveImage();
+        diff.slices[k] = getDiffractionImage(0);
+    });
     std::fill(ew.weighting.begin(), ew.weighting.end(), 1.0);
-    diff.slices[0] = getDiffractionImage(0);
     std::fill(diff.weighting.begin(), diff.weighting.end(), 1.0);
+    std::map<std::string, Image> ims;
     ims["EW"] = ew;
     ims["Diff"] = diff;
     sm->updateImages(ims, 1);
@@ -500,11 +510,10 @@ void SimulationCbed::simulate() {
     n_waves = 1;
     // CbedPosition: the mode's area is the point (x, x, y, y)
     initialiseProbeWave(sm->scenario.area[0], sm->scenario.area[2]);
-    runSlices(config);
     const int res = geom.resolution;
+    Image diff(res, res, sm->outputCount(geom.n_slices));
+    runSlices(config, [&](int k) { diff.slices[k] = getDiffractionImage(0); });
     std::map<std::string, Image> ims;
-    Image diff(res, res, 1);
-    diff.slices[0] = getDiffractionImage(0);
     std::fill(diff.weighting.begin(), diff.weighting.end(), 1.0);
     ims["Diff"] = diff;
     sm->updateImages(ims, 1);
@@ -534,20 +543,23 @@ void SimulationStem::simulate() {
     const int nd = (int)sm->detectors.size();
     std::vector<mse_detector> dets(nd);
     for (int d = 0; d < nd; ++d) dets[d] = sm->detectors[d].d;
-    std::vector<double> acc(pixels.size() * nd, 0.0);
+    const int n_out = sm->outputCount(geom.n_slices);
+    std::vector<double> acc(pixels.size() * nd * n_out, 0.0);  // [output][pixel][detector]
     const int batch = std::max(1, std::min(sm->parallel_pixels, (int)pixels.size()));
     // every probe draws its own focus change (initialiseProbeWave is called per probe)
     std::vector<double> df(sm->chromatic.enabled ? pixels.size() : 0);
     for (size_t i = 0; i < df.size(); ++i) df[i] = sm->focusChange(config, pixels[i]);
     check(mse_set_focus_spread(eng, df.data(), (int)df.size()));
-    if (!sm->plasmon.enabled) {
+    if (!sm->plasmon.enabled && sm->slice_step <= 0) {
         check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
                              sm->sqrt2 ? 1 : 0, 1, acc.data()));
         n_waves = batch;
     } else {
-        // every batch of pixels is one job with its own scattering history, as each of
-        // clTEM's STEM jobs is (simulationstem.cpp:190-232); detectors read the diffraction
-        // pattern moved back by the beam change (getStemPixel with d_kx, d_ky)
+        // intermediate outputs or plasmons: the batches go through the slice loop of
+        // runSlices with getStemPixel at every output (simulationstem.cpp:210-270); with
+        // plasmons every batch of pixels is one job with its own scattering history, as each
+        // of clTEM's STEM jobs is, and the detectors read the diffraction pattern moved back by
+        // the beam change (getStemPixel with d_kx, d_ky)
         const int n_batches = ((int)pixels.size() + batch - 1) / batch;
         std::vector<double> out;
         for (int b = 0; b < n_batches; ++b) {
@@ -560,20 +572,22 @@ void SimulationStem::simulate() {
                 n_waves = nb;
             }
             if (!df.empty()) check(mse_set_focus_spread(eng, df.data() + p0, nb));
+            // the probe set-up keys the mixed-set draws 0..nb-1: each batch draws with its own seed
+            if (sm->mixedSets() >= 2) check(mse_set_mixed_potentials(eng, 0, sm->mixedSets(), sm->mixedSeed(config, b)));
             check(mse_init_probes(eng, pos.data() + 2 * (size_t)p0, nb, &sm->probe));
-            runSlices(config * n_batches + b);
-            out.assign((size_t)nb * nd, 0.0);
-            check(mse_stem_detect(eng, dets.data(), nd, sm->sqrt2 ? 1 : 0, out.data()));
-            std::copy(out.begin(), out.end(), acc.begin() + (size_t)p0 * nd);
+            runSlices(config * n_batches + b, [&](int k) {
+                out.assign((size_t)nb * nd, 0.0);
+                check(mse_stem_detect(eng, dets.data(), nd, sm->sqrt2 ? 1 : 0, out.data()));
+                std::copy(out.begin(), out.end(), acc.begin() + ((size_t)k * pixels.size() + p0) * nd);
+            });
         }
     }
     std::map<std::string, Image> ims;
     for (int d = 0; d < nd; ++d) {
-        Image im(nx, ny, 1);
-        for (size_t i = 0; i < pixels.size(); ++i) {
-            im.slices[0][pixels[i]] = acc[i * nd + d];
-            im.weighting[pixels[i]] = 1.0;
-        }
+        Image im(nx, ny, n_out);
+        for (int k = 0; k < n_out; ++k)
+            for (size_t i = 0; i < pixels.size(); ++i) im.slices[k][pixels[i]] = acc[((size_t)k * pixels.size() + i) * nd + d];
+        for (size_t i = 0; i < pixels.size(); ++i) im.weighting[pixels[i]] = 1.0;
         ims[sm->detectors[d].name] = im;
     }
     sm->updateImages(ims, 1);
@@ -751,8 +765,8 @@ std::string settings_json(const SimulationManager& sm, const std::string& image)
     j["slice offset"]["val"] = sc.slice_offset;
     j["slice offset"]["units"] = "Å";
     j["slice count"] = g.n_slices;
-    j["intermediate output"]["slice interval"] = 0;  // only the exit surface is returned
-    j["intermediate output"]["enabled"] = false;
+    j["intermediate output"]["slice interval"] = sm.slice_step;
+    j["intermediate output"]["enabled"] = sm.slice_step > 0;
     j["maintain areas"] = false;
     const char* axes[] = {"x", "y", "z"};
     const double starts[] = {g.sim_x[0], g.sim_y[0], g.z_lims[0]}, ends[] = {g.sim_x[1], g.sim_y[1], g.z_lims[1]};
@@ -920,7 +934,17 @@ void save_outputs(const SimulationManager& sm, const std::string& prefix) {
             for (size_t j = 0; j < im.slices.size(); ++j) {
                 std::vector<double> d(npix);
                 for (size_t k = 0; k < npix; ++k) d[k] = value(im.slices[j], k);
-                const std::string fn = im.slices.size() > 1 ? base + "_" + std::to_string(j) + ".tif" : base + ".tif";
+                std::string fn = base + ".tif";
+                if (im.slices.size() > 1) {
+                    // saveTiffOutput (console/main.cpp:158-186): the slice that ends output j,
+                    // zero-padded to the width of the last slice index
+                    const int sc = sm.geometry().n_slices;
+                    const int slice_id = std::min((int)(j + 1) * sm.slice_step - 1, sc - 1);
+                    std::string id = std::to_string(slice_id);
+                    const size_t width = std::to_string(sc - 1).size();
+                    id = std::string(width > id.size() ? width - id.size() : 0, '0') + id;
+                    fn = base + "_" + id + ".tif";
+                }
                 save_tiff(fn, d, im.width, im.height);
             }
         };

@@ -12,6 +12,7 @@
 #include <array>
 #include <climits>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -141,6 +142,10 @@ struct SimulationManager {
     // mixed static potentials (setParallelPotentialsCount): with phonons and >= 2, jobs share
     // this many phonon configurations and draw one per probe and slice
     int parallel_potentials = 0;
+    // intermediateSliceStep: outputs after every slice_step slices and at the exit (0: exit only)
+    int slice_step = 0;
+    // intermediate outputs of a run: ceil(slices / slice_step), or 1
+    int outputCount(int n_slices) const { return slice_step > 0 ? (n_slices + slice_step - 1) / slice_step : 1; }
 
     // ChromaticAberration::getFocusChange for probe `probe` of job `job` (Angstrom): an
     // asymmetric Gaussian in energy, 10000 * Cc[mm] * dE / kV (chromaticaberration.cpp:43-58, 99-106)
@@ -149,8 +154,9 @@ struct SimulationManager {
     void sourceOffset(int job, double* dx, double* dy) const;
     // parallelPotentialsCount (simulationmanager.h:218-231): the number of shared sets, or 0
     int mixedSets() const;
-    // seed of job `job`'s set draws: Philox4x32-10 words 0, 1 at (job, 0x3B, 0, 7)
-    uint64_t mixedSeed(int job) const;
+    // seed of the set draws of job `job` (batch `batch` of its probes when they are set up
+    // one batch at a time): Philox4x32-10 words 0, 1 at (job, 0x3B, batch, 7)
+    uint64_t mixedSeed(int job, int batch = 0) const;
 
     // SimulationManager::updateImages: FP64 sum of every job's images and weights
     void updateImages(std::map<std::string, Image>& ims, int jobCount);
@@ -184,8 +190,10 @@ class SimulationGeneral {
     // the slice loop of simulate() (simulationcbed.cpp:142-199): with plasmons, after the slice
     // whose end reaches the next scattering depth the beam vector is rotated and the job's
     // propagator and remaining transmission functions follow it (modifyBeamTilt).  `job` keys
-    // the scattering draws.  Returns the number of scatterings applied.
-    int runSlices(int job);
+    // the scattering draws.  `output(k)` is called for intermediate output k after every
+    // slice_step slices (after that slice's scattering) and, if the last one was not at the
+    // exit surface, once more at the end.  Returns the number of scatterings applied.
+    int runSlices(int job, const std::function<void(int)>& output);
     void resetBeam();  // the geometry's beam again, with its transmission functions rebuilt
     std::vector<double> getDiffractionImage(int parallel_ind);
     std::vector<double> getExitWaveImage(unsigned t = 0, unsigned l = 0, unsigned b = 0, unsigned r = 0);

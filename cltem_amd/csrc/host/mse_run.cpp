@@ -7,6 +7,7 @@
 //           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
 //           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
 //           [--plasmon mfp_nm,characteristic_mrad,critical_mrad[,n_individual]] [--parallel-potentials N]
+//           [--intermediate SLICES]
 //           [--device D] [--tiff] --out PREFIX
 //   mse_run ... --dump-atoms FILE [--dump-config C]   (writes the displaced atoms of phonon
 //           configuration C as x,y,z doubles and Z ints, and exits without using the GPU)
@@ -136,6 +137,8 @@ int main(int argc, char** argv) {
                 sm->plasmon.characteristic_angle = v.at(1);
                 sm->plasmon.critical_angle = v.at(2);
                 sm->plasmon.individual = v.size() > 3 ? (int)v[3] : 0;
+            } else if (a == "--intermediate") {
+                sm->slice_step = std::stoi(next());
             } else if (a == "--parallel-potentials") {
                 sm->parallel_potentials = std::stoi(next());
             } else if (a == "--dump-settings") {

@@ -143,8 +143,9 @@ def mixed_potential_picks(seed: int, keys, n_slices: int, n_sets: int) -> np.nda
     return ((r * np.uint64(n_sets)) >> np.uint64(32)).astype(np.int64).reshape(len(keys), n_slices)
 
 
-def mixed_seed(seed: int, job: int) -> int:
-    """Seed of one job's set draws (the C++ host's SimulationManager::mixedSeed): words 0 and 1
-    of Philox4x32-10 keyed by the manager's seed at (job, 0x3B, 0, 7)."""
-    r = philox4x32(np.array([[job, 0x3B, 0, 7]], dtype=np.uint64), _key(seed))[0]
+def mixed_seed(seed: int, job: int, batch: int = 0) -> int:
+    """Seed of one job's set draws (the C++ host's SimulationManager::mixedSeed; `batch` for
+    probes set up one batch at a time): words 0 and 1 of Philox4x32-10 keyed by the manager's
+    seed at (job, 0x3B, batch, 7)."""
+    r = philox4x32(np.array([[job, 0x3B, batch, 7]], dtype=np.uint64), _key(seed))[0]
     return int(r[0]) | (int(r[1]) << 32)

@@ -365,9 +365,13 @@ def run_plane(trans, P):
 
 def run_stem(g, trans, P, positions, detectors, aberr=None, cond_ap_mrad=20.0, smooth_mrad=0.0,
              taper=False, sqrt2=True, nthreads=None, fp32=False):
-    """positions: (n, 2) absolute Angstrom; detectors: (ndet, 4) inner, outer, xc, yc in mrad."""
+    """positions: (n, 2) absolute Angstrom; detectors: (ndet, 4) inner, outer, xc, yc in mrad.
+    Runs g.n_slices slices: trans must hold at least that many."""
     trans = np.ascontiguousarray(trans, dtype=np.complex128)
     P = np.ascontiguousarray(P, dtype=np.complex128)
+    res = g.resolution
+    if trans.ndim != 3 or trans.shape[0] < g.n_slices or trans.shape[1:] != (res, res) or P.shape != (res, res):
+        raise ValueError("transmission functions or propagator do not match the geometry")
     pos = _f64(positions).reshape(-1, 2)
     det = _f64(detectors).reshape(-1, 4)
     ab = _f64(np.zeros(25) if aberr is None else aberr)

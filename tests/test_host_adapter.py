@@ -445,3 +445,66 @@ def test_cif_zone_axis_and_occupancy(tmp_path):
     assert abs(np.sum(Z1 == 8) / 3000 - 0.5) < 0.04
     _, Z3 = _cif_atoms(tmp_path, cif2, "--widths", f"{w},{w},{w}", "--seed", "6")
     assert not np.array_equal(Z1, Z3)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_intermediate_outputs_match_oracle(tmp_path, oracle):
+    """--intermediate 4: EW and diffraction after every 4 slices and at the exit surface
+    (SimulationCtem::simulate, simulationctem.cpp:285-400), written as clTEM names the stacks."""
+    prefix = str(tmp_path / "ctem_int")
+    subprocess.run([RUN, "--mode", "ctem", "--srtio3", "3,3,3", "--res", "256", "--dz", "1.9525",
+                    "--intermediate", "4", "--tiff", "--params", os.path.join(PARAMS, "kirkland.dat"),
+                    "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    w = wl.ctem_srtio3(3, 3, 3)
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    s = w["specimen"]
+    p = cltem_amd.load_params("kirkland")
+    tr = oracle.transmission(og, oracle.sort_atoms(og, s.x, s.y, s.z, s.Z), p.values, 0, 3)
+    P = oracle.propagator(og)
+    n = og.n_slices
+    n_out = -(-n // 4)
+    assert meta["EW"]["depth"] == n_out and n % 4 != 0  # the exit surface is an extra output
+    ew = out["EW"].view(np.complex128).reshape(n_out, 256, 256)
+    dif = out["Diff"].reshape(n_out, 256, 256)
+    width = len(str(n - 1))
+    for k in range(n_out):
+        m = min(4 * (k + 1), n)
+        psi, recip = oracle.run_plane(tr[:m], P)
+        assert np.linalg.norm(ew[k] - psi) / np.linalg.norm(psi) <= 1e-4, k
+        ref = oracle.diffraction(recip)
+        assert np.linalg.norm(dif[k] - ref) / np.linalg.norm(ref) <= 1e-3, k
+        sid = str(m - 1).zfill(width)  # saveTiffOutput: the slice that ends the output
+        for name in (f"EW_amplitude_{sid}", f"EW_phase_{sid}", f"Diff_{sid}"):
+            assert os.path.exists(f"{prefix}_{name}.tif"), name
+    settings = json.load(open(f"{prefix}_Diff.json"))
+    assert settings["intermediate output"] == {"enabled": True, "slice interval": 4}
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_stem_intermediate_outputs_match_oracle(tmp_path, oracle):
+    prefix = str(tmp_path / "stem_int")
+    x0, x1 = 2 * 3.905, 3 * 3.905
+    subprocess.run([RUN, "--mode", "stem", "--srtio3", "5,5,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{x0},{x1},{x0},{x1}", "--stem-pixels", "2", "--det", "ADF,60,200",
+                    "--alpha", "20", "--batch", "3", "--intermediate", "4",
+                    "--params", os.path.join(PARAMS, "kirkland.dat"), "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    w = wl.stem_srtio3(nx=5, ny=5, nz=3, resolution=256, pixels=2, window_cells=(2, 3))
+    og = oracle.geometry(oracle.scenario(**w["scenario"]))
+    s = w["specimen"]
+    p = cltem_amd.load_params("kirkland")
+    tr = oracle.transmission(og, oracle.sort_atoms(og, s.x, s.y, s.z, s.Z), p.values, 0, 3)
+    P = oracle.propagator(og)
+    n = og.n_slices
+    n_out = -(-n // 4)
+    got = out["ADF"].reshape(n_out, 4)
+    pos = wl.stem_positions(og)
+    for k in range(n_out):
+        m = min(4 * (k + 1), n)
+        og_m = type(og).from_buffer_copy(og)  # run_stem takes the slice count from the geometry
+        og_m.n_slices = m
+        ref = oracle.run_stem(og_m, np.ascontiguousarray(tr[:m]), P, pos, np.array([[60.0, 200.0, 0.0, 0.0]]))[:, 0]
+        assert np.max(np.abs(got[k] - ref) / np.abs(ref)) <= 1e-3, k
