@@ -115,23 +115,30 @@ def _run_group_body(res, items, params_name, device, autotune, graph, warmup, re
 
 
 def run_mixed_cbed(instances, params_name="kirkland", device=0, autotune=True, graph=True, warmup=0,
-                   repeats=1):
+                   repeats=1, concurrent=True):
     """Phonon-averaged CBED patterns |Psi(k)|^2 (fft-shifted, FP64) of every instance.
-    Each size group builds its transmission functions, runs `warmup` untimed passes, waits
-    for the other groups, then runs `repeats` timed passes (probe init, slice loop, FP64
-    sums) concurrently with them.  Returns (patterns in instance order, per-size results)."""
+    Each size group builds its transmission functions, runs `warmup` untimed passes, then
+    `repeats` timed passes (probe init, slice loop, FP64 sums): concurrently with the other
+    groups after a common barrier, or (concurrent=False) one group after another, each with
+    the whole GPU.  Returns (patterns in instance order, per-size results)."""
     groups: dict = {}
     for i, inst in enumerate(instances):
         groups.setdefault(inst.scenario["resolution"], []).append((i, inst))
     out, errors = {}, []
-    barrier = threading.Barrier(len(groups)) if len(groups) > 1 else None
-    threads = [threading.Thread(target=_run_group, args=(res, items, params_name, device, autotune, graph,
-                                                          warmup, max(1, repeats), barrier, out, errors))
-               for res, items in sorted(groups.items())]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
+    if concurrent:
+        barrier = threading.Barrier(len(groups)) if len(groups) > 1 else None
+        threads = [threading.Thread(target=_run_group, args=(res, items, params_name, device, autotune, graph,
+                                                              warmup, max(1, repeats), barrier, out, errors))
+                   for res, items in sorted(groups.items())]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    else:
+        for res, items in sorted(groups.items()):
+            _run_group(res, items, params_name, device, autotune, graph, warmup, max(1, repeats), None, out, errors)
+            if errors:
+                break
     if errors:
         real = [e for e in errors if not isinstance(e[1], threading.BrokenBarrierError)] or errors
         raise RuntimeError(f"size group {real[0][0]} failed: {real[0][1]}") from real[0][1]
