@@ -114,11 +114,11 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
             float* o = e->adj_phase.p + (size_t)s0 * N;
             launch(e, PK_POT, [&]() {
                 if (e->form == 0)
-                    mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+                    mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
                 else if (e->form == 1)
-                    mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+                    mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
                 else
-                    mse::k_phase_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, pa, s0);
+                    mse::k_phase_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
             });
         }
         const size_t ng = 3 * (size_t)std::max(n_atoms, 1);

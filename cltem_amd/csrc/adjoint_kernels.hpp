@@ -14,7 +14,8 @@
 
 namespace mse {
 
-// d/dr of vproj<FORM> (same parameter layout and constants):
+// d/dr of the projected potentials of vproj_tab<FORM> (kernels/transmission_potentials_projected_f.cl:71-181),
+// from the raw parameters (p = a1,b1,..,c1,d1,.. for Kirkland; a1..a5,b1..b5 otherwise):
 //   Kirkland  300.82 sum a K0(2 pi r sqrt b) + 150.41 sum (c/d) exp(-(pi r)^2/d)
 //             -> 300.82 sum a (-2 pi sqrt b) K1(2 pi r sqrt b) + 150.41 sum (c/d)(-2 pi^2 r/d) exp(..)
 //   Peng      150.41 sum (a/b) exp(-(pi r)^2/b) -> 150.41 sum (a/b)(-2 pi^2 r/b) exp(..)

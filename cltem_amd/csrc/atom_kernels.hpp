@@ -156,32 +156,87 @@ __device__ __forceinline__ float bessk1f(float x) {
 }
 
 // Projected potentials of one atom at distance r (kernels/transmission_potentials_projected_f.cl:104-181)
+// The same K0 and K1 with the hardware reciprocal, log2 and exp2 (v_rcp_f32, v_log_f32,
+// v_exp_f32, about 1 ulp each) in place of IEEE division and the library log and exp.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float ln_hw(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
+__device__ __forceinline__ float bessk0_hw(float x) {
+    if (x <= 1.0f) {
+        const float z = x * x;
+        const float term = poly4(z, 1.0f, 2.346487949187396e-1f, 1.187082088663404e-2f,
+                                 2.150707366040937e-4f, 1.425433617130587e-6f) *
+                           ln_hw(x) *
+                           rcp_hw(poly2(1.0f - z, 9.847324170755358e-1f, 1.518396076767770e-2f,
+                                        8.362215678646257e-5f));
+        return poly4(z, 1.159315156584126e-1f, 2.770731240515333e-1f, 2.066458134619875e-2f,
+                     4.574734709978264e-4f, 3.454715527986737e-6f) *
+                   rcp_hw(poly2(1.0f - z, 9.836249671709183e-1f, 1.627693622304549e-2f,
+                                9.809660603621949e-5f)) -
+               term;
+    }
+    const float pp[8] = {1.253314137315499f, 1.475731032429900e1f, 6.123767403223466e1f,
+                         1.121012633939949e2f, 9.285288485892228e1f, 3.198289277679660e1f,
+                         3.595376024148513f, 6.160228690102976e-2f};
+    const float qq[8] = {1.0f, 1.189963006673403e1f, 5.027773590829784e1f, 9.496513373427093e1f,
+                         8.318077493230258e1f, 3.181399777449301e1f, 4.443672926432041f,
+                         1.408295601966600e-1f};
+    const float z = rcp_hw(x);
+    return exp_hw(-x) * poly7(z, pp) * rcp_hw(poly7(z, qq)) * rsqrtf(x);
+}
+
+__device__ __forceinline__ float bessk1_hw(float x) {
+    if (x <= 1.0f) {
+        const float z = x * x;
+        const float term = poly4(z, 0.5f, 5.598072040178741e-2f, 1.818666382168295e-3f,
+                                 2.397509908859959e-5f, 1.239567816344855e-7f) *
+                           ln_hw(x) *
+                           rcp_hw(poly2(1.0f - z, 9.870202601341150e-1f, 1.292092053534579e-2f,
+                                        5.881933053917096e-5f));
+        return x * (poly4(z, -3.079657578292062e-1f, -8.109417631822442e-2f, -3.477550948593604e-3f,
+                          -5.385594871975406e-5f, -3.110372465429008e-7f) *
+                        rcp_hw(poly2(1.0f - z, 9.861813171751389e-1f, 1.375094061153160e-2f,
+                                     6.774221332947002e-5f)) +
+                    term) +
+               rcp_hw(x);
+    }
+    const float pp[8] = {1.253314137315502f, 1.457171340220454e1f, 6.063161173098803e1f,
+                         1.147386690867892e2f, 1.040442011439181e2f, 4.356596656837691e1f,
+                         7.265230396353690f, 3.144418558991021e-1f};
+    const float qq[8] = {1.0f, 1.125154514806458e1f, 4.427488496597630e1f, 7.616113213117645e1f,
+                         5.863377227890893e1f, 1.850303673841586e1f, 1.857244676566022f,
+                         2.538540887654872e-2f};
+    const float z = rcp_hw(x);
+    return exp_hw(-x) * poly7(z, pp) * rcp_hw(poly7(z, qq)) * rsqrtf(x);
+}
+
+// Projected potentials from per-element constants precomputed on the host in double (the
+// square roots and reciprocals of the parameters hoisted out of the per-point sums; the
+// Gaussian exponents carry log2(e)).  Table layout per Z (kPotProjStride floats):
+//   Kirkland  a_i | 2 pi sqrt(b_i) | c_i / d_i | pi^2 log2e / d_i
+//   Peng      a_i / b_i | pi^2 log2e / b_i
+//   Lobato    a_i b_i^-3/2 | 2 pi / sqrt(b_i)
+constexpr int kPotProjStride = 16;
+
 template <int FORM>
-__device__ __forceinline__ float vproj(const float* __restrict__ p, int ic, float r) {
-    if constexpr (FORM == 0) {  // Kirkland C.20: p = a1,b1,a2,b2,a3,b3,c1,d1,c2,d2,c3,d3
-        const float x = 2.0f * (float)M_PI * r;
+__device__ __forceinline__ float vproj_tab(const float* __restrict__ t, int ic, float r) {
+    if constexpr (FORM == 0) {  // kernels/transmission_potentials_projected_f.cl:71-112
         float suml = 0.f, sumg = 0.f;
-        for (int i = 0; i < ic * 2; i += 2) suml += p[i] * bessk0f(x * sqrtf(p[i + 1]));
-        const float xg = ((float)M_PI * r) * ((float)M_PI * r);
-        for (int i = ic * 2; i < ic * 4; i += 2) {
-            const float d_inv = 1.0f / p[i + 1];
-            sumg += (p[i] * d_inv) * expf(-xg * d_inv);
-        }
+        for (int i = 0; i < ic; ++i) suml += t[i] * bessk0_hw(r * t[3 + i]);
+        const float r2 = r * r;
+        for (int i = 0; i < ic; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
         return 300.8242834f * suml + 150.4121417f * sumg;
-    } else if constexpr (FORM == 1) {  // Peng: a1..a5, b1..b5
-        const float x = ((float)M_PI * r) * ((float)M_PI * r);
+    } else if constexpr (FORM == 1) {
+        const float r2 = r * r;
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) {
-            const float b_inv = 1.0f / p[i + 5];
-            sum += p[i] * b_inv * expf(-x * b_inv);
-        }
+        for (int i = 0; i < ic; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
         return 150.4121417f * sum;
-    } else {  // Lobato: a1..a5, b1..b5
-        const float x = 2.0f * (float)M_PI * r;
+    } else {
         float sum = 0.f;
         for (int i = 0; i < ic; ++i) {
-            const float bir = rsqrtf(p[i + 5]);
-            sum += p[i] * (bir * bir * bir) * (bessk0f(x * bir) + r * bessk1f(x * bir));
+            const float x = r * t[5 + i];
+            sum += t[i] * (bessk0_hw(x) + r * bessk1_hw(x));
         }
         return 945.090144399935f * sum;
     }
@@ -263,7 +318,7 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
 template <int FORM>
 __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                  const int* __restrict__ aZ, const int* __restrict__ bsp,
-                                                 const float* __restrict__ params, const PotArgs& a, int first,
+                                                 const float* __restrict__ ptab, const PotArgs& a, int first,
                                                  double* sx, double* sy, int* sZ) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
@@ -302,8 +357,8 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                     double rad = sqrt(r2);
                     if (rad < r_min) rad = r_min;
                     if (rad <= 8.0)
-                        sumz += (double)vproj<FORM>(params + (size_t)(sZ[l] - 1) * a.per_atom,
-                                                    a.i_count, (float)rad);
+                        sumz += (double)vproj_tab<FORM>(ptab + (size_t)(sZ[l] - 1) * kPotProjStride,
+                                                        a.i_count, (float)rad);
                 }
             }
         }
@@ -317,11 +372,11 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
 template <int FORM>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
-    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ ptab,
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, params, a, first, sx, sy, sZ);
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
@@ -333,11 +388,11 @@ __global__ __launch_bounds__(256) void k_potential_projected(
 template <int FORM>
 __global__ __launch_bounds__(256) void k_phase_projected(
     float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
-    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ params,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ ptab,
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, params, a, first, sx, sy, sZ);
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =

@@ -520,6 +520,34 @@ int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const doub
                 }
             }
         }
+        // projected constants per element (vproj_tab's table layout)
+        std::vector<float> tp((size_t)nz * mse::kPotProjStride, 0.f);
+        for (int z = 0; z < nz; ++z) {
+            const double* p = params + (size_t)z * per;
+            float* t = tp.data() + (size_t)z * mse::kPotProjStride;
+            for (int i = 0; i < i_per_atom; ++i) {
+                if (form == 0) {
+                    const double a = p[2 * i], b = p[2 * i + 1], c = p[2 * i_per_atom + 2 * i],
+                                 d = p[2 * i_per_atom + 2 * i + 1];
+                    t[i] = (float)a;
+                    t[3 + i] = (float)(2.0 * pi * std::sqrt(b));
+                    t[6 + i] = (float)(c / d);
+                    t[9 + i] = (float)(pi * pi * log2e / d);
+                } else {
+                    const double a = p[i], b = p[i_per_atom + i];
+                    if (form == 1) {
+                        t[i] = (float)(a / b);
+                        t[5 + i] = (float)(pi * pi * log2e / b);
+                    } else {
+                        t[i] = (float)(a / (b * std::sqrt(b)));
+                        t[5 + i] = (float)(2.0 * pi / std::sqrt(b));
+                    }
+                }
+            }
+        }
+        e->params_proj.ensure(tp.size());
+        HIPX(hipMemcpyAsync(e->params_proj.p, tp.data(), tp.size() * sizeof(float), hipMemcpyHostToDevice,
+                            e->stream));
         e->params3d.ensure(t3.size());
         HIPX(hipMemcpyAsync(e->params3d.p, t3.data(), t3.size() * sizeof(float), hipMemcpyHostToDevice, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
@@ -653,11 +681,11 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
                         mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
                 } else {
                     if (e->form == 0)
-                        mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
                     else if (e->form == 1)
-                        mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
                     else
-                        mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params.p, a, first + s0);
+                        mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
                 }
             });
         }

@@ -124,6 +124,7 @@ struct mse_engine {
     std::vector<double> h_kx, h_ky;
     DevBuf<float> params;
     DevBuf<float> params3d;  // per-element constants of the full-3D potentials (kPot3dStride per Z)
+    DevBuf<float> params_proj;  // per-element constants of the projected potentials (kPotProjStride per Z)
     int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
     std::vector<ConfigData> configs;
     DevBuf<const float2*> cfg_ptrs;
