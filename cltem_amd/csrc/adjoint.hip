@@ -98,7 +98,6 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
                 "atoms, geometry or transmission functions changed since mse_forward_checkpointed");
         REQUIRE(e->n_waves == e->adj_nw, MSE_ERR_STATE, "wave count changed since mse_forward_checkpointed");
         const mse_geometry& g = cfg_geom(e, c);
-        REQUIRE(!g.full3d, MSE_ERR_UNSUPPORTED, "position gradients are implemented for projected potentials");
         const int n = e->adj_nslices, K = e->adj_k, nw = e->adj_nw;
         const size_t N = e->N, plane = (size_t)nw * N;
         const int n_atoms = c.n_input;
@@ -113,7 +112,14 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
             dim3 grid(e->res / 16, e->res / 16, cs);
             float* o = e->adj_phase.p + (size_t)s0 * N;
             launch(e, PK_POT, [&]() {
-                if (e->form == 0)
+                if (g.full3d) {
+                    if (e->form == 0)
+                        mse::k_phase_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                    else if (e->form == 1)
+                        mse::k_phase_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                    else
+                        mse::k_phase_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                } else if (e->form == 0)
                     mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
                 else if (e->form == 1)
                     mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
@@ -160,14 +166,29 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
                 rb.aux_in = e->adj_seg.p + (size_t)(s - start) * plane;
                 rb.aux_out = e->adj_gt.p;
                 row_pass(e, mse::ROW_ADJ, e->adj_g.p, e->adj_g.p, nw, N, rb);
-                const int a0 = bs[(size_t)s * nblk], a1 = bs[(size_t)(s + 1) * nblk];
+                // atoms whose potentials reach slice s: its own, or s +- load_z in full 3D
+                const int lz = g.full3d ? pa.load_z : 0;
+                const int k0 = std::max(0, s - lz), k1 = std::min(n - 1, s + lz);
+                const int a0 = bs[(size_t)k0 * nblk], a1 = bs[(size_t)(k1 + 1) * nblk];
                 if (a1 <= a0) continue;  // no atoms: phi_s does not depend on positions
                 launch(e, PK_OTHER, [&]() {
                     mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gsum);
                 });
                 band_limit_plane(e, gsum);
-                dim3 grid(a1 - a0);
                 const float* ph = e->adj_phase.p + (size_t)s * N;
+                if (g.full3d) {
+                    dim3 tiles(e->res / 16, e->res / 16);
+                    launch(e, PK_OTHER, [&]() {
+                        if (e->form == 0)
+                            mse::k_atom_grad_full3d<0><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
+                        else if (e->form == 1)
+                            mse::k_atom_grad_full3d<1><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
+                        else
+                            mse::k_atom_grad_full3d<2><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
+                    });
+                    continue;
+                }
+                dim3 grid(a1 - a0);
                 launch(e, PK_OTHER, [&]() {
                     if (e->form == 0)
                         mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);

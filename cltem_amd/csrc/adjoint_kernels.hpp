@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "atom_kernels.hpp"
+#include "slice_kernels.hpp"
 
 namespace mse {
 
@@ -155,6 +156,181 @@ __global__ __launch_bounds__(256) void k_atom_grad_projected(
         grad[o + 0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
         grad[o + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
         grad[o + 2] = 0.0;  // projected: the potential does not depend on z within a slice
+    }
+}
+
+// d/dr of the full-3D potentials of v3d_tab<FORM>, from the same per-element table
+// (exponent scales carry log2 e, so d/dr 2^(-u r) = -ln2 u 2^(-u r)):
+//   Kirkland  V = 150.41/r sum a 2^(-r u) + 266.52 sum c d^-3/2 2^(-r^2 w)
+//   Peng      V = 266.52 sum a b^-3/2 2^(-r^2 w)
+//   Lobato    V = 472.55 sum a b^-3/2 2^(-r u) (s/r + 1),  s = sqrt(b)/pi
+template <int FORM>
+__device__ __forceinline__ float dv3d_tab(const float* __restrict__ t, int ic, float r) {
+    const float ln2 = 0.6931471805599453f;
+    const float r2 = r * r;
+    const float inv_r = 1.0f / r;
+    if constexpr (FORM == 0) {
+        float s0 = 0.f, s1 = 0.f, sg = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float e = exp2f(-r * t[3 + i]);
+            s0 += t[i] * e;
+            s1 += t[i] * t[3 + i] * e;
+        }
+        for (int i = 0; i < ic; ++i) sg += t[6 + i] * t[9 + i] * exp2f(-r2 * t[9 + i]);
+        return 150.4121417f * (-s0 * inv_r * inv_r - ln2 * s1 * inv_r) + 266.5157269f * (-2.0f * ln2 * r * sg);
+    } else if constexpr (FORM == 1) {
+        float sg = 0.f;
+        for (int i = 0; i < ic; ++i) sg += t[i] * t[5 + i] * exp2f(-r2 * t[5 + i]);
+        return 266.5157269f * (-2.0f * ln2 * r * sg);
+    } else {
+        float sum = 0.f;
+        for (int i = 0; i < ic; ++i) {
+            const float e = exp2f(-r * t[5 + i]);
+            sum += t[i] * e * (-ln2 * t[5 + i] * (t[10 + i] * inv_r + 1.0f) - t[10 + i] * inv_r * inv_r);
+        }
+        return 472.545072199968f * sum;
+    }
+}
+
+__device__ __forceinline__ void add_grad(double* __restrict__ grad, const int* __restrict__ orig, int i,
+                                         double gx, double gy, double gz) {
+    double* o = grad + 3 * (size_t)orig[i];
+    unsafeAtomicAdd(o + 0, gx);
+    unsafeAtomicAdd(o + 1, gy);
+    unsafeAtomicAdd(o + 2, gz);
+}
+
+// Position gradients of the full-3D phase of slice `slice`: the same 16x16 pixel tiles, block
+// rows, atoms and integration points as full3d_sumz, in the same order.  With p1_e the
+// potential at evaluation e of a (k, j) row sequence, the trapezoid gives
+//   sumz = sum_e 0.5 [h_e != 0] (p1_e + p1_prev(e)),
+// so p1_e carries 0.5 [h_e != 0] from its own step and 0.5 from the next evaluation when
+// that one has h != 0; prev(e) may belong to the atom before (the value carried between
+// atoms of a row).  Per pixel f = sigma sst dL/dphi, and dp1/d(x, y, z)_atom =
+// -v'(r) rad / r (no derivative where the r_min clamp holds).  Each atom's own terms are
+// wave-reduced and added with one atomic per wave; the carried term of an atom's first
+// evaluation goes to the atom before, reduced when the whole wave agrees on that atom.
+template <int FORM>
+__global__ __launch_bounds__(256) void k_atom_grad_full3d(
+    const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+    const int* __restrict__ aZ, const int* __restrict__ bsp, const int* __restrict__ orig,
+    const float* __restrict__ tab3d, PotArgs a, int slice, const float2* __restrict__ g_tau,
+    const float* __restrict__ phase, double* __restrict__ grad) {
+#pragma clang fp contract(off)
+    __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int sZ[256];
+    const int lid = threadIdx.x;
+    const int lane = lid & 63;
+    const int xid = blockIdx.x * 16 + (lid & 15);
+    const int yid = blockIdx.y * 16 + (lid >> 4);
+    int starti, endi, startj, endj;
+    tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+    int topz = slice - a.load_z, bottomz = slice + a.load_z;
+    if (topz < 0) topz = 0;
+    if (bottomz >= a.n_slices) bottomz = a.n_slices - 1;
+    const double int_r = 1.0 / a.integrals;
+    const double sst = a.dz * int_r;
+    const double current_z = a.z_min + (a.n_slices - slice) * a.dz;
+    const double im_pos_x = a.startx + xid * a.pixelscale;
+    const double im_pos_y = a.starty + yid * a.pixelscale;
+    const double r_min = 0.25 * a.pixelscale;
+    const double r_min2 = r_min * r_min;
+    // f = sigma sst dL/dphi, dL/dphi = -Im(conj(g) exp(i phi))
+    double f;
+    {
+        const size_t p = (size_t)yid * a.res + xid;
+        float s, c;
+        sincosf(phase[p], &s, &c);
+        const float2 g = g_tau[p];
+        f = 0.5 * a.sigma * sst * (-((double)g.x * s - (double)g.y * c));  // with the trapezoid's 1/2
+    }
+    const int nb = a.blocks_x * a.blocks_y;
+    for (int k = topz; k <= bottomz; ++k) {
+        for (int j = startj; j <= endj; ++j) {
+            const int start = bsp[k * nb + a.blocks_x * j + starti];
+            const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
+            // the previous evaluation of this row sequence: its atom and f dp1/dpos
+            int prev = -1;
+            double px = 0.0, py = 0.0, pz = 0.0;
+            for (int c0 = start; c0 < end; c0 += 256) {
+                const int cnt = min(256, end - c0);
+                __syncthreads();
+                if (lid < cnt) {
+                    sx[lid] = ax[c0 + lid];
+                    sy[lid] = ay[c0 + lid];
+                    sz[lid] = az[c0 + lid];
+                    sZ[lid] = aZ[c0 + lid];
+                }
+                __syncthreads();
+                for (int l = 0; l < cnt; ++l) {
+                    const int atom = c0 + l;
+                    double rad_x = im_pos_x - sx[l];
+                    double rad_y = im_pos_y - sy[l];
+                    const float* tz = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+                    double gx = 0.0, gy = 0.0, gz = 0.0;      // this atom's own terms
+                    double cx = 0.0, cy = 0.0, cz = 0.0;      // carried term for `carry_to`
+                    int carry_to = -1;
+                    bool any = false;
+                    for (int h = 0; h <= a.integrals; ++h) {
+                        rad_x -= a.shift_x;
+                        rad_y -= a.shift_y;
+                        const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
+                        const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
+                        if (!(xyrad2 <= 64.0 && rad_z <= 3.0)) continue;
+                        any = true;
+                        const double r2 = xyrad2 + rad_z * rad_z;
+                        double dx = 0.0, dy = 0.0, dzz = 0.0;
+                        if (!(r2 < r_min2)) {
+                            const double r = sqrt(r2);
+                            const double q = -f * (double)dv3d_tab<FORM>(tz, a.i_count, (float)r) / r;
+                            dx = q * rad_x;
+                            dy = q * rad_y;
+                            dzz = q * rad_z;
+                        }
+                        if (h != 0) {
+                            gx += dx;
+                            gy += dy;
+                            gz += dzz;
+                            if (prev == atom) {
+                                gx += px;
+                                gy += py;
+                                gz += pz;
+                            } else if (prev >= 0) {
+                                carry_to = prev;
+                                cx = px;
+                                cy = py;
+                                cz = pz;
+                            }
+                        }
+                        prev = atom;
+                        px = dx;
+                        py = dy;
+                        pz = dzz;
+                    }
+                    // own terms: one atomic triple per wave
+                    if (__ballot(any)) {
+                        gx = wave_sum(gx);
+                        gy = wave_sum(gy);
+                        gz = wave_sum(gz);
+                        if (lane == 0 && (gx != 0.0 || gy != 0.0 || gz != 0.0))
+                            add_grad(grad, orig, atom, gx, gy, gz);
+                    }
+                    // carried terms
+                    const unsigned long long m = __ballot(carry_to >= 0);
+                    if (m) {
+                        const int t0 = __shfl(carry_to, __ffsll((long long)m) - 1, 64);
+                        if (__ballot(carry_to >= 0 && carry_to != t0) == 0) {
+                            cx = wave_sum(cx);
+                            cy = wave_sum(cy);
+                            cz = wave_sum(cz);
+                            if (lane == 0) add_grad(grad, orig, t0, cx, cy, cz);
+                        } else if (carry_to >= 0) {
+                            add_grad(grad, orig, carry_to, cx, cy, cz);
+                        }
+                    }
+                }
+            }
+        }
     }
 }
 

@@ -401,19 +401,18 @@ __global__ __launch_bounds__(256) void k_phase_projected(
 
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the
 // slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
-// kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).
+// kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Returns sumz at one
+// pixel of slice `slice` (16x16 pixel tile per block).
 template <int FORM>
-__global__ __launch_bounds__(256) void k_potential_full3d(
-    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
-    const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
-    const float* __restrict__ tab3d, PotArgs a, int first) {
+__device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
+                                              const double* __restrict__ az, const int* __restrict__ aZ,
+                                              const int* __restrict__ bsp, const float* __restrict__ tab3d,
+                                              const PotArgs& a, int slice, double* sx, double* sy, double* sz,
+                                              int* sZ) {
 #pragma clang fp contract(off)
-    __shared__ double sx[256], sy[256], sz[256];
-    __shared__ int sZ[256];
     const int lid = threadIdx.x;
     const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
-    const int slice = first + blockIdx.z;
     int starti, endi, startj, endj;
     tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
     int topz = slice - a.load_z, bottomz = slice + a.load_z;
@@ -480,8 +479,37 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
             }
         }
     }
-    const double sst = a.dz * int_r;
+    return sumz;
+}
+
+template <int FORM>
+__global__ __launch_bounds__(256) void k_potential_full3d(
+    float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
+    const float* __restrict__ tab3d, PotArgs a, int first) {
+    __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int sZ[256];
+    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
+    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const double sst = a.dz * (1.0 / a.integrals);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(sst * a.sigma * sumz);
+}
+
+// The full-3D phase sst sigma sumz of the same slices, reduced to [-pi, pi] (used by the adjoint).
+template <int FORM>
+__global__ __launch_bounds__(256) void k_phase_full3d(
+    float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
+    const float* __restrict__ tab3d, PotArgs a, int first) {
+    __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int sZ[256];
+    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
+    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const double sst = a.dz * (1.0 / a.integrals);
+    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
+        (float)remainder(sst * a.sigma * sumz, 2.0 * M_PI);
 }
 
 }  // namespace mse

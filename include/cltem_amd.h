@@ -273,7 +273,7 @@ int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_u
 /* ---- adjoint (position gradients).  No clTEM counterpart: clTEM only runs forward.  The
  *      forward differentiated is doMultiSliceStep over all slices
  *      (microscope/simulationgeneral.cpp:682-769) from the entrance waves, with the
- *      projected transmission functions of initialiseSimulation
+ *      projected or full-3D transmission functions of initialiseSimulation
  *      (simulationgeneral.cpp:345-646).  ---- */
 /* From the current entrance waves (all referring to configuration cfg), run every slice of
  * cfg, keeping ceil(n/K) checkpoints of the waves, K = ceil(sqrt(n_slices)).  Leaves the
@@ -285,9 +285,13 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
  * L = sum (I - I_target)^2 it is 2 (I - I_target)).  Recomputes each segment of K slices
  * from its checkpoint and runs the adjoint slice loop backwards.  grad_xyz receives
  * dL/dx, dL/dy, dL/dz for every atom of the last mse_sort_atoms call on cfg, in input order
- * (3 doubles per atom; atoms outside the simulated volume get 0).  Projected potentials
- * only (MSE_ERR_UNSUPPORTED for full 3D); there dL/dz = 0 because an atom's potential does
- * not depend on its depth within its slice. */
+ * (3 doubles per atom; atoms outside the simulated volume get 0).  With projected
+ * potentials dL/dz = 0 because an atom's potential does not depend on its depth within its
+ * slice; with full-3D potentials the z derivative follows the sub-slice integration
+ * (kernels/transmission_potentials_full_3d_f.cl:229-278), including the value carried
+ * between atoms of one block row.  Gradients are undefined where a coordinate sits exactly
+ * on a cutoff of the forward sum (slice and block edges, the 8 A lateral and 3 A depth
+ * limits); MSE_ERR_UNSUPPORTED under mixed static potentials. */
 int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz);
 /* ---- sweep timing: `reps` back-to-back row passes (slice `slice`, every wave with its own
  *      configuration, current variant) and then `reps` column passes, timed with hipEvents on

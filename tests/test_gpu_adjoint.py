@@ -166,14 +166,51 @@ def test_adjoint_errors(engine):
     with pytest.raises(cltem_amd.MseError) as ex:
         engine.adjoint_positions(np.ones_like(I), len(spec.Z))
     assert ex.value.code == -2
-    scen3 = dict(scen, full3d=1, full3d_integrals=4)
-    setup(engine, scen3, "kirkland")
-    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
-    engine.build_transmission(0)
-    engine.set_wave_count(1)
-    engine.set_wave_configs([0])
-    engine.init_plane_wave(1.0)
-    I = engine.forward_checkpointed(0)
-    with pytest.raises(cltem_amd.MseError) as ex:
-        engine.adjoint_positions(np.ones_like(I), len(spec.Z))
-    assert ex.value.code == -4
+
+
+def away_from_depth_cutoffs(g, z, margin):
+    """The full-3D sum takes integration point h of slice s when its depth minus the atom's
+    is at most 3 A; those depths lie on z_min + m dz / integrals, so the potential jumps as an
+    atom crosses z = z_min + m dz / integrals - 3.  A central difference must not straddle one."""
+    step = g.slice_thickness / g.full3d_integrals
+    u = (z - g.z_lims[0] + 3.0) / step
+    frac = u - np.floor(u)
+    return (frac > margin / step) & (frac < 1 - margin / step)
+
+
+@pytest.mark.parametrize("params,tilt", [("kirkland", 0.0), ("lobato", 6.0), ("peng", 0.0)])
+def test_adjoint_full3d_matches_fd(engine, oracle, params, tilt):
+    """Full-3D potentials (8 integration points per 1 A slice, +-3 slices of neighbours):
+    x, y and z gradients against the Richardson combination of FP64 central differences at
+    h = 1e-3 and 5e-4 A.  Tilt shifts every integration point sideways (the shifted path of
+    the sum).  Atoms are kept 3e-3 A away from slice edges and from the 3 A depth cutoffs,
+    where the potential is discontinuous."""
+    spec = sp.random_box({"Au": 8, "C": 24}, (12.0, 12.0, 5.0), 1.2, seed=11)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.0, pad_z=(0.0, 0.0),
+                struct_min=tuple(lo), struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]),
+                beam_tilt_mrad=tilt, beam_azimuth=0.7, full3d=1, full3d_integrals=8)
+    g, p = setup(engine, scen, params)
+    og = oracle.geometry(oracle.scenario(**scen))
+    tgt = spec.displaced(0.05, 21, 0)
+    target_I = oracle.ctem_intensity(og, tgt.x, tgt.y, tgt.z, tgt.Z, p.values, p.form, p.i_per_atom)
+    L, grad, _ = gpu_loss_and_grad(engine, spec, target_I)
+    assert np.all(np.isfinite(grad))
+    ok = away_from_slice_edges(g, spec.z, 3e-3) & away_from_depth_cutoffs(g, spec.z, 3e-3)
+    atoms = np.flatnonzero(ok)[:10]
+    assert len(atoms) >= 6
+    sel_atoms = np.repeat(atoms, 3)
+    sel_axes = np.tile([0, 1, 2], len(atoms))
+    fd = {}
+    for h in (1e-3, 5e-4):
+        fd[h], L0 = oracle.ctem_loss_fd(og, spec.x, spec.y, spec.z, spec.Z, p.values, p.form, p.i_per_atom,
+                                        target_I, sel_atoms, sel_axes, h=h)
+    print(f"loss: GPU {L:.8e}, FP64 oracle {L0:.8e}, rel {abs(L - L0) / L0:.2e}")
+    assert abs(L - L0) / L0 <= 1e-4
+    fd_r = (4.0 * fd[5e-4] - fd[1e-3]) / 3.0
+    got = grad[sel_atoms, sel_axes]
+    z = sel_axes == 2
+    assert np.abs(fd_r[z]).max() > 0.05 * np.abs(fd_r).max()  # depth matters in full 3D
+    rel_norm, err, scale = compare(got, fd_r, f"full 3D {params} tilt {tilt}")
+    assert rel_norm <= 1e-3
+    assert np.all(err <= 1e-3 * np.abs(fd_r) + 1e-3 * scale)
