@@ -46,6 +46,30 @@ enum ColMode : int {
 
 constexpr int kMaxDet = 8;
 
+// A pointer known to address global memory.  Pointers read from memory are generic, and
+// generic (flat) loads are counted by both the vector-memory and the LDS wait counters, so
+// every LDS wait would also wait for them; global loads are counted only by the former.
+__device__ __forceinline__ float2 gload(const float2* p) {
+    const unsigned long long u = *(const __attribute__((address_space(1))) unsigned long long*)p;
+    float2 r;
+    __builtin_memcpy(&r, &u, sizeof(r));
+    return r;
+}
+__device__ __forceinline__ int gload(const int* p) { return *(const __attribute__((address_space(1))) int*)p; }
+// Load of a table that no kernel writes while it runs (wave maps, configurations, table
+// pointers), through the constant address space: at a wave-uniform address this is a scalar
+// load, which waits neither for the wave's vector loads nor for its stores.
+template <class T>
+__device__ __forceinline__ T cload(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+__device__ __forceinline__ float2 cload(const float2* p) {
+    const unsigned long long u = *(const __attribute__((address_space(4))) unsigned long long*)p;
+    float2 r;
+    __builtin_memcpy(&r, &u, sizeof(r));
+    return r;
+}
+
 struct RowArgs {
     const float2* const* cfg_trans;  // per configuration: base of its slices
     const int* cfg_nslices;          // per configuration: its number of slices (t = 1 beyond)
@@ -70,10 +94,10 @@ struct RowArgs {
 // Philox4x32-10 (key mix_seed) at counter (mix_key[wave], slice, 0x3A, 6).
 __device__ __forceinline__ int row_config(const RowArgs& a, int wave) {
     if (a.mix_sets > 1) {
-        const uint4 r = philox4x32_10(make_uint4(a.mix_key[wave], (uint32_t)a.slice, 0x3Au, 6u), a.mix_seed);
+        const uint4 r = philox4x32_10(make_uint4(cload(a.mix_key + wave), (uint32_t)a.slice, 0x3Au, 6u), a.mix_seed);
         return a.mix_first + (int)(((uint64_t)r.x * (uint64_t)a.mix_sets) >> 32);
     }
-    return a.wave_cfg[wave];
+    return cload(a.wave_cfg + wave);
 }
 
 struct ProbeArgs {
@@ -492,7 +516,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
 
     float4 q[NQ];
     auto fetch = [&](int item) {
-        const int wv = a.wave_map ? a.wave_map[item / ntiles] : item / ntiles;
+        const int wv = a.wave_map ? cload(a.wave_map + item / ntiles) : item / ntiles;
         const int xx = (item % ntiles) * TC;
         const float4* in4 = reinterpret_cast<const float4*>(data + (size_t)wv * wave_stride + xx);
 #pragma unroll
@@ -502,12 +526,57 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
             q[k] = in4[((size_t)r * N) / 2 + cc / 2];
         }
     };
-    int item = blockIdx.x;
+    // the previous tile's output, held in registers and stored once this tile's input is in
+    // LDS: its stores then drain during this tile's transforms instead of delaying the LDS
+    // writes (the vector-memory counter orders loads and stores together)
+    float4 o[NQ];
+    auto store = [&](int item) {
+        const int wv = a.wave_map ? cload(a.wave_map + item / ntiles) : item / ntiles;
+        const int xx = (item % ntiles) * TC;
+        float4* out4 = reinterpret_cast<float4*>(data + (size_t)wv * wave_stride + xx);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int idx = tid + k * NT;
+            const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+            out4[((size_t)r * N) / 2 + cc / 2] = o[k];
+        }
+    };
+    // with one propagator for every wave the y factors of a lane are the same in every tile
+    const bool per_wave = a.wave_prop != nullptr;
+    float2 pyv[E];
+    if (!per_wave) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) pyv[m] = gload(a.py + t + L * m);
+    }
+    int item = blockIdx.x, prev = -1;
     if (item < nitems) fetch(item);
     for (; item < nitems; item += gridDim.x) {
-        const int wv = a.wave_map ? a.wave_map[item / ntiles] : item / ntiles;
+        const int wv = a.wave_map ? cload(a.wave_map + item / ntiles) : item / ntiles;
         const int x0 = (item % ntiles) * TC;
         const int x = x0 + c;
+        // propagator factors of this column, loaded before the next tile's prefetch so that
+        // waiting for them does not wait for the prefetch
+        float2 pxv;
+        int lim;
+        {
+            const float2* pxt = a.px;
+            if (per_wave) {
+                pxt = cload(a.wave_prop + 2 * wv);
+                const float2* pyt = cload(a.wave_prop + 2 * wv + 1);
+#pragma unroll
+                for (int m = 0; m < E; ++m) pyv[m] = gload(pyt + t + L * m);
+            }
+            const int xs = (x < N / 2) ? x : N - x;
+            if constexpr (L == 64) {
+                // one column per wave: scalar loads, which the stores issued before the
+                // multiply do not hold up
+                pxv = cload(pxt + __builtin_amdgcn_readfirstlane(x));
+                lim = cload(a.lim + __builtin_amdgcn_readfirstlane(xs));  // the mask is symmetric in kx and ky
+            } else {
+                pxv = gload(pxt + x);
+                lim = gload(a.lim + xs);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
             const int idx = tid + k * NT;
@@ -515,6 +584,7 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
             tile[cc * CS + lpad(r)] = make_float2(q[k].x, q[k].y);
             tile[(cc + 1) * CS + lpad(r)] = make_float2(q[k].z, q[k].w);
         }
+        if (prev >= 0) store(prev);
         __syncthreads();
         if (item + (int)gridDim.x < nitems) fetch(item + gridDim.x);
         float2* ln = tile + c * CS;
@@ -522,22 +592,12 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
         line_gather<N, L>(v, ln, t);
         sync();
         fft_line<PL, false>(v, ln, t, tws, sync);
-        {
-            const int xs = (x < N / 2) ? x : N - x;
-            const float2* pxt = a.px;
-            const float2* pyt = a.py;
-            if (a.wave_prop) {
-                pxt = a.wave_prop[2 * wv];
-                pyt = a.wave_prop[2 * wv + 1];
-            }
-            const float2 pxv = pxt[x];
-            const int lim = a.lim[xs];  // the aperture mask is symmetric in kx and ky
 #pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const int y = t + L * m;
-                const int ys = (y < N / 2) ? y : N - y;
-                v[m] = ys <= lim ? cmul(v[m], cmul(pxv, pyt[y])) : make_float2(0.f, 0.f);
-            }
+        for (int m = 0; m < E; ++m) {
+            const int y = t + L * m;
+            const int ys = (y < N / 2) ? y : N - y;
+            const float2 f = cmul(v[m], cmul(pxv, pyv[m]));
+            v[m] = ys <= lim ? f : make_float2(0.f, 0.f);
         }
         sync();
         fft_line<PL, true>(v, ln, t, tws, sync);
@@ -545,16 +605,17 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
 #pragma unroll
         for (int m = 0; m < E; ++m) ln[lpad(t + L * m)] = v[m];
         __syncthreads();
-        float4* out4 = reinterpret_cast<float4*>(data + (size_t)wv * wave_stride + x0);
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
             const int idx = tid + k * NT;
             const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
             const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
-            out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
+            o[k] = make_float4(a0.x, a0.y, a1.x, a1.y);
         }
+        prev = item;
         __syncthreads();
     }
+    if (prev >= 0) store(prev);
 }
 
 // --------------------------------------------------------------------------------------
@@ -581,15 +642,18 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
 
     float2 qv[E], qt[E];
     auto fetch = [&](int item) {
-        const int wv = a.wave_map ? a.wave_map[item / N] : item / N, row = item % N;
+        // with one line per wave the item is wave-uniform: its wave, configuration and
+        // transmission row then come from scalar loads
+        if constexpr (L == 64) item = __builtin_amdgcn_readfirstlane(item);
+        const int wv = a.wave_map ? cload(a.wave_map + item / N) : item / N, row = item % N;
         const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
         const int cfg = row_config(a, wv);
 #pragma unroll
         for (int m = 0; m < E; ++m) qv[m] = src[t + L * m];
-        if (a.slice < a.cfg_nslices[cfg]) {
-            const float2* tr = a.cfg_trans[cfg] + (size_t)a.slice * N * N + (size_t)row * N;
+        if (a.slice < cload(a.cfg_nslices + cfg)) {
+            const float2* tr = cload(a.cfg_trans + cfg) + (size_t)a.slice * N * N + (size_t)row * N;
 #pragma unroll
-            for (int m = 0; m < E; ++m) qt[m] = tr[t + L * m];
+            for (int m = 0; m < E; ++m) qt[m] = gload(tr + t + L * m);
         } else {  // vacuum below a thinner specimen
 #pragma unroll
             for (int m = 0; m < E; ++m) qt[m] = make_float2(1.f, 0.f);
@@ -611,7 +675,9 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[m]);
         fft_line<PL, false>(v, ln, t, tws, sync);
         sync();
-        const int wv = a.wave_map ? a.wave_map[item / N] : item / N, row = item % N;
+        int it = item;
+        if constexpr (L == 64) it = __builtin_amdgcn_readfirstlane(it);
+        const int wv = a.wave_map ? cload(a.wave_map + it / N) : it / N, row = it % N;
         float2* dst = data + (size_t)wv * wave_stride + (size_t)row * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) dst[t + L * m] = v[m];
