@@ -127,3 +127,32 @@ def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
             assert np.max(np.abs(out - eager[:8]) / np.abs(eager[:8])) < 1e-5, (rv, cv)
     engine.set_kernel_variant(0, 4)
     engine.set_kernel_variant(1, 7)
+
+
+@pytest.mark.parametrize("res", [512, 384, 1024])
+def test_column_variants_agree(engine, oracle, res):
+    """The persistent column kernels (prefetching, half lanes, 8-column tiles) do the same
+    butterflies with the same twiddles as the default; only the compiler's fusing of multiplies and adds into FMAs may differ between kernels, so the
+    exit waves of a small specimen agree to float rounding (rel-L2 1e-6)."""
+    w = wl.stem_srtio3(nx=4, ny=4, nz=3, resolution=res, pixels=4, window_cells=(1, 2))
+    g, og, p, spec = setup_case(engine, oracle, w)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    pos = wl.stem_positions(g)[:6]
+    probe = cltem_amd.make_probe(w["cond_ap_mrad"])
+    engine.set_wave_count(len(pos))
+    engine.set_wave_configs([0] * len(pos))
+    ref = None
+    for cv in (7, 8, 10):
+        engine.set_kernel_variant(1, cv)
+        engine.init_probes(pos, probe)
+        engine.multislice()
+        got = np.stack([engine.exit_wave(k) for k in range(len(pos))])
+        assert np.all(np.isfinite(got))
+        if ref is None:
+            ref = got
+        else:
+            err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            print(f"{res}^2 column variant {cv}: rel-L2 {err:.2e} against variant 7")
+            assert err <= 1e-6, (res, cv)
+    engine.set_kernel_variant(1, 7)
