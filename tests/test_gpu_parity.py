@@ -132,8 +132,9 @@ def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
 @pytest.mark.parametrize("res", [512, 384, 1024])
 def test_column_variants_agree(engine, oracle, res):
     """The persistent column kernels (prefetching, half lanes, 8-column tiles) do the same
-    butterflies with the same twiddles as the default; only the compiler's fusing of multiplies and adds into FMAs may differ between kernels, so the
-    exit waves of a small specimen agree to float rounding (rel-L2 1e-6)."""
+    butterflies with the same twiddles as the default; only the compiler's fusing of
+    multiplies and adds into FMAs may differ between kernels, so the exit waves of a small
+    specimen agree to float rounding (rel-L2 1e-6)."""
     w = wl.stem_srtio3(nx=4, ny=4, nz=3, resolution=res, pixels=4, window_cells=(1, 2))
     g, og, p, spec = setup_case(engine, oracle, w)
     engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
