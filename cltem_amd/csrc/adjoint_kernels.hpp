@@ -23,7 +23,8 @@ namespace mse {
 //   Lobato    945.09 sum a b^-3/2 (K0(c r) + r K1(c r)), c = 2 pi / sqrt b
 //             -> 945.09 sum a b^-3/2 (-c)(K1(c r) + r K0(c r))   (K0' = -K1, (z K1)' = -z K0)
 template <int FORM>
-__device__ __forceinline__ float dvproj(const float* __restrict__ p, int ic, float r) {
+__device__ __forceinline__ float dvproj(const float* __restrict__ p, int, float r) {
+    constexpr int ic = kTerms<FORM>;
     const float pi = (float)M_PI;
     if constexpr (FORM == 0) {
         const float x = 2.0f * pi * r;
@@ -165,7 +166,8 @@ __global__ __launch_bounds__(256) void k_atom_grad_projected(
 //   Peng      V = 266.52 sum a b^-3/2 2^(-r^2 w)
 //   Lobato    V = 472.55 sum a b^-3/2 2^(-r u) (s/r + 1),  s = sqrt(b)/pi
 template <int FORM>
-__device__ __forceinline__ float dv3d_tab(const float* __restrict__ t, int ic, float r) {
+__device__ __forceinline__ float dv3d_tab(const float* __restrict__ t, int, float r) {
+    constexpr int ic = kTerms<FORM>;
     const float ln2 = 0.6931471805599453f;
     const float r2 = r * r;
     const float inv_r = 1.0f / r;

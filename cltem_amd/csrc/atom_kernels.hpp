@@ -211,6 +211,12 @@ __device__ __forceinline__ float bessk1_hw(float x) {
     return exp_hw(-x) * poly7(z, pp) * rcp_hw(poly7(z, qq)) * rsqrtf(x);
 }
 
+// Terms per parameterisation (mse_set_parameterisation accepts no other count): three
+// Lorentzian + three Gaussian pairs for Kirkland, five terms for Peng and Lobato.  As a
+// compile-time count the term loops unroll and an atom's constants are loaded once.
+template <int FORM>
+constexpr int kTerms = FORM == 0 ? 3 : 5;
+
 // Projected potentials from per-element constants precomputed on the host in double (the
 // square roots and reciprocals of the parameters hoisted out of the per-point sums; the
 // Gaussian exponents carry log2(e)).  Table layout per Z (kPotProjStride floats):
@@ -220,21 +226,26 @@ __device__ __forceinline__ float bessk1_hw(float x) {
 constexpr int kPotProjStride = 16;
 
 template <int FORM>
-__device__ __forceinline__ float vproj_tab(const float* __restrict__ t, int ic, float r) {
+__device__ __forceinline__ float vproj_tab(const float* __restrict__ t, int, float r) {
+    constexpr int IC = kTerms<FORM>;
     if constexpr (FORM == 0) {  // kernels/transmission_potentials_projected_f.cl:71-112
         float suml = 0.f, sumg = 0.f;
-        for (int i = 0; i < ic; ++i) suml += t[i] * bessk0_hw(r * t[3 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) suml += t[i] * bessk0_hw(r * t[3 + i]);
         const float r2 = r * r;
-        for (int i = 0; i < ic; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
         return 300.8242834f * suml + 150.4121417f * sumg;
     } else if constexpr (FORM == 1) {
         const float r2 = r * r;
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
         return 150.4121417f * sum;
     } else {
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) {
+#pragma unroll
+        for (int i = 0; i < IC; ++i) {
             const float x = r * t[5 + i];
             sum += t[i] * (bessk0_hw(x) + r * bessk1_hw(x));
         }
@@ -254,22 +265,27 @@ __device__ __forceinline__ float vproj_tab(const float* __restrict__ t, int ic, 
 constexpr int kPot3dStride = 16;
 
 template <int FORM>
-__device__ __forceinline__ float v3d_tab(const float* __restrict__ t, int ic, float r2) {
+__device__ __forceinline__ float v3d_tab(const float* __restrict__ t, int, float r2) {
+    constexpr int IC = kTerms<FORM>;
     if constexpr (FORM == 0) {
         const float r = __builtin_amdgcn_sqrtf(r2);
         float suml = 0.f, sumg = 0.f;
-        for (int i = 0; i < ic; ++i) suml += t[i] * __builtin_amdgcn_exp2f(-r * t[3 + i]);
-        for (int i = 0; i < ic; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) suml += t[i] * __builtin_amdgcn_exp2f(-r * t[3 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) sumg += t[6 + i] * __builtin_amdgcn_exp2f(-r2 * t[9 + i]);
         return 150.4121417f * __builtin_amdgcn_rcpf(r) * suml + 266.5157269f * sumg;
     } else if constexpr (FORM == 1) {
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
+#pragma unroll
+        for (int i = 0; i < IC; ++i) sum += t[i] * __builtin_amdgcn_exp2f(-r2 * t[5 + i]);
         return 266.5157269f * sum;
     } else {
         const float r = __builtin_amdgcn_sqrtf(r2);
         const float inv_r = __builtin_amdgcn_rcpf(r);
         float sum = 0.f;
-        for (int i = 0; i < ic; ++i)
+#pragma unroll
+        for (int i = 0; i < IC; ++i)
             sum += t[i] * __builtin_amdgcn_exp2f(-r * t[5 + i]) * (t[10 + i] * inv_r + 1.0f);
         return 472.545072199968f * sum;
     }
@@ -346,6 +362,13 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
             }
             __syncthreads();
             for (int l = 0; l < cnt; ++l) {
+                // the element's constants, loaded ahead of the distance arithmetic
+                float pt[kPotProjStride];
+                {
+                    const float* row = ptab + (size_t)(sZ[l] - 1) * kPotProjStride;
+#pragma unroll
+                    for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
+                }
                 const double rad_x = im_pos_x - sx[l];
                 const double rad_y = im_pos_y - sy[l];
                 const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
@@ -357,8 +380,7 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                     double rad = sqrt(r2);
                     if (rad < r_min) rad = r_min;
                     if (rad <= 8.0)
-                        sumz += (double)vproj_tab<FORM>(ptab + (size_t)(sZ[l] - 1) * kPotProjStride,
-                                                        a.i_count, (float)rad);
+                        sumz += (double)vproj_tab<FORM>(pt, a.i_count, (float)rad);
                 }
             }
         }
@@ -447,20 +469,42 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                 for (int l = 0; l < cnt; ++l) {
                     double rad_x = im_pos_x - sx[l];
                     double rad_y = im_pos_y - sy[l];
-                    const float* pz = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+                    // the element's constants, loaded once per atom (scalar registers)
+                    float pz[kPot3dStride];
+                    {
+                        const float* row = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+#pragma unroll
+                        for (int q = 0; q < kPot3dStride; ++q) pz[q] = row[q];
+                    }
                     if (no_shift) {
                         // the xy distance is the same at every integration point
                         const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
                         if (xyrad2 > 64.0) continue;
-                        for (int h = 0; h <= a.integrals; ++h) {
-                            const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
-                            if (rad_z <= 3.0) {
-                                const double r2 = xyrad2 + rad_z * rad_z;
-                                const double p1 = (double)v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
-                                sumz += (h != 0) * (p1 + p2) * 0.5;
-                                p2 = p1;
-                            }
+                        // rad_z(h) = (current_z - h dz / integrals) - z falls with h (each
+                        // rounding step is monotone), so the points with rad_z <= 3 are
+                        // h0..integrals; h0 is found with the very expression of the per-point test
+                        const double zl = sz[l];
+                        auto within = [&](int h) { return (current_z - h * a.dz * int_r) - zl <= 3.0; };
+                        const double h_est = ceil((current_z - zl - 3.0) / (a.dz * int_r));
+                        int h0 = (int)fmin(fmax(h_est, 0.0), (double)(a.integrals + 1));
+                        while (h0 > 0 && within(h0 - 1)) --h0;
+                        while (h0 <= a.integrals && !within(h0)) ++h0;
+                        if (h0 > a.integrals) continue;
+                        // from there the depth steps in float and the atom's trapezoid terms are
+                        // summed in float, then added to the pixel's double sum once
+                        const float xy2f = (float)xyrad2;
+                        const float stepf = (float)(a.dz * int_r);
+                        const float rz0 = (float)((current_z - h0 * a.dz * int_r) - zl);
+                        float p2f = (float)p2, acc = 0.f;
+                        for (int h = h0; h <= a.integrals; ++h) {
+                            const float rz = fmaf(-(float)(h - h0), stepf, rz0);
+                            const float r2 = fmaf(rz, rz, xy2f);
+                            const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
+                            if (h != 0) acc += p1 + p2f;
+                            p2f = p1;
                         }
+                        sumz += 0.5 * (double)acc;
+                        p2 = (double)p2f;
                         continue;
                     }
                     for (int h = 0; h <= a.integrals; ++h) {
