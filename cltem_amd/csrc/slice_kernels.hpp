@@ -494,8 +494,8 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
 // loads of item i+1 are already in flight into registers, and the stores of item i drain
 // during the next item's transforms.  Same arithmetic as k_col<..., COL_PROP>.
 // --------------------------------------------------------------------------------------
-template <class PL, int TC, class SYNC = WaveSync, bool TWREG = true>
-__global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict__ data, size_t wave_stride,
+template <class PL, int TC, class SYNC = WaveSync, bool TWREG = true, int WPE = 1>
+__global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE))) void k_col_prop_pipe(float2* __restrict__ data, size_t wave_stride,
                                                               const float2* __restrict__ tw, ColArgs a,
                                                               int nsets) {
     constexpr int N = PL::N, L = PL::L, E = PL::E;
@@ -541,13 +541,23 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
             out4[((size_t)r * N) / 2 + cc / 2] = o[k];
         }
     };
-    // with one propagator for every wave the y factors of a lane are the same in every tile
+    // With one propagator for every wave the y factors are the same in every tile: a lane
+    // keeps its E of them in registers when E <= 8, larger plans read them from an LDS copy
+    // (registers are short there).  Per-wave propagators are loaded per tile.
+    constexpr bool PY_REGS = E <= 8;
     const bool per_wave = a.wave_prop != nullptr;
-    float2 pyv[E];
-    if (!per_wave) {
+    float2 pyv[PY_REGS ? E : 1];
+    __shared__ float2 spy[PY_REGS ? 1 : N];
+    if constexpr (PY_REGS) {
+        if (!per_wave) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) pyv[m] = gload(a.py + t + L * m);
+            for (int m = 0; m < E; ++m) pyv[m] = gload(a.py + t + L * m);
+        }
+    } else {
+        if (!per_wave)
+            for (int i = tid; i < N; i += NT) spy[i] = a.py[i];  // ordered by the first tile's barrier
     }
+    const float2* pyt = a.py;
     int item = blockIdx.x, prev = -1;
     if (item < nitems) fetch(item);
     for (; item < nitems; item += gridDim.x) {
@@ -562,19 +572,30 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
             const float2* pxt = a.px;
             if (per_wave) {
                 pxt = cload(a.wave_prop + 2 * wv);
-                const float2* pyt = cload(a.wave_prop + 2 * wv + 1);
+                pyt = cload(a.wave_prop + 2 * wv + 1);
+                if constexpr (PY_REGS) {
 #pragma unroll
-                for (int m = 0; m < E; ++m) pyv[m] = gload(pyt + t + L * m);
+                    for (int m = 0; m < E; ++m) pyv[m] = gload(pyt + t + L * m);
+                }
             }
-            const int xs = (x < N / 2) ? x : N - x;
-            if constexpr (L == 64) {
-                // one column per wave: scalar loads, which the stores issued before the
-                // multiply do not hold up
-                pxv = cload(pxt + __builtin_amdgcn_readfirstlane(x));
-                lim = cload(a.lim + __builtin_amdgcn_readfirstlane(xs));  // the mask is symmetric in kx and ky
-            } else {
-                pxv = gload(pxt + x);
-                lim = gload(a.lim + xs);
+            // the 64 / L columns of this wave: scalar loads, which the stores issued before the
+            // multiply do not hold up, then each lane takes its own column's values (the
+            // aperture mask is symmetric in kx and ky)
+            constexpr int CPW = 64 / L;
+            static_assert(L <= 64 && 64 % L == 0 && TC % CPW == 0, "whole columns per wave");
+            const int xw = __builtin_amdgcn_readfirstlane(x - c % CPW);
+            const int j = c % CPW;
+            pxv = cload(pxt + xw);
+            lim = cload(a.lim + ((xw < N / 2) ? xw : N - xw));
+#pragma unroll
+            for (int k = 1; k < CPW; ++k) {
+                const int xk = xw + k;
+                const float2 pk = cload(pxt + xk);
+                const int lk = cload(a.lim + ((xk < N / 2) ? xk : N - xk));
+                if (j == k) {
+                    pxv = pk;
+                    lim = lk;
+                }
             }
         }
 #pragma unroll
@@ -596,7 +617,12 @@ __global__ __launch_bounds__(TC * PL::L) void k_col_prop_pipe(float2* __restrict
         for (int m = 0; m < E; ++m) {
             const int y = t + L * m;
             const int ys = (y < N / 2) ? y : N - y;
-            const float2 f = cmul(v[m], cmul(pxv, pyv[m]));
+            float2 pyy;
+            if constexpr (PY_REGS)
+                pyy = pyv[m];
+            else
+                pyy = per_wave ? gload(pyt + y) : spy[y];
+            const float2 f = cmul(v[m], cmul(pxv, pyy));
             v[m] = ys <= lim ? f : make_float2(0.f, 0.f);
         }
         sync();

@@ -5,6 +5,22 @@
 
 namespace mse_rt {
 
+// Blocks of a persistent kernel that stay resident per CU (HIP's occupancy query for that
+// kernel, asked once), at most `cap`: the persistent sweeps size their grids to what runs
+// at once, so no block waits for another to finish before it starts.
+template <auto KERNEL>
+int resident_per_cu(int block_threads, int cap) {
+    static const int n = [&]() {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, KERNEL, block_threads, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            b = cap;
+        }
+        return std::max(1, std::min(b, cap));
+    }();
+    return n;
+}
+
 template <int N>
 void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets, size_t wave_stride,
                 const mse::RowArgs& args, int variant) {
@@ -52,7 +68,8 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
         if constexpr (N <= 512) {  // half-lane lines keep the prefetch registers within budget up to 512
             if (mode == mse::ROW_IFFT_T_FFT && variant == 5 && in == out) {
                 constexpr int LINES = 256 / P1::L;
-                const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+                const int per_cu = resident_per_cu<mse::k_row_step_pipe<P1, mse::WaveSync>>(256, 4);
+                const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N * nlines_sets + LINES - 1) / LINES));
                 launch(e, PK_ROW, [&]() {
                     mse::k_row_step_pipe<P1, mse::WaveSync><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p,
                                                                                          args, nlines_sets);
@@ -62,7 +79,8 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
         }
         if (mode == mse::ROW_IFFT_T_FFT && variant == 6 && in == out) {  // twiddles read at use
             constexpr int LINES = 256 / P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+            const int per_cu = resident_per_cu<mse::k_row_step_pipe<P0, mse::WaveSync, false>>(256, 4);
+            const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N * nlines_sets + LINES - 1) / LINES));
             launch(e, PK_ROW, [&]() {
                 mse::k_row_step_pipe<P0, mse::WaveSync, false><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p,
                                                                                             args, nlines_sets);
@@ -71,7 +89,8 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
         }
         if (mode == mse::ROW_IFFT_T_FFT && variant >= 4 && in == out) {
             constexpr int LINES = 256 / P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * 4, (N * nlines_sets + LINES - 1) / LINES));
+            const int per_cu = resident_per_cu<mse::k_row_step_pipe<P0, mse::WaveSync>>(256, 4);
+            const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N * nlines_sets + LINES - 1) / LINES));
             launch(e, PK_ROW, [&]() {
                 mse::k_row_step_pipe<P0, mse::WaveSync><<<blocks, 256, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
                                                                                      nlines_sets);
@@ -135,7 +154,8 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
         if constexpr (N <= 512) {
             if (variant == 8 && mode == mse::COL_PROP && in == out) {  // half lanes: two blocks per CU
                 constexpr int NT = 16 * P1::L;
-                const int blocks = std::max(1, std::min(e->num_cus * (2048 / NT), (N / 16) * nsets));
+                const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P1, 16, mse::WaveSync>>(NT, 2048 / NT);
+                const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N / 16) * nsets));
                 tiles = N / 16;
                 launch(e, kind, [&]() {
                     mse::k_col_prop_pipe<P1, 16, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride,
@@ -146,7 +166,8 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
         }
         if (variant == 10 && mode == mse::COL_PROP && in == out) {  // 8-column tiles (no spills at 1024)
             constexpr int NT = 8 * P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 8) * nsets));
+            const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, 8, mse::WaveSync>>(NT, 1024 / NT);
+            const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N / 8) * nsets));
             tiles = N / 8;
             launch(e, kind, [&]() {
                 mse::k_col_prop_pipe<P0, 8, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride, e->tw.p,
@@ -156,7 +177,8 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
         }
         if (variant == 9 && mode == mse::COL_PROP && in == out) {  // twiddles read at use
             constexpr int NT = 16 * P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
+            const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, 16, mse::WaveSync, false>>(NT, 1024 / NT);
+            const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N / 16) * nsets));
             tiles = N / 16;
             launch(e, kind, [&]() {
                 mse::k_col_prop_pipe<P0, 16, mse::WaveSync, false><<<blocks, NT, 0, e->stream>>>(out, wave_stride,
@@ -166,7 +188,8 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
         }
         if (variant >= 7 && mode == mse::COL_PROP && in == out) {
             constexpr int NT = 16 * P0::L;
-            const int blocks = std::max(1, std::min(e->num_cus * (1024 / NT), (N / 16) * nsets));
+            const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, 16, mse::WaveSync>>(NT, 1024 / NT);
+            const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N / 16) * nsets));
             tiles = N / 16;
             launch(e, kind, [&]() {
                 mse::k_col_prop_pipe<P0, 16, mse::WaveSync><<<blocks, NT, 0, e->stream>>>(out, wave_stride, e->tw.p, args,
