@@ -185,17 +185,27 @@ template <bool INV> struct Dft<16, INV> : DftComposite<4, 4, INV> {};
 template <int N_, int L_, int R0_, int R1_, int R2_>
 struct Plan3 {
     static constexpr int N = N_, L = L_, E = N_ / L_;
-    static constexpr int R0 = R0_, R1 = R1_, R2 = R2_;
+    static constexpr int R0 = R0_, R1 = R1_, R2 = R2_, R3 = 1;
     static_assert(R0_ * R1_ * R2_ == N_, "radices must multiply to N");
     static_assert(E % R0_ == 0 && E % R1_ == 0 && E % R2_ == 0, "radix must divide E");
+};
+// Four passes, for sizes whose elements per lane admit no three-radix split (768 points on
+// 64 lanes: 12 per lane, 4 * 4 * 4 * 12).  With the radices in this order every pass's p
+// divides L, so each pass needs one twiddle set per lane.
+template <int N_, int L_, int R0_, int R1_, int R2_, int R3_>
+struct Plan4 {
+    static constexpr int N = N_, L = L_, E = N_ / L_;
+    static constexpr int R0 = R0_, R1 = R1_, R2 = R2_, R3 = R3_;
+    static_assert(R0_ * R1_ * R2_ * R3_ == N_, "radices must multiply to N");
+    static_assert(E % R0_ == 0 && E % R1_ == 0 && E % R2_ == 0 && E % R3_ == 0, "radix must divide E");
 };
 
 template <int N>
 struct PlanFor;
 template <> struct PlanFor<256> { using P = Plan3<256, 32, 8, 8, 4>; };
-template <> struct PlanFor<384> { using P = Plan3<384, 16, 8, 8, 6>; };
+template <> struct PlanFor<384> { using P = Plan4<384, 32, 4, 4, 2, 12>; };
 template <> struct PlanFor<512> { using P = Plan3<512, 64, 8, 8, 8>; };
-template <> struct PlanFor<768> { using P = Plan3<768, 32, 8, 8, 12>; };
+template <> struct PlanFor<768> { using P = Plan4<768, 64, 4, 4, 4, 12>; };
 template <> struct PlanFor<1024> { using P = Plan3<1024, 64, 16, 8, 8>; };
 
 // Alternative plans with half the lanes per line (twice the elements per lane): more
@@ -203,6 +213,8 @@ template <> struct PlanFor<1024> { using P = Plan3<1024, 64, 16, 8, 8>; };
 template <int N>
 struct PlanHalfLanes { using P = typename PlanFor<N>::P; };
 template <> struct PlanHalfLanes<256> { using P = Plan3<256, 16, 8, 8, 4>; };
+template <> struct PlanHalfLanes<384> { using P = Plan3<384, 16, 8, 8, 6>; };
+template <> struct PlanHalfLanes<768> { using P = Plan3<768, 32, 8, 8, 12>; };
 template <> struct PlanHalfLanes<512> { using P = Plan3<512, 32, 8, 8, 8>; };
 template <> struct PlanHalfLanes<1024> { using P = Plan3<1024, 32, 16, 8, 8>; };
 
@@ -267,17 +279,30 @@ struct PassTwMem {
     __device__ __forceinline__ float2 get(int, int r, int k) const { return tw[r * k * step]; }
 };
 
+struct NoPassTw {
+    __device__ __forceinline__ void load(const float2* __restrict__, int) {}
+};
+
 template <class PL, bool IN_REGS = true>
 struct PlanTw {
     using T1 = typename std::conditional<IN_REGS, PassTw<PL::N, PL::L, PL::R1, PL::R0>,
                                          PassTwMem<PL::N, PL::L, PL::R1, PL::R0>>::type;
     using T2 = typename std::conditional<IN_REGS, PassTw<PL::N, PL::L, PL::R2, PL::R0 * PL::R1>,
                                          PassTwMem<PL::N, PL::L, PL::R2, PL::R0 * PL::R1>>::type;
+    // fourth pass (four-pass plans only)
+    static constexpr int R3 = PL::R3 > 1 ? PL::R3 : 2;
+    using T3 = typename std::conditional<
+        (PL::R3 > 1),
+        typename std::conditional<IN_REGS, PassTw<PL::N, PL::L, R3, PL::R0 * PL::R1 * PL::R2>,
+                                  PassTwMem<PL::N, PL::L, R3, PL::R0 * PL::R1 * PL::R2>>::type,
+        NoPassTw>::type;
     T1 p1;
     T2 p2;
+    T3 p3;
     __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
         p1.load(tw, t);
         p2.load(tw, t);
+        p3.load(tw, t);
     }
 };
 
@@ -339,7 +364,15 @@ __device__ __forceinline__ void fft_line(float2 (&v)[PL::E], float2* line, int t
     sync();
     line_gather<N, L, true>(v, line, t);
     sync();
-    stockham_pass<N, L, PL::R2, PL::R0 * PL::R1, INV, true>(v, line, t, &tw.p2);
+    if constexpr (PL::R3 == 1) {
+        stockham_pass<N, L, PL::R2, PL::R0 * PL::R1, INV, true>(v, line, t, &tw.p2);
+    } else {
+        stockham_pass<N, L, PL::R2, PL::R0 * PL::R1, INV, false, typename TWS::T2>(v, line, t, &tw.p2);
+        sync();
+        line_gather<N, L>(v, line, t);
+        sync();
+        stockham_pass<N, L, PL::R3, PL::R0 * PL::R1 * PL::R2, INV, true>(v, line, t, &tw.p3);
+    }
 }
 
 }  // namespace mse
