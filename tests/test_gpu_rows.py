@@ -233,3 +233,46 @@ def test_ccd_dose_matches_oracle_draw_for_draw(engine, oracle):
     err = rel_l2(got, want)
     print(f"K2, binning 2, 100 e/pixel: rel-L2 {err:.3e}")
     assert err <= 1e-3
+
+
+def test_full3d_depth_cutoff_exact(engine, oracle):
+    """A gold atom placed so that one integration point of a slice lies exactly 3 A above it
+    in double (rad_z == 3: the point counts), then at the nearest position where rad_z > 3
+    (it does not).  The GPU sum finds an atom's first counted point by search instead of
+    testing every point; it must decide the boundary as the FP64 per-point test of the
+    oracle does.  The two placements differ by one ulp, so the change they make to the
+    slice's phase is that one point's weight (about 8e-6 rad with 2 A slices and 4
+    integration points); the GPU's change must match the oracle's (float noise cancels
+    between the two GPU runs, while a wrong decision on either side would leave a change of
+    zero or double)."""
+    spec = sp.random_box({"Au": 6, "C": 20}, (10.0, 10.0, 8.0), 1.5, seed=5)
+    lo, hi = spec.limits
+    lo, hi = lo - 0.5, hi + 0.5  # fixed limits, so that moving an atom keeps the geometry
+    integrals = 4
+    scen = dict(mode=1, resolution=256, voltage_kv=300.0, slice_thickness=2.0, struct_min=tuple(lo),
+                struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]), full3d=1, full3d_integrals=integrals)
+    g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
+    s, h = g.n_slices // 2, 1
+    dz, int_r = g.slice_thickness, 1.0 / integrals
+    point_z = (g.z_lims[0] + (g.n_slices - s) * dz) - h * dz * int_r  # the kernels' expression
+    z_in = point_z - 3.0
+    z_out = z_in
+    while point_z - z_out <= 3.0:
+        z_out = np.nextafter(z_out, -np.inf)
+    assert point_z - z_in == 3.0 and point_z - z_out > 3.0
+    au = int(np.flatnonzero(spec.Z == 79)[0])
+    gpu, ref = [], []
+    for z in (z_in, z_out):
+        zs = spec.z.copy()
+        zs[au] = z
+        placed = sp.Specimen(spec.x.copy(), spec.y.copy(), zs, spec.Z.copy())
+        g, og, p, o = prepare(engine, oracle, scen, placed, "lobato")
+        gpu.append(engine.read_transmission(0, s))
+        ref.append(oracle.transmission(og, o, p.values, p.form, p.i_per_atom, first=s, count=1)[0])
+    d_gpu = np.angle(gpu[0] / gpu[1])
+    d_ref = np.angle(ref[0] / ref[1])
+    err = np.abs(d_gpu - d_ref).max()
+    print(f"slice {s}: the point moves the phase by {np.abs(d_ref).max():.3e} rad (oracle), "
+          f"{np.abs(d_gpu).max():.3e} rad (GPU); difference {err:.2e} rad")
+    assert np.abs(d_ref).max() > 5e-6
+    assert err <= 0.05 * np.abs(d_ref).max()
