@@ -297,6 +297,7 @@ struct PotArgs {
     double min_x, max_x, min_y, max_y;  // paddedFullLimits
     int blocks_x, blocks_y, load_x, load_y, load_z, n_slices;
     double cos_phi, sin_phi, sin_2theta, tan_theta;  // beam tilt geometry (projected)
+    int untilted;        // no tilt: the tilted metric reduces to rad_x^2 + rad_y^2 bit for bit
     double sigma;        // sigma_eff
     double dz, z_min;    // full 3D
     double shift_x, shift_y;
@@ -347,6 +348,8 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
     const double im_pos_x = a.startx + xid * a.pixelscale;
     const double im_pos_y = a.starty + yid * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
+    const double r_min2 = r_min * r_min;
+    const float r_minf = (float)r_min;
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int j = startj; j <= endj; ++j) {
@@ -371,16 +374,23 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                 }
                 const double rad_x = im_pos_x - sx[l];
                 const double rad_y = im_pos_y - sy[l];
-                const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
-                const double zt = z_prime / a.tan_theta;
-                const double x_prime = rad_x + zt * a.cos_phi;
-                const double y_prime = rad_y + zt * a.sin_phi;
-                const double r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
-                if (r2 <= 64.0000000001) {
-                    double rad = sqrt(r2);
-                    if (rad < r_min) rad = r_min;
-                    if (rad <= 8.0)
-                        sumz += (double)vproj_tab<FORM>(pt, a.i_count, (float)rad);
+                double r2;
+                if (a.untilted) {
+                    // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad vanish
+                    // below the rounding of x'^2 + y'^2, which equals this exactly
+                    r2 = rad_x * rad_x + rad_y * rad_y;
+                } else {
+                    const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
+                    const double zt = z_prime / a.tan_theta;
+                    const double x_prime = rad_x + zt * a.cos_phi;
+                    const double y_prime = rad_y + zt * a.sin_phi;
+                    r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
+                }
+                // rad = max(sqrt(r2), r_min) <= 8: certain for r2 <= 64; the double square root
+                // decides only the sliver just above 64.  The potential takes rad in float.
+                if (r2 <= 64.0000000001 && (r2 <= 64.0 || sqrt(r2) <= 8.0)) {
+                    const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
+                    sumz += (double)vproj_tab<FORM>(pt, a.i_count, rf);
                 }
             }
         }

@@ -177,6 +177,7 @@ mse::PotArgs pot_args(mse_engine* e, const ConfigData& c) {
     a.sin_phi = std::sin(g.beam_azimuth);
     a.sin_2theta = std::sin(2.0 * beam_theta);
     a.tan_theta = std::tan(beam_theta);
+    a.untilted = g.beam_tilt_mrad == 0.0;
     a.sigma = g.sigma_eff;
     a.dz = g.slice_thickness;
     a.z_min = g.z_lims[0];
@@ -193,6 +194,7 @@ mse::PotArgs pot_args(mse_engine* e, const ConfigData& c) {
         a.sin_phi = std::sin(phi);
         a.sin_2theta = std::sin(2.0 * theta);
         a.tan_theta = std::tan(theta);
+        a.untilted = kx == 0.0 && ky == 0.0;
         a.shift_x = (kx / kz) * g.slice_thickness / g.full3d_integrals;
         a.shift_y = (ky / kz) * g.slice_thickness / g.full3d_integrals;
     }
