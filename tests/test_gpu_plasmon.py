@@ -88,10 +88,13 @@ def two_scatterings(g):
     return ev
 
 
-@pytest.mark.parametrize("params,full3d", [("kirkland", False), ("lobato", True)])
-def test_cbed_two_scatterings_match_oracle(engine, oracle, params, full3d):
+# 384 and 768 points: the column kernels read per-wave propagator rows from memory there
+# (more than 8 elements per lane) instead of holding them in registers
+@pytest.mark.parametrize("params,full3d,res", [("kirkland", False, 256), ("lobato", True, 256),
+                                               ("kirkland", False, 384), ("kirkland", False, 768)])
+def test_cbed_two_scatterings_match_oracle(engine, oracle, params, full3d, res):
     spec = sp.srtio3(3, 3, 4)
-    scen = cbed_scenario(spec, full3d=int(full3d), full3d_integrals=4)
+    scen = cbed_scenario(spec, res=res, full3d=int(full3d), full3d_integrals=4)
     g = cltem_amd.plan_geometry(cltem_amd.make_scenario(**scen))
     p = cltem_amd.load_params(params)
     engine.set_geometry(g)
@@ -109,7 +112,7 @@ def test_cbed_two_scatterings_match_oracle(engine, oracle, params, full3d):
     err = rel_l2(got, ref)
     k = events[7]
     shift = ((k[0] - og0.k_vec[0]) / og0.inverse_scale, (k[1] - og0.k_vec[1]) / og0.inverse_scale)
-    print(f"{params} full3d={full3d}: two scatterings, pattern moved by ({shift[0]:.2f}, {shift[1]:.2f}) px, "
+    print(f"{params} full3d={full3d} {res}^2: two scatterings, pattern moved by ({shift[0]:.2f}, {shift[1]:.2f}) px, "
           f"rel-L2 {err:.3e}")
     assert err <= 1e-4
     # the scatterings are visible: the untilted run differs
