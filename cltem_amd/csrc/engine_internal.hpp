@@ -253,7 +253,7 @@ void sync_wave_prop(mse_engine* e);
 //          9: as 7 with twiddles read at use
 //         10: as 7 with 8-column tiles (512 threads: no register spills at 1024 points)
 constexpr int kRowVariants = 7;
-constexpr int kColVariants = 11;
+constexpr int kColVariants = 12;
 
 // Sweep launchers, one translation unit per resolution (sweeps_<N>.hip).
 template <int N>

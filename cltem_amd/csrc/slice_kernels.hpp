@@ -645,6 +645,95 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
 }
 
 // --------------------------------------------------------------------------------------
+// Column slice step, one 16-column tile per block with no prefetch, sized for two blocks
+// per CU (at most 64 VGPRs): twiddles and y factors are read from per-block copies in LDS,
+// so that the second block's loads and stores overlap the first one's transforms.  Same
+// arithmetic as k_col<..., COL_PROP>.  Grid (N / TC, nsets).
+// --------------------------------------------------------------------------------------
+template <class PL, int TC>
+__global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(8), amdgpu_num_vgpr(64))) void k_col_prop_lean(
+    float2* __restrict__ data, size_t wave_stride, const float2* __restrict__ tw, ColArgs a) {
+    constexpr int N = PL::N, L = PL::L, E = PL::E;
+    constexpr int NT = TC * L;
+    constexpr int CS = LdsLine<N>::col_stride;
+    constexpr int NQ = (TC * N / 2) / NT;
+    static_assert(L == 64 && (TC * N / 2) % NT == 0, "one column per wave");
+    // twiddle indices r k N / (p R) of passes 2 and 3 stay below NTW
+    constexpr int NTW = (PL::R1 - 1) * (PL::R0 - 1) * (N / (PL::R0 * PL::R1)) + 1 >
+                                (PL::R2 - 1) * (PL::R0 * PL::R1 - 1) * (N / (PL::R0 * PL::R1 * PL::R2)) + 1
+                            ? (PL::R1 - 1) * (PL::R0 - 1) * (N / (PL::R0 * PL::R1)) + 1
+                            : (PL::R2 - 1) * (PL::R0 * PL::R1 - 1) * (N / (PL::R0 * PL::R1 * PL::R2)) + 1;
+    static_assert(PL::R3 == 1, "three-pass plans");
+    __shared__ float2 tile[TC * CS];
+    __shared__ float2 stw[NTW];
+    __shared__ float2 spy[N];
+    const int tid = threadIdx.x;
+    const int c = tid / L;
+    const int t = tid % L;
+    const int wv = a.wave_map ? cload(a.wave_map + blockIdx.y) : (int)blockIdx.y;
+    const int x0 = blockIdx.x * TC;
+    const int x = x0 + c;
+    const float4* in4 = reinterpret_cast<const float4*>(data + (size_t)wv * wave_stride + x0);
+    float4 q[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int idx = tid + k * NT;
+        const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+        q[k] = in4[((size_t)r * N) / 2 + cc / 2];
+    }
+    const float2* pxt = a.px;
+    const float2* pyt = a.py;
+    if (a.wave_prop) {
+        pxt = cload(a.wave_prop + 2 * wv);
+        pyt = cload(a.wave_prop + 2 * wv + 1);
+    }
+    for (int i = tid; i < N; i += NT) {
+        if (i < NTW) stw[i] = tw[i];
+        spy[i] = pyt[i];
+    }
+    const int xs = (x < N / 2) ? x : N - x;
+    const float2 pxv = cload(pxt + __builtin_amdgcn_readfirstlane(x));
+    const int lim = cload(a.lim + __builtin_amdgcn_readfirstlane(xs));  // the mask is symmetric in kx and ky
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int idx = tid + k * NT;
+        const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+        tile[cc * CS + lpad(r)] = make_float2(q[k].x, q[k].y);
+        tile[(cc + 1) * CS + lpad(r)] = make_float2(q[k].z, q[k].w);
+    }
+    __syncthreads();
+    PlanTw<PL, false> tws;
+    tws.load(stw, t);
+    const WaveSync sync{};
+    float2* ln = tile + c * CS;
+    float2 v[E];
+    line_gather<N, L>(v, ln, t);
+    sync();
+    fft_line<PL, false>(v, ln, t, tws, sync);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int y = t + L * m;
+        const int ys = (y < N / 2) ? y : N - y;
+        const float2 f = cmul(v[m], cmul(pxv, spy[y]));
+        v[m] = ys <= lim ? f : make_float2(0.f, 0.f);
+    }
+    sync();
+    fft_line<PL, true>(v, ln, t, tws, sync);
+    sync();
+#pragma unroll
+    for (int m = 0; m < E; ++m) ln[lpad(t + L * m)] = v[m];
+    __syncthreads();
+    float4* out4 = reinterpret_cast<float4*>(data + (size_t)wv * wave_stride + x0);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int idx = tid + k * NT;
+        const int r = idx / (TC / 2), cc = 2 * (idx % (TC / 2));
+        const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
+        out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
+    }
+}
+
+// --------------------------------------------------------------------------------------
 // Persistent, software-pipelined row slice step (ROW_IFFT_T_FFT).  Every line walks the
 // (wave, row) items independently; the next item's wave row and transmission row are
 // loaded into registers while the current row is transformed.

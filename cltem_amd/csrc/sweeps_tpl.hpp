@@ -186,6 +186,16 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
             });
             return tiles;
         }
+        if (variant == 11 && mode == mse::COL_PROP && in == out) {  // two lean blocks per CU
+            if constexpr (P0::L == 64 && P0::R3 == 1 && N <= 512) {
+                dim3 grid(N / 16, nsets);
+                tiles = N / 16;
+                launch(e, kind, [&]() {
+                    mse::k_col_prop_lean<P0, 16><<<grid, 16 * P0::L, 0, e->stream>>>(out, wave_stride, e->tw.p, args);
+                });
+                return tiles;
+            }
+        }
         if (variant >= 7 && mode == mse::COL_PROP && in == out) {
             constexpr int NT = 16 * P0::L;
             const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, 16, mse::WaveSync>>(NT, 1024 / NT);

@@ -259,10 +259,11 @@ int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes
  *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at
  *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes,
  *      6 as 4 with twiddles read at use.
- *      kernel 1 = column pass (variants 0..10): 0 16-column tiles, 1 half lanes, 2 8-column
+ *      kernel 1 = column pass (variants 0..11): 0 16-column tiles, 1 half lanes, 2 8-column
  *      tiles, 3 block barriers, 4/5 twiddles read at use (16/8 columns), 6 32-column tiles,
  *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes, 9 as 7
- *      with twiddles read at use, 10 as 7 with 8-column tiles.
+ *      with twiddles read at use, 10 as 7 with 8-column tiles, 11 one tile per block at two
+ *      blocks per CU, twiddles and propagator rows from LDS (512 points; as 7 elsewhere).
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
  *      (reps launches each; times in microseconds per launch), selects the fastest and
  *      leaves the waves' contents undefined. ---- */
