@@ -25,6 +25,9 @@ batched through one graph-captured slice loop): value is configuration.slices/s.
 --workload mixed measures the mixed-size batched CBED case (48 SrTiO3 instances on
 256^2..768^2 grids, 5 configurations each, one engine per size running concurrently):
 value is wave.slices/s over the concurrent timed window.
+--workload adjoint measures position gradients of the Au/C adjoint case (2000 atoms,
+256^2, 46 slices): one step rebuilds the transmission functions from the positions and
+runs the checkpointed slice loop and the adjoint slice loop; value is gradients/s.
 """
 from __future__ import annotations
 
@@ -291,9 +294,78 @@ def bench_mixed(args):
     return 0
 
 
+def bench_adjoint(args):
+    """Position gradients of the Au/C adjoint case (2000 atoms, 256^2, 46 slices, projected
+    Kirkland, L = sum (I - I_target)^2): one step rebuilds the transmission functions from
+    the positions (binning + potentials), runs the slice loop with sqrt(N) checkpoints and
+    the adjoint slice loop, as one iteration of a position fit would."""
+    w = wl.adjoint_auc()
+    spec, tgt = w["specimen"], w["target"]
+    engine = cltem_amd.Engine(0)
+    dev, cus, _ = engine.device_info()
+    g, p = setup(engine, w)
+    ns, n_atoms = g.n_slices, len(spec)
+    log(f"{dev}: {n_atoms} atoms, {g.resolution}^2, {ns} slices, adjoint position gradients")
+    build_config(engine, 0, tgt)
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    engine.init_plane_wave(1.0)
+    target = engine.forward_checkpointed(0)[0]
+
+    def forward():
+        build_config(engine, 0, spec)
+        engine.init_plane_wave(1.0)
+        return engine.forward_checkpointed(0)[0]
+
+    def gradient():
+        intensity = forward()
+        return engine.adjoint_positions((2.0 * (intensity - target))[None], n_atoms)
+
+    for _ in range(args.warmup):
+        grad = gradient()
+    engine.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grad = gradient()
+    engine.synchronize()
+    t_grad = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        forward()
+    engine.synchronize()
+    t_fwd = (time.perf_counter() - t0) / args.steps
+    value = 1.0 / t_grad
+    log(f"gradient {1e3 * t_grad:.2f} ms (forward alone {1e3 * t_fwd:.2f} ms) -> {value:.2f} gradients/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle_py as orc
+        og = orc.geometry(orc.scenario(**w["scenario"]))
+        t0 = time.perf_counter()
+        orc.ctem_intensity(og, spec.x, spec.y, spec.z, spec.Z, p.values, p.form, p.i_per_atom, nthreads=16)
+        dt = time.perf_counter() - t0
+        n_fd = 2 * 3 * n_atoms
+        cpu = {"value": 1.0 / (n_fd * dt), "unit": "gradients/s", "cores": 16, "kind": "port",
+               "sample": f"one FP64 forward of the oracle (binning, potentials, {ns} slices) in {dt:.2f} s "
+                         f"on 16 threads, scaled to the {n_fd} forwards of a central-difference gradient"}
+    engine.close()
+    line = {
+        "metric": "adjoint position gradients/s (2000 Au/C atoms, 256², 46 slices, sqrt(N) checkpoints)",
+        "value": round(value, 3), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_grad, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Au/C 1:3 by seeded random sequential addition, target displaced by 0.05 A)",
+        "config": {"workload": f"CTEM position gradients, {n_atoms} atoms in 40^3 A, {g.resolution}^2, {ns} slices, "
+                               f"200 kV, projected Kirkland, L = sum (I - I_target)^2"},
+        "forward_ms": round(1e3 * t_fwd, 3), "gradient_over_forward": round(t_grad / t_fwd, 2),
+        "grad_norm": float(np.linalg.norm(grad)), "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("stem", "sio2", "mixed"), default="stem")
+    ap.add_argument("--workload", choices=("stem", "sio2", "mixed", "adjoint"), default="stem")
     ap.add_argument("--configs", type=int, default=30, help="sio2: frozen-phonon configurations")
     ap.add_argument("--cpu-slices", type=int, default=24, help="sio2: slices in the CPU baseline sample")
     ap.add_argument("--mixed-sequential", action="store_true",
@@ -324,6 +396,8 @@ def main():
         return bench_sio2(args)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     w = wl.stem_srtio3()
     spec = w["specimen"]
