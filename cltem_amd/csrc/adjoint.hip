@@ -34,16 +34,17 @@ void copy_planes(mse_engine* e, float2* dst, const float2* src, size_t count) {
     HIPX(hipMemcpyAsync(dst, src, count * sizeof(float2), hipMemcpyDeviceToDevice, e->stream));
 }
 
-// BL(x) = F2^-1 M F2 x on one plane, in place (the passes of mse_build_transmission)
-void band_limit_plane(mse_engine* e, float2* p) {
+// BL(x) = F2^-1 M F2 x on `count` consecutive planes, in place (the passes of
+// mse_build_transmission)
+void band_limit_planes(mse_engine* e, float2* p, int count) {
     mse::RowArgs ra = base_row_args(e);
     ra.scale = 1.0f;
-    row_pass(e, mse::ROW_FFT, p, p, 1, e->N, ra);
+    row_pass(e, mse::ROW_FFT, p, p, count, e->N, ra);
     mse::ColArgs ca = base_col_args(e);
     ca.lim = e->band_lim.p;
     ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
-    col_pass(e, mse::COL_BAND, p, p, 1, e->N, ca);
-    row_pass(e, mse::ROW_IFFT, p, p, 1, e->N, ra);
+    col_pass(e, mse::COL_BAND, p, p, count, e->N, ca);
+    row_pass(e, mse::ROW_IFFT, p, p, count, e->N, ra);
 }
 
 }  // namespace
@@ -138,7 +139,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         HIPX(hipMemcpyAsync(bs.data(), c.block_start.p, bs.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
 
         e->adj_g.ensure(plane);
-        e->adj_gt.ensure(plane + N);  // per-wave g_t, then their sum
+        e->adj_gt.ensure(plane);  // per-wave g_t
         e->adj_seg.ensure((size_t)K * plane);
         HIPX(hipStreamSynchronize(e->stream));
 
@@ -148,7 +149,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         row_pass(e, mse::ROW_ADJ_SEED, e->adj_exit.p, e->adj_g.p, nw, N, ra);
 
         mse::ColArgs ca = base_col_args(e);
-        float2* gsum = e->adj_gt.p + plane;
+        e->adj_gtau.ensure((size_t)std::max(n, 1) * N);  // g_t of every slice, summed over the waves
         const int n_seg = (n + K - 1) / K;
         for (int j = n_seg - 1; j >= 0; --j) {
             const int start = j * K, len = std::min(K, n - start);
@@ -166,36 +167,40 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
                 rb.aux_in = e->adj_seg.p + (size_t)(s - start) * plane;
                 rb.aux_out = e->adj_gt.p;
                 row_pass(e, mse::ROW_ADJ, e->adj_g.p, e->adj_g.p, nw, N, rb);
-                // atoms whose potentials reach slice s: its own, or s +- load_z in full 3D
-                const int lz = g.full3d ? pa.load_z : 0;
-                const int k0 = std::max(0, s - lz), k1 = std::min(n - 1, s + lz);
-                const int a0 = bs[(size_t)k0 * nblk], a1 = bs[(size_t)(k1 + 1) * nblk];
-                if (a1 <= a0) continue;  // no atoms: phi_s does not depend on positions
+                float2* gs = e->adj_gtau.p + (size_t)s * N;
                 launch(e, PK_OTHER, [&]() {
-                    mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gsum);
+                    mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gs);
                 });
-                band_limit_plane(e, gsum);
-                const float* ph = e->adj_phase.p + (size_t)s * N;
-                if (g.full3d) {
-                    dim3 tiles(e->res / 16, e->res / 16);
-                    launch(e, PK_OTHER, [&]() {
-                        if (e->form == 0)
-                            mse::k_atom_grad_full3d<0><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
-                        else if (e->form == 1)
-                            mse::k_atom_grad_full3d<1><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
-                        else
-                            mse::k_atom_grad_full3d<2><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, s, gsum, ph, e->adj_grad.p);
-                    });
-                    continue;
-                }
-                dim3 grid(a1 - a0);
+            }
+        }
+        // the position terms of every slice in one pass: g_tau = BL(g_t) of all slices, then
+        // one gradient launch over all atoms (projected: each atom reads its own slice's
+        // planes) or over all slices (full 3D: each slice sums the atoms of s +- load_z)
+        const int a_first = bs[0], a_end = bs[(size_t)n * nblk];
+        if (n > 0 && a_end > a_first) {
+            band_limit_planes(e, e->adj_gtau.p, n);
+            const float2* gt = e->adj_gtau.p;
+            const float* ph = e->adj_phase.p;
+            if (g.full3d) {
+                dim3 tiles(e->res / 16, e->res / 16, n);
                 launch(e, PK_OTHER, [&]() {
                     if (e->form == 0)
-                        mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                        mse::k_atom_grad_full3d<0><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, 0, gt, ph, e->adj_grad.p);
                     else if (e->form == 1)
-                        mse::k_atom_grad_projected<1><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                        mse::k_atom_grad_full3d<1><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, 0, gt, ph, e->adj_grad.p);
                     else
-                        mse::k_atom_grad_projected<2><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a0, e->params.p, pa, gsum, ph, e->adj_grad.p);
+                        mse::k_atom_grad_full3d<2><<<tiles, 256, 0, e->stream>>>(c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, c.orig.p, e->params3d.p, pa, 0, gt, ph, e->adj_grad.p);
+                });
+            } else {
+                dim3 grid(a_end - a_first);
+                const int* bsp = c.block_start.p;
+                launch(e, PK_OTHER, [&]() {
+                    if (e->form == 0)
+                        mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
+                    else if (e->form == 1)
+                        mse::k_atom_grad_projected<1><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
+                    else
+                        mse::k_atom_grad_projected<2><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
                 });
             }
         }

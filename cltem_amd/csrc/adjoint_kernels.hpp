@@ -78,19 +78,34 @@ static __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-// One block of 256 threads per atom of slice s (sorted range [first, first + gridDim.x)).
+// One block of 256 threads per sorted atom of slices 0..n_slices-1 (sorted range
+// [first, first + gridDim.x)); its slice s is found from the slice starts bs[k * nblk].
 // The block visits the pixels of the atom's 8 A neighbourhood, keeps those whose tile block
 // range contains the atom's block (as the forward sum does), and reduces
-//   -sigma * dL/dphi(p) * v'(r) * dr/d(rad) in double.
-// grad has 3 doubles (x, y, z) per input atom, indexed through orig.
+//   -sigma * dL/dphi(p) * v'(r) * dr/d(rad) in double, with dL/dphi from slice s's planes
+// g_tau + s N and phase + s N.  grad has 3 doubles (x, y, z) per input atom, indexed
+// through orig.
 template <int FORM>
 __global__ __launch_bounds__(256) void k_atom_grad_projected(
     const double* __restrict__ ax, const double* __restrict__ ay, const int* __restrict__ aZ,
-    const int* __restrict__ orig, int first, const float* __restrict__ params, PotArgs a,
-    const float2* __restrict__ g_tau, const float* __restrict__ phase, double* __restrict__ grad) {
+    const int* __restrict__ orig, int first, const int* __restrict__ bs, int nblk, int n_slices,
+    const float* __restrict__ params, PotArgs a, const float2* __restrict__ g_tau_all,
+    const float* __restrict__ phase_all, double* __restrict__ grad) {
 #pragma clang fp contract(off)
     __shared__ double red[2][4];
     const int i = first + blockIdx.x;
+    // the slice holding sorted atom i: the last k with bs[k * nblk] <= i
+    int lo = 0, hi = n_slices - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bs[(size_t)mid * nblk] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const size_t plane = (size_t)a.res * a.res;
+    const float2* __restrict__ g_tau = g_tau_all + (size_t)lo * plane;
+    const float* __restrict__ phase = phase_all + (size_t)lo * plane;
     const double xa = ax[i], ya = ay[i];
     const float* pz = params + (size_t)(aZ[i] - 1) * a.per_atom;
     // the atom's xy block, as k_bin_atoms computes it
@@ -202,7 +217,7 @@ __device__ __forceinline__ void add_grad(double* __restrict__ grad, const int* _
     unsafeAtomicAdd(o + 2, gz);
 }
 
-// Position gradients of the full-3D phase of slice `slice`: the same 16x16 pixel tiles, block
+// Position gradients of the full-3D phase of slices first + blockIdx.z: the same 16x16 pixel tiles, block
 // rows, atoms and integration points as full3d_sumz, in the same order.  With p1_e the
 // potential at evaluation e of a (k, j) row sequence, the trapezoid gives
 //   sumz = sum_e 0.5 [h_e != 0] (p1_e + p1_prev(e)),
@@ -216,9 +231,13 @@ template <int FORM>
 __global__ __launch_bounds__(256) void k_atom_grad_full3d(
     const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
     const int* __restrict__ aZ, const int* __restrict__ bsp, const int* __restrict__ orig,
-    const float* __restrict__ tab3d, PotArgs a, int slice, const float2* __restrict__ g_tau,
-    const float* __restrict__ phase, double* __restrict__ grad) {
+    const float* __restrict__ tab3d, PotArgs a, int first, const float2* __restrict__ g_tau_all,
+    const float* __restrict__ phase_all, double* __restrict__ grad) {
 #pragma clang fp contract(off)
+    // slice first + blockIdx.z and its planes
+    const int slice = first + blockIdx.z;
+    const float2* __restrict__ g_tau = g_tau_all + (size_t)slice * a.res * a.res;
+    const float* __restrict__ phase = phase_all + (size_t)slice * a.res * a.res;
     __shared__ double sx[256], sy[256], sz[256];
     __shared__ int sZ[256];
     const int lid = threadIdx.x;
