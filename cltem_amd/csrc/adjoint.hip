@@ -162,15 +162,16 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
             }
             for (int s = start + len - 1; s >= start; --s) {
                 col_pass(e, mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
+                float2* gs = e->adj_gtau.p + (size_t)s * N;
                 mse::RowArgs rb = base_row_args(e);
                 rb.slice = s;
                 rb.aux_in = e->adj_seg.p + (size_t)(s - start) * plane;
-                rb.aux_out = e->adj_gt.p;
+                rb.aux_out = nw == 1 ? gs : e->adj_gt.p;  // one wave: its g_t is the sum
                 row_pass(e, mse::ROW_ADJ, e->adj_g.p, e->adj_g.p, nw, N, rb);
-                float2* gs = e->adj_gtau.p + (size_t)s * N;
-                launch(e, PK_OTHER, [&]() {
-                    mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gs);
-                });
+                if (nw > 1)
+                    launch(e, PK_OTHER, [&]() {
+                        mse::k_sum_planes<<<(unsigned)((N + 255) / 256), 256, 0, e->stream>>>(e->adj_gt.p, nw, N, N, gs);
+                    });
             }
         }
         // the position terms of every slice in one pass: g_tau = BL(g_t) of all slices, then
