@@ -292,7 +292,9 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
  * (kernels/transmission_potentials_full_3d_f.cl:229-278), including the value carried
  * between atoms of one block row.  Gradients are undefined where a coordinate sits exactly
  * on a cutoff of the forward sum (slice and block edges, the 8 A lateral and 3 A depth
- * limits); MSE_ERR_UNSUPPORTED under mixed static potentials. */
+ * limits); MSE_ERR_UNSUPPORTED under mixed static potentials.  Device memory: the
+ * checkpoints (ceil(n / K) * n_waves planes), one segment (K * n_waves planes), and the
+ * wave-summed g_t and the phase of every slice (n complex and n float planes). */
 int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz);
 /* ---- sweep timing: `reps` back-to-back row passes (slice `slice`, every wave with its own
  *      configuration, current variant) and then `reps` column passes, timed with hipEvents on
