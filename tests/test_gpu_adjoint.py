@@ -214,3 +214,27 @@ def test_adjoint_full3d_matches_fd(engine, oracle, params, tilt):
     rel_norm, err, scale = compare(got, fd_r, f"full 3D {params} tilt {tilt}")
     assert rel_norm <= 1e-3
     assert np.all(err <= 1e-3 * np.abs(fd_r) + 1e-3 * scale)
+
+
+@pytest.mark.parametrize("full3d", [False, True])
+def test_adjoint_without_atoms_in_range(engine, full3d):
+    """Atoms outside the padded limits are filtered out of every slice: the exit intensity is
+    that of vacuum and every gradient is exactly zero."""
+    scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.0, struct_min=(0.0, 0.0, 0.0),
+                struct_max=(10.0, 10.0, 6.0), area=(0.0, 10.0, 0.0, 10.0), full3d=int(full3d),
+                full3d_integrals=4)
+    g, p = setup(engine, scen, "kirkland")
+    x = np.array([500.0, -400.0, 5.0])
+    y = np.array([5.0, 5.0, 700.0])
+    z = np.array([3.0, 2.0, 1.0])
+    Z = np.array([79, 6, 14], dtype=np.int32)
+    r = engine.sort_atoms(0, x, y, z, Z)
+    assert r["n_in_range"] == 0
+    engine.build_transmission(0)
+    engine.set_wave_count(1)
+    engine.set_wave_configs([0])
+    engine.init_plane_wave(1.0)
+    I = engine.forward_checkpointed(0)
+    np.testing.assert_allclose(I[0], 1.0, atol=1e-5)
+    grad = engine.adjoint_positions(np.ones_like(I), len(Z))
+    assert grad.shape == (3, 3) and np.all(grad == 0.0)
