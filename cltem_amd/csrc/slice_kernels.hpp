@@ -650,20 +650,30 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
 // so that the second block's loads and stores overlap the first one's transforms.  Same
 // arithmetic as k_col<..., COL_PROP>.  Grid (N / TC, nsets).
 // --------------------------------------------------------------------------------------
+// Twiddle table entries a plan's passes read: pass with radix R after p points reads
+// r k N / (p R) for r < R, k < p.
+template <class PL>
+constexpr int plan_twiddles() {
+    constexpr int N = PL::N;
+    const int R[4] = {PL::R0, PL::R1, PL::R2, PL::R3};
+    int p = R[0], most = 0;
+    for (int i = 1; i < 4 && R[i] > 1; ++i) {
+        const int top = (R[i] - 1) * (p - 1) * (N / (p * R[i]));
+        most = top > most ? top : most;
+        p *= R[i];
+    }
+    return most + 1;
+}
+
 template <class PL, int TC>
-__global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(8), amdgpu_num_vgpr(64))) void k_col_prop_lean(
-    float2* __restrict__ data, size_t wave_stride, const float2* __restrict__ tw, ColArgs a) {
+__device__ __forceinline__ void col_prop_lean(float2* __restrict__ data, size_t wave_stride,
+                                              const float2* __restrict__ tw, const ColArgs& a) {
     constexpr int N = PL::N, L = PL::L, E = PL::E;
     constexpr int NT = TC * L;
     constexpr int CS = LdsLine<N>::col_stride;
     constexpr int NQ = (TC * N / 2) / NT;
     static_assert(L == 64 && (TC * N / 2) % NT == 0, "one column per wave");
-    // twiddle indices r k N / (p R) of passes 2 and 3 stay below NTW
-    constexpr int NTW = (PL::R1 - 1) * (PL::R0 - 1) * (N / (PL::R0 * PL::R1)) + 1 >
-                                (PL::R2 - 1) * (PL::R0 * PL::R1 - 1) * (N / (PL::R0 * PL::R1 * PL::R2)) + 1
-                            ? (PL::R1 - 1) * (PL::R0 - 1) * (N / (PL::R0 * PL::R1)) + 1
-                            : (PL::R2 - 1) * (PL::R0 * PL::R1 - 1) * (N / (PL::R0 * PL::R1 * PL::R2)) + 1;
-    static_assert(PL::R3 == 1, "three-pass plans");
+    constexpr int NTW = plan_twiddles<PL>();
     __shared__ float2 tile[TC * CS];
     __shared__ float2 stw[NTW];
     __shared__ float2 spy[N];
@@ -731,6 +741,19 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(8), 
         const float2 a0 = tile[cc * CS + lpad(r)], a1 = tile[(cc + 1) * CS + lpad(r)];
         out4[((size_t)r * N) / 2 + cc / 2] = make_float4(a0.x, a0.y, a1.x, a1.y);
     }
+}
+
+// 16-column tiles (512 points): 64 VGPRs, two 1024-thread blocks per CU
+template <class PL>
+__global__ __launch_bounds__(16 * PL::L) __attribute__((amdgpu_waves_per_eu(8), amdgpu_num_vgpr(64))) void k_col_prop_lean(
+    float2* __restrict__ data, size_t wave_stride, const float2* __restrict__ tw, ColArgs a) {
+    col_prop_lean<PL, 16>(data, wave_stride, tw, a);
+}
+// 8-column tiles (four-pass plans, 12 elements per lane): two 512-thread blocks per CU
+template <class PL>
+__global__ __launch_bounds__(8 * PL::L) __attribute__((amdgpu_waves_per_eu(4))) void k_col_prop_lean8(
+    float2* __restrict__ data, size_t wave_stride, const float2* __restrict__ tw, ColArgs a) {
+    col_prop_lean<PL, 8>(data, wave_stride, tw, a);
 }
 
 // --------------------------------------------------------------------------------------

@@ -186,12 +186,19 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
             });
             return tiles;
         }
-        if (variant == 11 && mode == mse::COL_PROP && in == out) {  // two lean blocks per CU
-            if constexpr (P0::L == 64 && P0::R3 == 1 && N <= 512) {
+        if (variant == 11 && mode == mse::COL_PROP && in == out) {  // lean: two blocks per CU
+            if constexpr (P0::L == 64 && N <= 512) {  // 16-column tiles, 64 VGPRs
                 dim3 grid(N / 16, nsets);
                 tiles = N / 16;
                 launch(e, kind, [&]() {
-                    mse::k_col_prop_lean<P0, 16><<<grid, 16 * P0::L, 0, e->stream>>>(out, wave_stride, e->tw.p, args);
+                    mse::k_col_prop_lean<P0><<<grid, 16 * P0::L, 0, e->stream>>>(out, wave_stride, e->tw.p, args);
+                });
+                return tiles;
+            } else if constexpr (P0::L == 64 && P0::R3 > 1) {  // four-pass plans: 8-column tiles, 128 VGPRs
+                dim3 grid(N / 8, nsets);
+                tiles = N / 8;
+                launch(e, kind, [&]() {
+                    mse::k_col_prop_lean8<P0><<<grid, 8 * P0::L, 0, e->stream>>>(out, wave_stride, e->tw.p, args);
                 });
                 return tiles;
             }

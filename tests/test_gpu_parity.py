@@ -129,7 +129,7 @@ def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
     engine.set_kernel_variant(1, 7)
 
 
-@pytest.mark.parametrize("res", [512, 384, 1024])
+@pytest.mark.parametrize("res", [512, 384, 768, 1024])
 def test_column_variants_agree(engine, oracle, res):
     """The column kernels (persistent: prefetching, half lanes, 8-column tiles; lean: two
     blocks per CU, 512 points) do the same butterflies with the same twiddles as the default; only the compiler's fusing of
