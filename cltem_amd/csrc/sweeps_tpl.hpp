@@ -231,6 +231,8 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
             else
                 go(P0{}, C16{}, mse::WaveSync{}, REG{});
         }
+        else if (!hot && P0::L == 64 && P0::E > 8)  // 16-column tiles of 1024 threads spill here
+            go(P0{}, C8{}, mse::WaveSync{}, REG{});
         else
             go(P0{}, C16{}, mse::WaveSync{}, REG{});
     }
