@@ -1236,7 +1236,6 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
         REQUIRE(e && pos_xy && probe && dets && accum, MSE_ERR_ARG, "NULL argument");
         require_geom(e);
         REQUIRE(n_probes >= 1 && batch >= 1 && n_det >= 1, MSE_ERR_ARG, "counts must be positive");
-        (void)use_graph;
         ConfigData& c = config(e, cfg);
         REQUIRE(c.trans.p && c.built && c.trans.n >= (size_t)cfg_geom(e, c).n_slices * e->N, MSE_ERR_STATE,
                 "transmission functions of this configuration are not built for the current geometry");
@@ -1258,6 +1257,9 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
                                 e->stream));
         }
         sync_wave_prop(e);
+        // the probes leave the specimen at the configuration's own exit surface (a thinner
+        // specimen than the engine geometry's; mixed static potentials use the engine's slicing)
+        const int ns = e->mix_sets >= 2 ? e->g.n_slices : cfg_geom(e, c).n_slices;
         ensure_detectors(e, dets, n_det);
         e->partials.ensure((size_t)batch * n_det * (e->res / 8));
         e->det_out.ensure((size_t)n_probes * n_det);
@@ -1280,7 +1282,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.sqrt2 = sqrt2;
             k.row_var = e->row_var;
             k.col_var = e->col_var;
-            k.n_slices = e->g.n_slices;
+            k.n_slices = ns;
             k.waves = e->waves.p;
             k.partials = e->partials.p;
             k.det_out = e->det_out.p;
@@ -1299,10 +1301,6 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.mix_sets = e->mix_sets;
             k.mix_seed = e->mix_seed;
             k.mix_key = e->mix_key.p;
-        k.mix_first = e->mix_first;
-        k.mix_sets = e->mix_sets;
-        k.mix_seed = e->mix_seed;
-        k.mix_key = e->mix_key.p;
             k.lim = e->prop_lim.p;
             k.kx = e->kx.p;
             k.ky = e->ky.p;
@@ -1320,9 +1318,15 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
                 destroy_stem_graph(e);
                 hipGraph_t gr = nullptr;
                 HIPX(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-                probes_into_waves(e, batch, probe);
-                multislice(e, batch, 0, e->g.n_slices, true, &da);
-                reduce_detectors(e, batch, n_det, sqrt2, e->det_out.p, 1, e->dev_offset.p);
+                try {
+                    probes_into_waves(e, batch, probe);
+                    multislice(e, batch, 0, ns, true, &da);
+                    reduce_detectors(e, batch, n_det, sqrt2, e->det_out.p, 1, e->dev_offset.p);
+                } catch (...) {  // leave the stream out of capture mode for later calls
+                    (void)hipStreamEndCapture(e->stream, &gr);
+                    if (gr) (void)hipGraphDestroy(gr);
+                    throw;
+                }
                 HIPX(hipStreamEndCapture(e->stream, &gr));
                 hipError_t st = hipGraphInstantiate(&e->stem_exec, gr, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(gr);
@@ -1345,7 +1349,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
                 HIPX(hipGraphLaunch(e->stem_exec, e->stream));
             } else {
                 probes_into_waves(e, nb, probe);
-                multislice(e, nb, 0, e->g.n_slices, true, &da);
+                multislice(e, nb, 0, ns, true, &da);
                 reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
             }
         }
@@ -1614,6 +1618,7 @@ int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, 
         e->h_wave_cfg.assign(e->n_waves, cfg);
         HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), e->n_waves * sizeof(int), hipMemcpyHostToDevice,
                             e->stream));
+        sync_wave_prop(e);  // the per-wave propagator table must cover the new wave set
         const size_t total = e->N * n_waves;
         k_fill_plane<<<(unsigned)((total + 255) / 256), 256, 0, e->stream>>>(e->waves.p, e->res, e->N, n_waves, 1.0f);
         HIPX(hipGetLastError());
@@ -1712,6 +1717,7 @@ int mse_bandwidth_probe(mse_engine* e, int pattern, int variant, int n_waves, in
             HIPX(hipMemcpyAsync(e->wave_cfg.p, e->h_wave_cfg.data(), n_waves * sizeof(int), hipMemcpyHostToDevice,
                                 e->stream));
             e->n_waves = n_waves;
+            sync_wave_prop(e);
         }
         hipEvent_t a, b;
         HIPX(hipEventCreate(&a));

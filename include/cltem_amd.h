@@ -264,7 +264,8 @@ int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes
  *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes, 9 as 7
  *      with twiddles read at use, 10 as 7 with 8-column tiles, 11 one tile per block at two
  *      blocks per CU, twiddles and propagator rows from LDS (16-column tiles at 512 points,
- *      8-column tiles for the four-pass plans of 384 and 768; as 7 elsewhere).
+ *      8-column tiles for the four-pass 768-point plan; every other size, 384 included,
+ *      runs variant 7).
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
  *      (reps launches each; times in microseconds per launch), selects the fastest and
  *      leaves the waves' contents undefined. ---- */

@@ -16,10 +16,16 @@
  *   orc_sort_atoms .................. brute-force tiny cases, stability, edge clamps
  *   orc_bessk0/orc_bessk1 ........... scipy.special.k0/k1
  *   orc_potential_* ................. scipy radial tables, superposition, cutoffs
+ *   orc_potential_slice (full 3D) ... sum over slices = scipy quad of V3D over the signed
+ *                                     3 A window + Euler-Maclaurin term (1e-9); the value
+ *                                     carried between atoms of a block row and the
+ *                                     cumulative tilt shift by explicit point-value sums
  *   orc_fft2 ........................ numpy.fft.fft2(norm="ortho") golden vectors
  *   orc_propagator / band limit ..... exact masks and exp(-i pi lambda k^2 dz)
  *   orc_probe ....................... numpy chi(), aperture-disc pixel count
  *   orc_run_* ....................... empty specimen identities, numpy multislice
+ *   orc_stem_pixel .................. centred and off-centre annuli on an empty specimen =
+ *                                     sqrt(2) x a pixel set counted in numpy
  *   orc_ctem_image .................. aberration-free wide aperture = |psi|^2; two-beam
  *                                     closed form with defocus, C30, both envelopes, edge
  *   orc_ctem_intensity .............. orc_run_plane over orc_transmission of the sorted atoms

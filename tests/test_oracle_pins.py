@@ -400,3 +400,203 @@ def test_multislice_vs_numpy_restatement(oracle):
         ref = np.fft.ifft2(rec, norm="ortho")
     np.testing.assert_allclose(psi, ref, atol=1e-12)
     np.testing.assert_allclose(recip, rec, atol=1e-12)
+
+
+# ---- full-3D slice sums (kernels/transmission_potentials_full_3d_d.cl:239-279) ----------
+def _toy3d(oracle, dz=1.0, integrals=20, tilt_mrad=0.0, azimuth=0.0):
+    """64^2 toy field [0, 16]^2 (0.25 A pixels, two 8 A blocks, load_x 1), z in [0, 10] with
+    no padding: slices of dz = 1 A with tops at z = 10 - s, full 3D with `integrals` points."""
+    sc = dict(mode=1, resolution=64, slice_thickness=dz, struct_min=(0, 0, 0), struct_max=(16, 16, 10),
+              area=(0, 16, 0, 16), pad_xy=(0.0, 0.0), pad_z=(0.0, 0.0), full3d=1,
+              full3d_integrals=integrals, beam_tilt_mrad=tilt_mrad, beam_azimuth=azimuth)
+    g = oracle.geometry(oracle.scenario(**sc))
+    assert (g.z_lims[0], g.z_lims[1], g.n_slices, g.load_z) == (0.0, 10.0, int(round(10 / dz)), 3)
+    return g
+
+
+def _v3d_fn(oracle, name, Z):
+    import ctypes as C
+    p = P.load(name)
+    vp = np.ascontiguousarray(p.values)
+    ptr = vp.ctypes.data_as(C.POINTER(C.c_double))
+    L = oracle.lib()
+    return p, (lambda r: L.orc_v3d(p.form, ptr, p.i_per_atom, Z, float(r))), vp
+
+
+@pytest.mark.parametrize("name,Z", [("kirkland", 38), ("peng", 22), ("lobato", 8)])
+@pytest.mark.parametrize("z0", [4.0, 5.0, 6.5])
+def test_full3d_slice_sum_integrates_isolated_atom(oracle, name, Z, z0):
+    """For an isolated atom whose 3 A window edge z0 + 3 falls on a slice top (z0 = 4, 5: an
+    atom on a slice boundary at 5), summing sst * sumz over every slice gives the trapezoid
+    rule, step dz/integrals, of V3D(sqrt(rho^2 + (z - z0)^2)) over the admitted range
+    [bottom of slice zid + load_z, z0 + 3]: signed window (points below the atom are kept,
+    points more than 3 A above are not), each slice's own trapezoid with no double-counted
+    boundary point.  Checked against scipy quad plus the Euler-Maclaurin term
+    h^2/12 (f'(b) - f'(a)), to O(h^4).  z0 = 6.5 puts the edge inside a slice: the admitted
+    points then start at the first grid point at or below z0 + 3 and the first one carries an
+    extra half step (its stale p2 is 0), i.e. the range ends half a step above it."""
+    g = _toy3d(oracle)
+    p, f3, _ = _v3d_fn(oracle, name, Z)
+    x0, y0 = 6.1, 7.3
+    r = oracle.sort_atoms(g, [x0], [y0], [z0], [Z])
+    zid = int(r["z_ids"][0])
+    assert zid == int(np.floor(10.0 - z0))
+    h = g.slice_thickness / g.full3d_integrals
+    total = np.zeros((64, 64))
+    for s in range(g.n_slices):
+        total += h * oracle.potential_slice(g, r, p.values, p.form, p.i_per_atom, s, nthreads=1)
+    z_bot = 10.0 - (min(zid + 3, g.n_slices - 1) + 1) * g.slice_thickness
+    top_slice = max(zid - 3, 0)
+    z_top_slice = 10.0 - top_slice * g.slice_thickness
+    if z0 + 3.0 >= z_top_slice:
+        z_top, extra = z_top_slice, 0.0
+    else:  # the first admitted grid point below the edge, plus the lone half step
+        k = np.ceil((z_top_slice - (z0 + 3.0)) / h - 1e-9)
+        z_top, extra = z_top_slice - k * h, 0.5 * h
+    # pixels 0.6 to 3 A from the atom's axis: the integrand is smooth on the step's scale there,
+    # so the Euler-Maclaurin expansion holds to O(h^4)
+    for (xi, yi) in ((26, 31), (30, 30), (20, 40), (34, 22)):
+        rho = float(np.hypot(xi * 0.25 - x0, yi * 0.25 - y0))
+        assert rho <= 8.0
+        f = lambda zz: f3(np.hypot(rho, zz - z0))
+        a, b = z_bot, z_top
+        integral = si.quad(f, a, b, limit=500, epsabs=0, epsrel=1e-13, points=[z0] if a < z0 < b else None)[0]
+        d = 1e-5
+        fp = lambda zz: (f(zz + d) - f(zz - d)) / (2 * d)
+        em = h * h / 12.0 * (fp(b) - fp(a))
+        got = total[yi, xi]
+        if extra == 0.0:
+            print(f"rho {rho:.2f}: sum {got:.10g}, quad + EM {integral + em:.10g}")
+            assert got == pytest.approx(integral + em, rel=1e-9, abs=1e-9), (rho, got, integral)
+        else:
+            # the lone half step: [z_top, z_top + h/2] by its left value, error <= h^2/8 |f'|
+            lone = extra * f(z_top)
+            print(f"rho {rho:.2f}: sum - lone {got - lone:.10g}, quad + EM {integral + em:.10g}")
+            assert got - lone == pytest.approx(integral + em, rel=1e-9, abs=1e-9)
+            half = si.quad(f, z_top, z_top + extra, epsabs=0, epsrel=1e-13)[0]
+            assert abs(got - integral - half - em) <= h * h / 8 * abs(fp(z_top)) * 1.05 + 1e-9 * abs(got)
+
+
+@pytest.mark.parametrize("order", ["a_first", "b_first"])
+def test_full3d_carried_p2_between_atoms_of_a_block_row(oracle, order):
+    """Two atoms in one (slice, block row) bin, both within 8 A of the pixel.  In slice 3
+    (z in [6, 7]) atom A at z = 4 admits every point (7 - 4 = 3 <= 3); atom B at z = 3.83 first
+    admits h0 = 4 (7 - 0.2 - 3.83 <= 3).  The reference keeps p2 across atoms of a row, so
+    B's first admitted point is paired with the last point of the atom before it.  Expected
+    sumz from point values of V3D with explicit trapezoid weights; the atoms' order in the row
+    is their input order (stable binning), so swapping them changes the sum."""
+    name, Zn = "kirkland", 38
+    g = _toy3d(oracle)
+    p, f3, _ = _v3d_fn(oracle, name, Zn)
+    A = (2.5, 2.0, 4.0)
+    B = (1.5, 2.5, 3.83)
+    atoms = [A, B] if order == "a_first" else [B, A]
+    r = oracle.sort_atoms(g, [a[0] for a in atoms], [a[1] for a in atoms], [a[2] for a in atoms], [Zn, Zn])
+    assert list(r["z_ids"]) == [6, 6] and list(r["block_ids"]) == [0, 0]
+    xi, yi = 8, 8  # pixel at (2, 2)
+    V = oracle.potential_slice(g, r, p.values, p.form, p.i_per_atom, 3, nthreads=1)
+    I, step = g.full3d_integrals, g.slice_thickness / g.full3d_integrals
+
+    def pts(atom):
+        rho = np.hypot(xi * 0.25 - atom[0], yi * 0.25 - atom[1])
+        zs = 7.0 - np.arange(I + 1) * step
+        return np.array([f3(np.hypot(rho, z - atom[2])) for z in zs]), (zs - atom[2]) <= 3.0
+
+    pa, ka = pts(A)
+    pb, kb = pts(B)
+    assert ka.all() and (not kb[:4].any()) and kb[4:].all()
+    trap_a = sum(0.5 * (pa[h] + pa[h - 1]) for h in range(1, I + 1))
+    trap_b = sum(0.5 * (pb[h] + pb[h - 1]) for h in range(5, I + 1))
+    if order == "a_first":
+        expect = trap_a + trap_b + 0.5 * (pb[4] + pa[I])   # B's first point pairs with A's last
+    else:
+        expect = trap_b + 0.5 * (pb[4] + 0.0) + trap_a      # B first in its row: p2 starts at 0
+    assert V[yi, xi] == pytest.approx(expect, rel=1e-13)
+    # and the two orders differ by exactly the carried value
+    assert abs(0.5 * pa[I]) > 1e-3 * abs(expect)
+
+
+def test_full3d_tilt_shift_is_cumulative(oracle):
+    """With a beam tilt each integration point moves the lateral distance by one more step:
+    point h of an atom uses rad - (h + 1) * (kx, ky) / kz * dz / integrals
+    (kernels/transmission_potentials_full_3d_d.cl:245-246), so the slice sum of a single atom
+    equals the trapezoid over points at those shifted distances."""
+    tilt, az = 30.0, 0.7   # mrad, rad
+    g = _toy3d(oracle, tilt_mrad=tilt, azimuth=az)
+    name, Zn = "peng", 22
+    p, f3, _ = _v3d_fn(oracle, name, Zn)
+    atom = (6.0, 7.0, 4.0)  # window edge at slice 3's top: every point of slices 3..9 admitted
+    r = oracle.sort_atoms(g, [atom[0]], [atom[1]], [atom[2]], [Zn])
+    I, step = g.full3d_integrals, g.slice_thickness / g.full3d_integrals
+    k = np.array(g.k_vec[:])
+    sx, sy = k[0] / k[2] * step, k[1] / k[2] * step
+    assert abs(np.hypot(k[0], k[1]) / k[2] - np.tan(tilt * 1e-3)) < 1e-12
+    xi, yi = 25, 27
+    for s in (3, 5, 9):
+        V = oracle.potential_slice(g, r, p.values, p.form, p.i_per_atom, s, nthreads=1)
+        top = 10.0 - s
+        vals = []
+        for h in range(I + 1):
+            rx = xi * 0.25 - atom[0] - (h + 1) * sx
+            ry = yi * 0.25 - atom[1] - (h + 1) * sy
+            vals.append(f3(np.sqrt(rx * rx + ry * ry + (top - h * step - atom[2]) ** 2)))
+        expect = sum(0.5 * (vals[h] + vals[h - 1]) for h in range(1, I + 1))
+        assert V[yi, xi] == pytest.approx(expect, rel=1e-12), s
+        # without the cumulative shift (a single shift per atom) the sum is different
+        alt = []
+        for h in range(I + 1):
+            rx = xi * 0.25 - atom[0] - sx
+            ry = yi * 0.25 - atom[1] - sy
+            alt.append(f3(np.sqrt(rx * rx + ry * ry + (top - h * step - atom[2]) ** 2)))
+        assert abs(sum(0.5 * (alt[h] + alt[h - 1]) for h in range(1, I + 1)) - expect) > 1e-9 * abs(expect)
+
+
+# ---- off-centre detector (kernels/band_pass_d.cl:15-38, simulationstem.cpp:98-131) -----
+@pytest.mark.parametrize("det", [(5.0, 25.0, 10.0, -5.0), (0.0, 12.0, -14.0, 9.0), (8.0, 30.0, 3.0, 17.0)])
+def test_off_centre_annulus_counts_pixels(oracle, det):
+    """Empty specimen: |Psi(k)| = 1 inside the condenser aperture (k < alpha / lambda) and 0
+    outside, so a detector reads sqrt(2) x the number of aperture pixels inside its annulus.
+    The annulus is centred at (W/2 + xc, H/2 + yc) pixels of the fft-shifted image (xc, yc in
+    mrad over the angular pixel scale), counted here in numpy from the k-grid."""
+    w = wl.stem_srtio3(resolution=256, pixels=4)
+    g = oracle.geometry(oracle.scenario(**w["scenario"]))
+    trans = np.ones((g.n_slices, 256, 256), complex)
+    P_ = oracle.propagator(g)
+    pos = wl.stem_positions(g)[:2]
+    out = oracle.run_stem(g, trans, P_, pos, np.array([det]), cond_ap_mrad=20.0)
+    # independent count: shifted pixel (xs, ys) holds unshifted ((xs + W/2) % W, (ys + H/2) % H)
+    n = 256
+    kx, ky = np.fft.fftfreq(n, d=g.real_scale), np.fft.fftfreq(n, d=g.real_scale)
+    us = (np.arange(n) + n // 2) % n
+    kk = np.hypot(kx[us][None, :], ky[us][:, None])
+    aperture = kk < 0.020 / g.wavelength
+    sc = g.inverse_scale_angle
+    ys, xs = np.mgrid[0:n, 0:n]
+    rr = np.hypot(xs - (n / 2 + det[2] / sc), ys - (n / 2 + det[3] / sc))
+    ring = (rr >= det[0] / sc) & (rr <= det[1] / sc)
+    count = int(np.sum(aperture & ring))
+    assert 0 < count < int(np.sum(aperture))
+    np.testing.assert_allclose(out[:, 0], np.sqrt(2) * count, rtol=1e-12)
+
+
+@pytest.mark.parametrize("det", [(5.0, 25.0, 10.0, -5.0), (0.0, 12.0, -14.0, 9.0), (8.0, 30.0, 3.0, 17.0)])
+def test_off_centre_annulus_direction(oracle, det):
+    """The aperture disc above is symmetric, so it cannot tell the offset's direction: on a
+    random unshifted Psi(k) the detector is sqrt(2) x the sum of |Psi|^2 over the ring pixels
+    of the fft-shifted image centred at (W/2 + xc, H/2 + yc), and the mirrored centres read
+    other values."""
+    n = 128
+    rng = np.random.default_rng(11)
+    recip = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    sc = 0.9  # mrad per pixel
+    got = oracle.stem_pixel(recip, det[0], det[1], det[2], det[3], sc, sqrt2=True)
+    shifted = np.fft.fftshift(np.abs(recip) ** 2)
+    ys, xs = np.mgrid[0:n, 0:n]
+
+    def ring(cx, cy):
+        rr = np.hypot(xs - (n / 2 + cx / sc), ys - (n / 2 + cy / sc))
+        return np.sqrt(2) * float(np.sum(shifted[(rr >= det[0] / sc) & (rr <= det[1] / sc)]))
+
+    assert got == pytest.approx(ring(det[2], det[3]), rel=1e-12)
+    for mx, my in ((-1, 1), (1, -1), (-1, -1)):
+        assert abs(ring(mx * det[2], my * det[3]) - got) > 1e-6 * got
