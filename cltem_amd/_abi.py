@@ -153,6 +153,7 @@ SIGNATURES = [
     ("mse_init_probes_range", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.POINTER(MseProbe)]),
     ("mse_multislice", C.c_int, [_eng, C.c_int, C.c_int]),
     ("mse_stem_detect", C.c_int, [_eng, C.POINTER(MseDetector), C.c_int, C.c_int, _dp]),
+    ("mse_stem_detect_noalloc", C.c_int, [_eng, C.POINTER(MseDetector), C.c_int, C.c_int, C.c_void_p]),
     ("mse_diffraction", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_exit_wave", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_ctem_image", C.c_int, [_eng, C.POINTER(MseImaging), C.c_int, _dp]),

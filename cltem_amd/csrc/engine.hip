@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -25,11 +26,40 @@ namespace mse_geom {
 void kgrid(int res, double real_scale, double* kx, double* ky, double* bandwidthkmax);
 }
 
+// Device workspace of mse_stem_detect_noalloc: detector bitmasks followed by the FP64
+// per-tile partial sums.  Each GPU holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for the current device
+// (noalloc_scratch below), never through this name.
+static constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 
 namespace mse_rt {
 
 
 void destroy_stem_graph(mse_engine* e);
+
+// Per-device address of `scratch`, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+void init_noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)device < g_scratch_ptr.size() && g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+        throw Error(MSE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    g_scratch_ptr[device] = p;
+}
+
+void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    void* p = (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+    REQUIRE(p, MSE_ERR_STATE, "no workspace address for device " + std::to_string(device));
+    return p;
+}
 
 struct DeviceGuard {
     int prev = -1;
@@ -308,6 +338,43 @@ void ensure_detectors(mse_engine* e, const mse_detector* dets, int n_det) {
     e->det_key = key;
 }
 
+// Detector radii in pixels of the fft-shifted image, as ensure_detectors derives them.
+struct DetectorRadii {
+    double inner[mse::kMaxDet], outer[mse::kMaxDet], cx[mse::kMaxDet], cy[mse::kMaxDet];
+};
+
+DetectorRadii detector_radii(const mse_engine* e, const mse_detector* dets, int n_det) {
+    DetectorRadii r{};
+    const double angle_scale = e->g.inverse_scale_angle;
+    for (int d = 0; d < n_det; ++d) {
+        r.inner[d] = dets[d].inner_mrad / angle_scale;
+        r.outer[d] = dets[d].outer_mrad / angle_scale;
+        r.cx[d] = e->res / 2.0 + dets[d].xc_mrad / angle_scale;
+        r.cy[d] = e->res / 2.0 + dets[d].yc_mrad / angle_scale;
+    }
+    return r;
+}
+
+// The bitmasks of ensure_detectors built on the device: one 32-pixel word per thread,
+// blockIdx.y = detector.  The same double expression without contraction and a correctly
+// rounded square root give the host's decisions bit for bit.  Every word is written.
+__global__ void k_detector_bits(unsigned* __restrict__ bits, int res, DetectorRadii r) {
+#pragma clang fp contract(off)
+    const size_t words = (size_t)res * res / 32;
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int d = blockIdx.y;
+    if (w >= words) return;
+    unsigned m = 0u;
+    for (int k = 0; k < 32; ++k) {
+        const size_t b = w * 32 + k;
+        const int yid = (int)(b / res), xid = (int)(b % res);
+        const double dx = xid - r.cx[d], dy = yid - r.cy[d];
+        const double radius = sqrt(dx * dx + dy * dy);
+        if (radius <= r.outer[d] && radius >= r.inner[d]) m |= 1u << k;
+    }
+    bits[(size_t)d * words + w] = m;
+}
+
 int run_guarded(const char* fn, mse_engine* e, const std::function<void()>& body) {
     try {
         if (e) {
@@ -340,6 +407,7 @@ int mse_create(int device_ordinal, mse_engine** out) {
         REQUIRE(device_ordinal >= 0 && device_ordinal < count, MSE_ERR_ARG,
                 "device ordinal " + std::to_string(device_ordinal) + " not present (" + std::to_string(count) + " devices)");
         DeviceGuard dg(device_ordinal);
+        init_noalloc_scratch(device_ordinal);
         auto* e = new mse_engine();
         e->device = device_ordinal;
         {
@@ -1197,6 +1265,46 @@ int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt
         }
         HIPX(hipMemcpyAsync(out, e->det_out.p, (size_t)nw * n_det * sizeof(double), hipMemcpyDeviceToHost, e->stream));
         HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_stem_detect_noalloc(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out_dev) {
+    return run_guarded("mse_stem_detect_noalloc", e, [&]() {
+        REQUIRE(e && dets && out_dev, MSE_ERR_ARG, "NULL argument");
+        check_waves(e);
+        REQUIRE(n_det >= 1 && n_det <= mse::kMaxDet, MSE_ERR_ARG, "1 to 8 detectors");
+        REQUIRE(!e->any_beam, MSE_ERR_UNSUPPORTED,
+                "waves of a configuration with its own beam need translated diffraction images; use mse_stem_detect");
+        const int nw = e->n_waves;
+        const size_t words = e->N / 32;
+        const size_t mask_bytes = (n_det * words * sizeof(unsigned) + 255) / 256 * 256;
+        // at most res/8 column tiles per wave and detector (8-column tiles)
+        const size_t partial_bytes = (size_t)nw * n_det * (e->res / 8) * sizeof(double);
+        REQUIRE(mask_bytes + partial_bytes <= kNoAllocScratchBytes, MSE_ERR_ARG,
+                "workspace of " + std::to_string(mask_bytes + partial_bytes) + " bytes (" + std::to_string(n_det) +
+                    " detectors at " + std::to_string(e->res) + "^2, " + std::to_string(nw) +
+                    " waves) exceeds the fixed " + std::to_string(kNoAllocScratchBytes) +
+                    " bytes: n_det*res^2/8 + n_waves*n_det*res must fit");
+        auto* base = static_cast<unsigned char*>(noalloc_scratch(e->device));
+        auto* bits = reinterpret_cast<unsigned*>(base);
+        auto* partials = reinterpret_cast<double*>(base + mask_bytes);
+        const DetectorRadii radii = detector_radii(e, dets, n_det);
+        launch(e, PK_OTHER, [&]() {
+            k_detector_bits<<<dim3((unsigned)((words + 255) / 256), n_det), 256, 0, e->stream>>>(bits, e->res, radii);
+        });
+        mse::ColArgs ca = base_col_args(e);
+        ca.scale = (float)(1.0 / e->res);
+        ca.det_scale = 1.0;
+        ca.det_bits = bits;
+        ca.ndet = n_det;
+        ca.partials = partials;
+        // the detector pass runs the default plan on 16- or 8-column tiles: at most res/8 tiles
+        const int ntiles = col_pass(e, mse::COL_FFT_DETECT, e->waves.p, e->waves.p, nw, e->N, ca);
+        const int nwd = nw * n_det;
+        launch(e, PK_OTHER, [&]() {
+            mse::k_reduce_partials<<<(nwd + 255) / 256, 256, 0, e->stream>>>(partials, ntiles, nwd, out_dev, 0,
+                                                                              sqrt2 ? M_SQRT2 : 1.0, nullptr);
+        });
     });
 }
 

@@ -216,6 +216,16 @@ int mse_multislice_graph(mse_engine* e, int first, int count);
 /* SimulationStem::getStemPixel for every wave and detector (microscope/simulationstem.cpp:42-131):
  * out[wave*n_det + d] = (sqrt2 ? sqrt(2) : 1) * sum over the annulus of |Psi_shifted|^2, FP64 */
 int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out);
+/* mse_stem_detect without any allocation.  No clTEM counterpart: it is for callers that
+ * cannot allocate device memory (code under stream capture, serving loops, harnesses that
+ * forbid hipMalloc).  out_dev is caller-owned DEVICE memory of n_waves*n_det doubles, written
+ * with the same values as mse_stem_detect; the call only queues work on the engine stream
+ * (mse_synchronize before reading out_dev on another stream).  The detector bitmasks and the
+ * per-tile partial sums live in a fixed 4 MiB device workspace per GPU, shared by every engine
+ * on that GPU: n_det*res^2/8 + n_waves*n_det*res bytes must fit (MSE_ERR_ARG otherwise),
+ * calls that overlap on one GPU race, and each call rewrites what it reads.
+ * MSE_ERR_UNSUPPORTED when a wave's configuration has a beam of its own (plasmon tilt). */
+int mse_stem_detect_noalloc(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out_dev);
 /* getDiffractionImage (simulationgeneral.cpp:772-805): |fftshift Psi(k)|^2, N doubles */
 int mse_diffraction(mse_engine* e, int wave, double* out);
 /* getExitWaveImage (simulationgeneral.cpp:808-829) without cropping: interleaved re/im, 2N doubles */
