@@ -58,6 +58,7 @@ def make_scenario(**kw) -> MseScenario:
     s.limit_factor = kw.get("limit_factor", 2.0 / 3.0)
     s.full3d = int(kw.get("full3d", 0))
     s.full3d_integrals = kw.get("full3d_integrals", 20)
+    s.moving_stem_frame = int(kw.get("moving_stem_frame", 0))
     return s
 
 
@@ -65,6 +66,17 @@ def plan_geometry(s: MseScenario) -> MseGeometry:
     g = MseGeometry()
     _check(_abi.load().mse_plan_geometry(C.byref(s), C.byref(g)))
     return g
+
+
+def stem_pixel_frames(g: MseGeometry, pixels) -> np.ndarray:
+    """Potential-grid origin (paddedSimLimitsX/Y(pixel)[0]) of each STEM pixel: its own field
+    under the moving frame, sim_x[0], sim_y[0] otherwise.  Returns (n, 2) doubles."""
+    lib = _abi.load()
+    px = np.asarray(pixels, dtype=np.int64).reshape(-1)
+    out = np.zeros((len(px), 2))
+    for i, p in enumerate(px):
+        _check(lib.mse_stem_pixel_frame(C.byref(g), int(p), _dp(out[i])))
+    return out
 
 
 def make_probe(cond_ap_mrad=20.0, smooth_mrad=0.0, taper=False, aberrations=None) -> MseProbe:
@@ -336,6 +348,20 @@ class Engine:
         _check(self.lib.mse_stem_batch(self.h, config, _dp(pos), len(pos), batch, C.byref(probe),
                                        detectors, len(detectors), int(sqrt2), int(use_graph),
                                        _dp(accum)))
+        self.n_waves = max(self.n_waves, batch)
+        return accum
+
+    def stem_batch_frames(self, config: int, positions, frames, batch: int, probe: MseProbe, detectors,
+                          sqrt2: bool = True, accum=None) -> np.ndarray:
+        """Moving STEM frame: probe p in its own field whose potential-grid origin is frames[p]."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
+        fr = np.ascontiguousarray(frames, dtype=np.float64).reshape(-1, 2)
+        assert fr.shape == pos.shape
+        if accum is None:
+            accum = np.zeros((len(pos), len(detectors)))
+        assert accum.shape == (len(pos), len(detectors)) and accum.dtype == np.float64
+        _check(self.lib.mse_stem_batch_frames(self.h, config, _dp(pos), _dp(fr), len(pos), batch, C.byref(probe),
+                                              detectors, len(detectors), int(sqrt2), _dp(accum)))
         self.n_waves = max(self.n_waves, batch)
         return accum
 

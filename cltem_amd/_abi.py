@@ -29,6 +29,7 @@ class MseScenario(C.Structure):
         ("limit_factor", C.c_double),
         ("full3d", C.c_int),
         ("full3d_integrals", C.c_int),
+        ("moving_stem_frame", C.c_int),
     ]
 
 
@@ -68,6 +69,10 @@ class MseGeometry(C.Structure):
         ("stem_pixels", C.c_int * 2),
         ("full3d", C.c_int),
         ("full3d_integrals", C.c_int),
+        ("moving_frame", C.c_int),
+        ("frame_step", C.c_double),
+        ("frame_half", C.c_double * 2),
+        ("pad_xy", C.c_double * 2),
     ]
 
 
@@ -138,6 +143,7 @@ SIGNATURES = [
     ("mse_synchronize", C.c_int, [_eng]),
     ("mse_device_info", C.c_int, [_eng, C.c_char_p, C.c_int, _ip, _dp]),
     ("mse_plan_geometry", C.c_int, [C.POINTER(MseScenario), C.POINTER(MseGeometry)]),
+    ("mse_stem_pixel_frame", C.c_int, [C.POINTER(MseGeometry), C.c_int, _dp]),
     ("mse_set_geometry", C.c_int, [_eng, C.POINTER(MseGeometry)]),
     ("mse_set_parameterisation", C.c_int, [_eng, C.c_int, C.c_int, _dp, C.c_int]),
     ("mse_sort_atoms", C.c_int, [_eng, C.c_int, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip, _ip, _ip]),
@@ -169,6 +175,8 @@ SIGNATURES = [
     ("mse_adjoint_positions", C.c_int, [_eng, _dp, _dp]),
     ("mse_stem_batch", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
                                  C.POINTER(MseDetector), C.c_int, C.c_int, C.c_int, _dp]),
+    ("mse_stem_batch_frames", C.c_int, [_eng, C.c_int, _dp, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
+                                        C.POINTER(MseDetector), C.c_int, C.c_int, _dp]),
     ("mse_set_kernel_variant", C.c_int, [_eng, C.c_int, C.c_int]),
     ("mse_autotune", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _ip]),
     ("mse_bandwidth_probe", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),

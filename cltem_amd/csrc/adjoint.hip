@@ -34,19 +34,6 @@ void copy_planes(mse_engine* e, float2* dst, const float2* src, size_t count) {
     HIPX(hipMemcpyAsync(dst, src, count * sizeof(float2), hipMemcpyDeviceToDevice, e->stream));
 }
 
-// BL(x) = F2^-1 M F2 x on `count` consecutive planes, in place (the passes of
-// mse_build_transmission)
-void band_limit_planes(mse_engine* e, float2* p, int count) {
-    mse::RowArgs ra = base_row_args(e);
-    ra.scale = 1.0f;
-    row_pass(e, mse::ROW_FFT, p, p, count, e->N, ra);
-    mse::ColArgs ca = base_col_args(e);
-    ca.lim = e->band_lim.p;
-    ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
-    col_pass(e, mse::COL_BAND, p, p, count, e->N, ca);
-    row_pass(e, mse::ROW_IFFT, p, p, count, e->N, ra);
-}
-
 }  // namespace
 }  // namespace mse_rt
 

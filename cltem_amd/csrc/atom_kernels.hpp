@@ -304,7 +304,22 @@ struct PotArgs {
     int integrals;
     int per_atom;        // parameters per Z (12 or 10)
     int i_count;
+    // moving STEM frames: when set, blockIdx.z selects a frame whose potential-grid origin is
+    // (frame_xy[2z], frame_xy[2z+1]) and every frame builds the same slice `first`
+    const double* frame_xy;
 };
+
+// The arguments of the plane blockIdx.z of a launch over slices [first, first + gridDim.z),
+// or over frames of slice `first` (frame_xy set); returns the slice.
+__device__ __forceinline__ int plane_args(const PotArgs& a, int first, PotArgs& b) {
+    b = a;
+    if (a.frame_xy) {
+        b.startx = a.frame_xy[2 * blockIdx.z];
+        b.starty = a.frame_xy[2 * blockIdx.z + 1];
+        return first;
+    }
+    return first + (int)blockIdx.z;
+}
 
 __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int gy, int& starti,
                                                  int& endi, int& startj, int& endj) {
@@ -329,19 +344,18 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
     return make_float2(c, s);
 }
 
-// Projected potential sum (sumz) at one pixel of slice first + blockIdx.z: one 16x16 pixel
+// Projected potential sum (sumz) at one pixel of slice `slice`: one 16x16 pixel
 // tile per block; the atoms of each block row are staged through LDS in chunks of 256,
 // with no limit on how many a row holds.
 template <int FORM>
 __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                  const int* __restrict__ aZ, const int* __restrict__ bsp,
-                                                 const float* __restrict__ ptab, const PotArgs& a, int first,
+                                                 const float* __restrict__ ptab, const PotArgs& a, int slice,
                                                  double* sx, double* sy, int* sZ) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
     const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
-    const int slice = first + blockIdx.z;
     int starti, endi, startj, endj;
     tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
     int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
@@ -408,7 +422,9 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first, sx, sy, sZ);
+    PotArgs b;
+    const int slice = plane_args(a, first, b);
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
@@ -424,7 +440,7 @@ __global__ __launch_bounds__(256) void k_phase_projected(
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first, sx, sy, sZ);
+    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
@@ -543,7 +559,9 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     const float* __restrict__ tab3d, PotArgs a, int first) {
     __shared__ double sx[256], sy[256], sz[256];
     __shared__ int sZ[256];
-    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
+    PotArgs b;
+    const int slice = plane_args(a, first, b);
+    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);

@@ -739,35 +739,9 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         const int chunk = g.full3d ? 1 : 16;
         for (int s0 = 0; s0 < count; s0 += chunk) {
             const int cs = std::min(chunk, count - s0);
-            dim3 grid(e->res / 16, e->res / 16, cs);
-            float2* o = outp + (size_t)s0 * e->N;
-            launch(e, PK_POT, [&]() {
-                if (g.full3d) {
-                    if (e->form == 0)
-                        mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
-                    else if (e->form == 1)
-                        mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
-                    else
-                        mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first + s0);
-                } else {
-                    if (e->form == 0)
-                        mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
-                    else if (e->form == 1)
-                        mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
-                    else
-                        mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first + s0);
-                }
-            });
+            potential_planes(e, c, g.full3d, a, outp + (size_t)s0 * e->N, cs, first + s0);
         }
-        // band limit: FFT -> keep |k| <= kmax*2/3 -> IFFT (simulationgeneral.cpp:596-602)
-        mse::RowArgs ra = base_row_args(e);
-        ra.scale = 1.0f;
-        row_pass(e, mse::ROW_FFT, outp, outp, count, e->N, ra);
-        mse::ColArgs ca = base_col_args(e);
-        ca.lim = e->band_lim.p;
-        ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
-        col_pass(e, mse::COL_BAND, outp, outp, count, e->N, ca);
-        row_pass(e, mse::ROW_IFFT, outp, outp, count, e->N, ra);
+        band_limit_planes(e, outp, count);
     });
 }
 
@@ -1075,6 +1049,40 @@ void active_waves(mse_engine* e, int nw) {
         HIPX(hipStreamSynchronize(e->stream));
     }
     e->map_key = key;
+}
+
+void potential_planes(mse_engine* e, const ConfigData& c, bool full3d, const mse::PotArgs& a, float2* out,
+                      int planes, int first) {
+    const dim3 grid(e->res / 16, e->res / 16, planes);
+    launch(e, PK_POT, [&]() {
+        if (full3d) {
+            if (e->form == 0)
+                mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+            else if (e->form == 1)
+                mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+            else
+                mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+        } else {
+            if (e->form == 0)
+                mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+            else if (e->form == 1)
+                mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+            else
+                mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+        }
+    });
+}
+
+// band limit: FFT -> keep |k| <= kmax*2/3 -> IFFT (simulationgeneral.cpp:596-602)
+void band_limit_planes(mse_engine* e, float2* p, int planes) {
+    mse::RowArgs ra = base_row_args(e);
+    ra.scale = 1.0f;
+    row_pass(e, mse::ROW_FFT, p, p, planes, e->N, ra);
+    mse::ColArgs ca = base_col_args(e);
+    ca.lim = e->band_lim.p;
+    ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
+    col_pass(e, mse::COL_BAND, p, p, planes, e->N, ca);
+    row_pass(e, mse::ROW_IFFT, p, p, planes, e->N, ra);
 }
 
 void slice_step(mse_engine* e, float2* waves, int nw, int slice) {
@@ -1460,6 +1468,116 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
                 multislice(e, nb, 0, ns, true, &da);
                 reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
             }
+        }
+        HIPX(hipMemcpyAsync(accum, e->det_out.p, (size_t)n_probes * n_det * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+        HIPX(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mse_stem_batch_frames(mse_engine* e, int cfg, const double* pos_xy, const double* frame_xy, int n_probes,
+                          int batch, const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
+                          double* accum) {
+    return run_guarded("mse_stem_batch_frames", e, [&]() {
+        REQUIRE(e && pos_xy && frame_xy && probe && dets && accum, MSE_ERR_ARG, "NULL argument");
+        require_geom(e);
+        REQUIRE(e->form >= 0, MSE_ERR_STATE, "mse_set_parameterisation has not been called");
+        REQUIRE(n_probes >= 1 && batch >= 1 && n_det >= 1, MSE_ERR_ARG, "counts must be positive");
+        REQUIRE(batch <= 65535, MSE_ERR_ARG, "at most 65535 probes per batch");
+        for (int i = 0; i < 2 * n_probes; ++i)
+            REQUIRE(std::isfinite(frame_xy[i]) && std::isfinite(pos_xy[i]), MSE_ERR_ARG, "position not finite");
+        ConfigData& c = config(e, cfg);
+        REQUIRE(c.sorted, MSE_ERR_STATE, "atoms of this configuration are not sorted for this geometry");
+        REQUIRE(!c.own_beam, MSE_ERR_UNSUPPORTED, "moving frames run with the engine's beam");
+        REQUIRE(e->mix_sets < 2, MSE_ERR_UNSUPPORTED, "moving frames build their own transmission functions");
+        const mse_geometry& g = cfg_geom(e, c);
+        const int ns = g.n_slices;
+        if (e->n_waves < batch || !e->waves.p || e->mix_key.n < (size_t)e->n_waves) {
+            HIPX(hipStreamSynchronize(e->stream));
+            size_wave_buffers(e, std::max(batch, e->n_waves));
+            e->n_waves = std::max(batch, e->n_waves);
+        }
+        std::vector<int> wc(e->n_waves, cfg);
+        if (e->h_wave_cfg != wc) {
+            e->h_wave_cfg = wc;
+            HIPX(hipMemcpyAsync(e->wave_cfg.p, wc.data(), wc.size() * sizeof(int), hipMemcpyHostToDevice,
+                                e->stream));
+        }
+        sync_wave_prop(e);
+        ensure_detectors(e, dets, n_det);
+        HIPX(hipStreamSynchronize(e->stream));  // buffers below may be reallocated
+        e->partials.ensure((size_t)batch * n_det * (e->res / 8));
+        e->det_out.ensure((size_t)n_probes * n_det);
+        e->frame_xy.ensure(2 * (size_t)batch);
+        e->frame_trans.ensure((size_t)batch * e->N);
+        // the row pass finds wave w's transmission function as plane w of frame_trans: a table
+        // of one "configuration" per wave, each one slice deep, read at slice 0
+        e->frame_ptrs.ensure(batch);
+        e->frame_ns.ensure(batch);
+        e->frame_wcfg.ensure(batch);
+        if (!e->h_focus.empty()) e->probe_df.ensure(std::max<size_t>(batch, e->n_waves));
+        {
+            std::vector<const float2*> ptrs(batch);
+            std::vector<int> ones(batch, 1), ids(batch);
+            for (int w = 0; w < batch; ++w) {
+                ptrs[w] = e->frame_trans.p + (size_t)w * e->N;
+                ids[w] = w;
+            }
+            HIPX(hipMemcpyAsync(e->frame_ptrs.p, ptrs.data(), batch * sizeof(const float2*), hipMemcpyHostToDevice,
+                                e->stream));
+            HIPX(hipMemcpyAsync(e->frame_ns.p, ones.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(e->frame_wcfg.p, ids.data(), batch * sizeof(int), hipMemcpyHostToDevice, e->stream));
+            HIPX(hipMemcpyAsync(e->det_out.p, accum, (size_t)n_probes * n_det * sizeof(double),
+                                hipMemcpyHostToDevice, e->stream));
+            HIPX(hipStreamSynchronize(e->stream));  // the host vectors go out of scope
+        }
+        mse::PotArgs pa = pot_args(e, c);
+        pa.frame_xy = e->frame_xy.p;
+        mse::RowArgs ra = base_row_args(e);
+        ra.cfg_trans = e->frame_ptrs.p;
+        ra.cfg_nslices = e->frame_ns.p;
+        ra.wave_cfg = e->frame_wcfg.p;
+        ra.slice = 0;
+        ra.mix_sets = 0;
+        const mse::ColArgs ca = base_col_args(e);
+        mse::ColArgs da = base_col_args(e);
+        da.det_scale = 1.0;
+        da.det_bits = e->det_bits.p;
+        da.ndet = n_det;
+        da.partials = e->partials.p;
+        std::vector<double> rel(2 * (size_t)batch), org(2 * (size_t)batch);
+        for (int p0 = 0; p0 < n_probes; p0 += batch) {
+            const int nb = std::min(batch, n_probes - p0);
+            for (int i = 0; i < 2 * nb; ++i) {
+                const double f = frame_xy[2 * (size_t)p0 + i];
+                // each probe relative to its own field's origin (simulationcbed.cpp:54-59); the
+                // potential grid at that origin plus the source-size offset (simulationgeneral.cpp:559-560)
+                rel[i] = pos_xy[2 * (size_t)p0 + i] - f;
+                org[i] = f + c.ref_off[i & 1];
+            }
+            HIPX(hipMemcpyAsync(e->probe_pos.p, rel.data(), 2 * (size_t)nb * sizeof(double), hipMemcpyHostToDevice,
+                                e->stream));
+            HIPX(hipMemcpyAsync(e->frame_xy.p, org.data(), 2 * (size_t)nb * sizeof(double), hipMemcpyHostToDevice,
+                                e->stream));
+            if (!e->h_focus.empty()) {
+                REQUIRE((size_t)p0 + nb <= e->h_focus.size(), MSE_ERR_ARG,
+                        "fewer focus changes than probes (mse_set_focus_spread)");
+                HIPX(hipMemcpyAsync(e->probe_df.p, e->h_focus.data() + p0, nb * sizeof(double),
+                                    hipMemcpyHostToDevice, e->stream));
+            }
+            probes_into_waves(e, nb, probe);
+            for (int s = 0; s < ns; ++s) {
+                potential_planes(e, c, g.full3d, pa, e->frame_trans.p, nb, s);
+                band_limit_planes(e, e->frame_trans.p, nb);
+                row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nb, e->N, ra);
+                if (s == ns - 1)
+                    e->det_tiles = col_pass(e, mse::COL_PROP_DETECT, e->waves.p, e->waves.p, nb, e->N, da);
+                else
+                    col_pass(e, mse::COL_PROP, e->waves.p, e->waves.p, nb, e->N, ca);
+            }
+            reduce_detectors(e, nb, n_det, sqrt2, e->det_out.p + (size_t)p0 * n_det, 1);
+            // rel and the frames are reused by the next batch: its uploads must not overtake
+            HIPX(hipStreamSynchronize(e->stream));
         }
         HIPX(hipMemcpyAsync(accum, e->det_out.p, (size_t)n_probes * n_det * sizeof(double), hipMemcpyDeviceToHost,
                             e->stream));

@@ -170,6 +170,12 @@ struct mse_engine {
     std::vector<int> map_key;                 // wave configurations and slice counts it was built for
     std::vector<int> map_count, map_offset;   // per slice: active waves, offset into wave_map (-1: prefix)
     DevBuf<long long> dev_offset;
+    // moving STEM frame (mse_stem_batch_frames): per wave of a batch its frame origin, the
+    // transmission plane of the current slice, and the row pass's lookup tables into them
+    DevBuf<double> frame_xy;
+    DevBuf<float2> frame_trans;
+    DevBuf<const float2*> frame_ptrs;
+    DevBuf<int> frame_ns, frame_wcfg;
     int num_cus = 256;
     int det_tiles = 0;             // column tiles of the last detector launch
     // profiling
@@ -233,6 +239,12 @@ int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, 
              const mse::ColArgs& args, int kind = PK_COL, int variant = -1);
 // one forward slice (row pass with t, column pass with the propagator) on nw waves in place
 void slice_step(mse_engine* e, float2* waves, int nw, int slice);
+// exp(i sigma V) of `planes` planes into out from the sorted atoms of c: slices
+// first..first+planes-1, or with a.frame_xy set one plane per frame of slice `first`
+void potential_planes(mse_engine* e, const ConfigData& c, bool full3d, const mse::PotArgs& a, float2* out,
+                      int planes, int first);
+// transmission band limit of `planes` planes in place: FFT, |k| <= 2/3 kmax, IFFT
+void band_limit_planes(mse_engine* e, float2* planes_p, int planes);
 // refresh wave_prop / any_beam after the waves' configurations or a configuration's beam changed
 void sync_wave_prop(mse_engine* e);
 

@@ -45,6 +45,18 @@ void kgrid(int res, double real_scale, double* kx, double* ky, double* bandwidth
     *bandwidthkmax = std::min(std::abs(ky[imid]), std::abs(kx[imid]));
 }
 
+// StemArea::getPixelSimArea (utilities/commonstructs.h:379-396): the pixel's scan position,
+// snapped down to a multiple of the full field's pixel size, from the STEM area's start
+void frame_position(const mse_geometry* g, int pixel, double* xp, double* yp) {
+    const unsigned nx = (unsigned)g->stem_pixels[0];
+    const unsigned x_px = (unsigned)pixel % nx;
+    const unsigned y_px = (unsigned)pixel / nx;
+    const double pos_x = g->stem_scale[0] * x_px;
+    const double pos_y = g->stem_scale[1] * y_px;
+    *xp = g->stem_start[0] + g->frame_step * std::floor(pos_x / g->frame_step);
+    *yp = g->stem_start[1] + g->frame_step * std::floor(pos_y / g->frame_step);
+}
+
 }  // namespace mse_geom
 
 extern "C" int mse_plan_geometry(const mse_scenario* s, mse_geometry* g) {
@@ -76,6 +88,31 @@ extern "C" int mse_plan_geometry(const mse_scenario* s, mse_geometry* g) {
     for (int i = 0; i < 2; ++i) {
         g->sim_x[i] = g->full_x[i];
         g->sim_y[i] = g->full_y[i];
+        g->pad_xy[i] = s->pad_xy[i];
+    }
+    if (s->stem_pixels[0] > 0 && s->stem_pixels[1] > 0) {
+        // StemArea::getScaleX/Y (utilities/commonstructs.cpp:99-107)
+        g->stem_scale[0] = (a[1] - a[0]) / s->stem_pixels[0];
+        g->stem_scale[1] = (a[3] - a[2]) / s->stem_pixels[1];
+    }
+    g->stem_start[0] = a[0];
+    g->stem_start[1] = a[2];
+    if (s->mode == 2 && s->moving_stem_frame) {
+        if (!(s->stem_pixels[0] > 0 && s->stem_pixels[1] > 0)) return MSE_ERR_ARG;
+        // moving frame: each pixel's field is a zero-width area at the pixel's snapped
+        // position, padded by the STEM area padding (getPixelSimArea, commonstructs.h:379-396);
+        // the snapping step is the full field's pixel size (simulationmanager.cpp:485-501)
+        g->moving_frame = 1;
+        const double full_range = (s->pad_xy[1] - s->pad_xy[0]) + ((a[1] + x_diff) - (a[0] - x_diff));
+        g->frame_step = full_range / (double)s->resolution;
+        // a zero-width area: range_total = padding, half of it each side
+        g->frame_half[0] = g->frame_half[1] = (s->area_padding - 0.0) / 2;
+        double xp, yp;
+        mse_geom::frame_position(g, 0, &xp, &yp);  // pixel 0's field sets realScale
+        g->sim_x[0] = (xp - g->frame_half[0]) + s->pad_xy[0];
+        g->sim_x[1] = (xp + g->frame_half[0]) + s->pad_xy[1];
+        g->sim_y[0] = (yp - g->frame_half[1]) + s->pad_xy[0];
+        g->sim_y[1] = (yp + g->frame_half[1]) + s->pad_xy[1];
     }
 
     // SimulationCell::roundPaddingZ (structure/simulationcell.cpp:43-61)
@@ -131,11 +168,21 @@ extern "C" int mse_plan_geometry(const mse_scenario* s, mse_geometry* g) {
         delete[] kx;
         delete[] ky;
     }
-    if (s->stem_pixels[0] > 0 && s->stem_pixels[1] > 0) {
-        g->stem_scale[0] = (a[1] - a[0]) / s->stem_pixels[0];
-        g->stem_scale[1] = (a[3] - a[2]) / s->stem_pixels[1];
+    return MSE_OK;
+}
+
+extern "C" int mse_stem_pixel_frame(const mse_geometry* g, int pixel, double* start_xy) {
+    if (!g || !start_xy || pixel < 0) return MSE_ERR_ARG;
+    if (!g->moving_frame) {
+        start_xy[0] = g->sim_x[0];
+        start_xy[1] = g->sim_y[0];
+        return MSE_OK;
     }
-    g->stem_start[0] = a[0];
-    g->stem_start[1] = a[2];
+    if (pixel >= g->stem_pixels[0] * g->stem_pixels[1]) return MSE_ERR_ARG;
+    double xp, yp;
+    mse_geom::frame_position(g, pixel, &xp, &yp);
+    // the padded lower limits of the zero-width area at (xp, yp)
+    start_xy[0] = (xp - g->frame_half[0]) + g->pad_xy[0];
+    start_xy[1] = (yp - g->frame_half[1]) + g->pad_xy[0];
     return MSE_OK;
 }

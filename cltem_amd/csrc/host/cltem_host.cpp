@@ -217,6 +217,8 @@ int SimulationManager::mixedSets() const {
     // count below the number of iterations (simulationmanager.h:218-231)
     const bool phonons = phonon_configs > 1 && u_rms > 0.0;
     if (!phonons || plasmon.enabled || parallel_potentials < 2) return 0;
+    // STEM shares sets only with the static area (parallelStem())
+    if (scenario.mode == 2 && !parallel_stem) return 0;
     if (scenario.mode != 2 && parallel_potentials >= phonon_configs) return 0;
     return parallel_potentials;
 }
@@ -291,8 +293,10 @@ std::map<std::string, Image> SimulationManager::averaged() const {
 }
 
 mse_geometry SimulationManager::geometry() const {
+    mse_scenario s = scenario;
+    s.moving_stem_frame = (s.mode == 2 && !parallel_stem) ? 1 : 0;
     mse_geometry g;
-    check(mse_plan_geometry(&scenario, &g));
+    check(mse_plan_geometry(&s, &g));
     return g;
 }
 
@@ -376,7 +380,9 @@ bool SimulationGeneral::initialiseSimulation() {
     sortAtoms();
     // simulationgeneral.cpp:559-560: the potential grid origin carries the source-size offset
     check(mse_set_reference_offset(eng, 0, reference_perturb_x, reference_perturb_y));
-    check(mse_build_transmission(eng, 0, 0, geom.n_slices));
+    // a moving STEM frame builds every pixel's transmission functions in its own field
+    // (mse_stem_batch_frames), none in pixel 0's
+    if (!geom.moving_frame) check(mse_build_transmission(eng, 0, 0, geom.n_slices));
     return true;
 }
 
@@ -550,7 +556,16 @@ void SimulationStem::simulate() {
     std::vector<double> df(sm->chromatic.enabled ? pixels.size() : 0);
     for (size_t i = 0; i < df.size(); ++i) df[i] = sm->focusChange(config, pixels[i]);
     check(mse_set_focus_spread(eng, df.data(), (int)df.size()));
-    if (!sm->plasmon.enabled && sm->slice_step <= 0) {
+    if (geom.moving_frame) {
+        // parallel_stem off: one field per pixel (currentAreaBase(pixel), simulationmanager.cpp:485-501)
+        if (sm->plasmon.enabled || sm->slice_step > 0)
+            throw std::runtime_error("the moving STEM frame runs without plasmons or intermediate outputs here");
+        std::vector<double> frames(2 * pixels.size());
+        for (size_t i = 0; i < pixels.size(); ++i) check(mse_stem_pixel_frame(&geom, pixels[i], &frames[2 * i]));
+        check(mse_stem_batch_frames(eng, 0, pos.data(), frames.data(), (int)pixels.size(), batch, &sm->probe,
+                                    dets.data(), nd, sm->sqrt2 ? 1 : 0, acc.data()));
+        n_waves = batch;
+    } else if (!sm->plasmon.enabled && sm->slice_step <= 0) {
         check(mse_stem_batch(eng, 0, pos.data(), (int)pixels.size(), batch, &sm->probe, dets.data(), nd,
                              sm->sqrt2 ? 1 : 0, 1, acc.data()));
         n_waves = batch;
@@ -846,7 +861,7 @@ std::string settings_json(const SimulationManager& sm, const std::string& image)
         st["area"]["padding"]["val"] = sc.area_padding;
         st["area"]["padding"]["units"] = "Å";
         st["static area"]["concurrent pixels"] = sm.parallel_pixels;
-        st["static area"]["enabled"] = true;
+        st["static area"]["enabled"] = sm.parallel_stem;
     }
     if (mode == 3) {
         j["cbed"]["position"]["x"] = sc.area[0];

@@ -131,6 +131,9 @@ struct SimulationManager {
     std::vector<Detector> detectors;
     AtomList atoms;            // undisplaced structure
     int parallel_pixels = 64;  // probes per batch
+    // STEM "static area" (parallelStem, simulationmanager.h:194-200): off, every pixel runs in
+    // a field of its own (getPixelSimArea) with its transmission functions rebuilt for it
+    bool parallel_stem = true;
     int phonon_configs = 1;    // frozen-phonon iterations (0 or 1 = no phonons)
     double u_rms = 0.0;
     uint64_t seed = 20261202;

@@ -4,7 +4,7 @@
 //           (--xyz file.xyz | --srtio3 NX,NY,NZ | --cif file.cif --widths X,Y,Z [--zone u,v,w]
 //            [--normal a,b,c] [--tilts a,b,g]) --res 512 --dz 1.9525 --kv 200
 //           [--area x0,x1,y0,y1] [--stem-pixels N] [--det name,inner,outer[,xc,yc]]...
-//           [--alpha mrad] [--batch B] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
+//           [--alpha mrad] [--batch B] [--moving-frame] [--phonons K --urms A --seed S] [--tilt mrad,azimuth]
 //           [--full3d integrals] [--chromatic Cc_mm,hwhm_neg_eV,hwhm_pos_eV] [--source-size fwhm_A]
 //           [--plasmon mfp_nm,characteristic_mrad,critical_mrad[,n_individual]] [--parallel-potentials N]
 //           [--intermediate SLICES]
@@ -112,6 +112,8 @@ int main(int argc, char** argv) {
                 sm->probe.cond_ap_mrad = std::stod(next());
             } else if (a == "--batch") {
                 sm->parallel_pixels = std::stoi(next());
+            } else if (a == "--moving-frame") {  // the settings' "static area enabled": false
+                sm->parallel_stem = false;
             } else if (a == "--phonons") {
                 sm->phonon_configs = std::stoi(next());
             } else if (a == "--urms") {

@@ -47,6 +47,8 @@ typedef struct mse_scenario {
     double limit_factor;     /* SimulationManager::max_inverse_factor, 2/3 */
     int full3d;              /* SimulationManager::use_full_3d */
     int full3d_integrals;    /* SimulationManager::full_3d_integrals, 20 */
+    int moving_stem_frame;   /* STEM only: 1 = !SimulationManager::parallel_stem, every pixel in a
+                                field of its own (StemArea::getPixelSimArea); 0 = clTEM's default */
 } mse_scenario;
 
 /* Derived quantities the reference computes on demand from the manager. */
@@ -74,6 +76,12 @@ typedef struct mse_geometry {
     double stem_start[2];        /* StemArea raw start */
     int stem_pixels[2];
     int full3d, full3d_integrals;
+    /* moving STEM frame (simulationmanager.cpp:485-501): sim_x/sim_y above are pixel 0's field
+     * and fix real_scale; pixel p's field starts at mse_stem_pixel_frame(g, p) */
+    int moving_frame;
+    double frame_step;           /* the full field's pixel size the frame origins snap to */
+    double frame_half[2];        /* half the STEM area padding, each side of a frame */
+    double pad_xy[2];            /* SimulationCell xy padding */
 } mse_geometry;
 
 /* Probe (condenser) parameters: SimulationCbed::initialiseProbeWave
@@ -128,6 +136,11 @@ int mse_device_info(mse_engine* e, char* name, int name_len, int* cu_count, doub
 
 /* ---- geometry (host only; SimulationManager geometry methods, see mse_geometry) ---- */
 int mse_plan_geometry(const mse_scenario* s, mse_geometry* g);
+/* paddedSimLimitsX/Y(pixel)[0] of a moving-frame STEM geometry: the potential-grid origin of
+ * STEM pixel `pixel` (SimulationManager::currentAreaBase, simulationmanager.cpp:485-501, and
+ * StemArea::getPixelSimArea, utilities/commonstructs.h:379-396).  For any other geometry
+ * every pixel shares sim_x[0], sim_y[0]. */
+int mse_stem_pixel_frame(const mse_geometry* g, int pixel, double* start_xy);
 
 /* ---- SimulationGeneral::initialiseSimulation (microscope/simulationgeneral.cpp:345-646):
  *      k-grid, band-limit and propagator tables (propagator kernel, kernels/propagator_f.cl:18-51) ---- */
@@ -264,6 +277,19 @@ int mse_diffraction_sum(mse_engine* e, const int* wave_group, int n_groups, int 
 int mse_stem_batch(mse_engine* e, int config, const double* pos_xy, int n_probes, int batch,
                    const mse_probe* probe, const mse_detector* dets, int n_det, int sqrt2,
                    int use_graph, double* accum);
+/* The same with the moving STEM frame (parallel_stem off): probe p runs in a field of its own
+ * whose origin is frame_xy[2p], frame_xy[2p+1] (mse_stem_pixel_frame of its pixel); its
+ * position is taken relative to that origin (simulationcbed.cpp:54-59) and its potential grid
+ * starts there plus the configuration's reference offset (mse_set_reference_offset).  clTEM
+ * regenerates every transmission function for each pixel job (initialiseSimulation,
+ * microscope/simulationgeneral.cpp:345-358); here, slice by slice, the transmission functions
+ * of the batch's frames are built from the sorted atoms of `config` (its transmission
+ * functions need not be built), band limited, and each multiplies its own wave.  The engine
+ * geometry (grid, field size, optics) is the moving-frame geometry.  Device memory: one
+ * transmission plane per wave of the batch.  Eager launches only. */
+int mse_stem_batch_frames(mse_engine* e, int config, const double* pos_xy, const double* frame_xy,
+                          int n_probes, int batch, const mse_probe* probe, const mse_detector* dets,
+                          int n_det, int sqrt2, double* accum);
 
 /* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..6):
  *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at

@@ -55,6 +55,31 @@ void corrected_limits(const double* a, double padding, double* lx, double* ly) {
     ly[1] = a[3] + y_diff;
 }
 
+// SimulationManager::currentAreaBase for STEM without parallel pixels
+// (simulationmanager.cpp:485-501): StemArea::getPixelSimArea(pixel, rs)
+// (utilities/commonstructs.h:379-396), whose corrected limits are returned.  rs is the pixel
+// size of the full field (cell padding range + corrected STEM area range over the resolution).
+void moving_frame_area(const OrcScenario* s, int pixel, double* lx, double* ly) {
+    const double pad_range = s->pad_xy[1] - s->pad_xy[0];
+    double area_x[2], area_y[2];
+    corrected_limits(s->area, s->area_padding, area_x, area_y);
+    const double area_range = area_x[1] - area_x[0];
+    const double rs = (pad_range + area_range) / static_cast<double>(s->resolution);
+    const unsigned int xPixels = (unsigned int)s->stem_pixels[0];
+    const unsigned int yPixels = (unsigned int)s->stem_pixels[1];
+    const unsigned int x_px = pixel % xPixels;
+    const unsigned int y_px = (unsigned int)std::floor(pixel / xPixels);
+    // StemArea::getScaleX/Y: (finish - start) / pixels
+    const double pos_x = ((s->area[1] - s->area[0]) / xPixels) * x_px;
+    const double pos_y = ((s->area[3] - s->area[2]) / yPixels) * y_px;
+    const double real_pos_x = rs * std::floor(pos_x / rs);
+    const double real_pos_y = rs * std::floor(pos_y / rs);
+    const double xPos = s->area[0] + real_pos_x;
+    const double yPos = s->area[2] + real_pos_y;
+    const double frame[4] = {xPos, xPos, yPos, yPos};
+    corrected_limits(frame, s->area_padding, lx, ly);
+}
+
 // ---------------------------------------------------------------------------------------
 // Bessel K0/K1: Numerical Recipes in C 2nd ed. section 6.6 rational forms, with the
 // coefficient tables the reference kernel uses
@@ -728,6 +753,17 @@ int orc_geometry(const OrcScenario* s, OrcGeometry* g) {
         g->sim_x[i] = g->full_x[i];
         g->sim_y[i] = g->full_y[i];
     }
+    if (s->mode == 2 && s->moving_stem_frame) {
+        // moving frame: realScale and every scale after it follow paddedSimLimitsX/Y(0)
+        if (s->stem_pixels[0] <= 0 || s->stem_pixels[1] <= 0) return -1;
+        double fx[2], fy[2];
+        moving_frame_area(s, 0, fx, fy);
+        g->sim_x[0] = fx[0] + s->pad_xy[0];
+        g->sim_x[1] = fx[1] + s->pad_xy[1];
+        g->sim_y[0] = fy[0] + s->pad_xy[0];
+        g->sim_y[1] = fy[1] + s->pad_xy[1];
+        g->moving_frame = 1;
+    }
     // SimulationCell::roundPaddingZ (structure/simulationcell.cpp:43-61)
     {
         const double slice_thickness = s->slice_thickness, slice_offset = s->slice_offset;
@@ -797,6 +833,21 @@ int orc_geometry(const OrcScenario* s, OrcGeometry* g) {
     }
     g->stem_start[0] = s->area[0];
     g->stem_start[1] = s->area[2];
+    return 0;
+}
+
+int orc_geometry_pixel(const OrcScenario* s, int pixel, OrcGeometry* g) {
+    if (orc_geometry(s, g) != 0) return -1;
+    if (!g->moving_frame) return 0;
+    if (pixel < 0 || pixel >= s->stem_pixels[0] * s->stem_pixels[1]) return -1;
+    // paddedSimLimitsX/Y(pixel) = currentAreaBase(pixel) corrected + cell padding
+    // (simulationmanager.cpp:434-440); the scales stay those of pixel 0 (realScale)
+    double fx[2], fy[2];
+    moving_frame_area(s, pixel, fx, fy);
+    g->sim_x[0] = fx[0] + s->pad_xy[0];
+    g->sim_x[1] = fx[1] + s->pad_xy[1];
+    g->sim_y[0] = fy[0] + s->pad_xy[0];
+    g->sim_y[1] = fy[1] + s->pad_xy[1];
     return 0;
 }
 

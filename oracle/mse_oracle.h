@@ -63,6 +63,7 @@ typedef struct OrcScenario {
     double limit_factor;      /* max_inverse_factor = 2/3 */
     int full3d;
     int full3d_integrals;
+    int moving_stem_frame;    /* STEM: 1 = !SimulationManager::parallel_stem */
 } OrcScenario;
 
 typedef struct OrcGeometry {
@@ -89,11 +90,15 @@ typedef struct OrcGeometry {
     double stem_start[2];
     int stem_pixels[2];
     int full3d, full3d_integrals;
+    int moving_frame;              /* sim_x/sim_y are the field of one STEM pixel */
 } OrcGeometry;
 
 double orc_wavelength(double kv);
 double orc_sigma(double kv);
 int orc_geometry(const OrcScenario* s, OrcGeometry* g);
+/* the geometry of STEM pixel `pixel`'s job: with the moving frame, sim_x/sim_y are
+ * paddedSimLimitsX/Y(pixel) (simulationmanager.cpp:434-440,485-501); otherwise orc_geometry */
+int orc_geometry_pixel(const OrcScenario* s, int pixel, OrcGeometry* g);
 
 /* sortAtoms: host filter, atom_sort_d, stable bucket fill.
  * out arrays need capacity n; block_start needs n_slices*bx*by+1. */

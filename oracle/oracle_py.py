@@ -35,6 +35,7 @@ class OrcScenario(C.Structure):
         ("limit_factor", C.c_double),
         ("full3d", C.c_int),
         ("full3d_integrals", C.c_int),
+        ("moving_stem_frame", C.c_int),
     ]
 
 
@@ -74,6 +75,7 @@ class OrcGeometry(C.Structure):
         ("stem_pixels", C.c_int * 2),
         ("full3d", C.c_int),
         ("full3d_integrals", C.c_int),
+        ("moving_frame", C.c_int),
     ]
 
 
@@ -97,6 +99,7 @@ def lib():
         L.orc_sigma.restype = C.c_double
         L.orc_sigma.argtypes = [C.c_double]
         L.orc_geometry.argtypes = [C.POINTER(OrcScenario), C.POINTER(OrcGeometry)]
+        L.orc_geometry_pixel.argtypes = [C.POINTER(OrcScenario), C.c_int, C.POINTER(OrcGeometry)]
         L.orc_sort_atoms.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, C.c_int, dp, dp, dp,
                                      ip, ip, ip, ip, ip, ip]
         L.orc_bessk0.restype = C.c_double
@@ -190,6 +193,7 @@ def scenario(**kw) -> OrcScenario:
     s.limit_factor = kw.get("limit_factor", 2.0 / 3.0)
     s.full3d = int(kw.get("full3d", 0))
     s.full3d_integrals = kw.get("full3d_integrals", 20)
+    s.moving_stem_frame = int(kw.get("moving_stem_frame", 0))
     return s
 
 
@@ -197,6 +201,14 @@ def geometry(s: OrcScenario) -> OrcGeometry:
     g = OrcGeometry()
     if lib().orc_geometry(C.byref(s), C.byref(g)) != 0:
         raise ValueError("orc_geometry rejected the scenario")
+    return g
+
+
+def geometry_pixel(s: OrcScenario, pixel: int) -> OrcGeometry:
+    """The geometry of STEM pixel `pixel`'s job (its own field under the moving frame)."""
+    g = OrcGeometry()
+    if lib().orc_geometry_pixel(C.byref(s), int(pixel), C.byref(g)) != 0:
+        raise ValueError("orc_geometry_pixel rejected the scenario or pixel")
     return g
 
 

@@ -129,6 +129,10 @@ def test_settings_json_follows_cltem_schema():
         assert "val" in st["microscope"]["aberrations"][k]
     for k in ("C12", "C21", "C23", "C32", "C34", "C41", "C43", "C45", "C52", "C54", "C56"):
         assert set(st["microscope"]["aberrations"][k]) == {"mag", "ang", "units"}
+    assert st["stem"]["static area"]["enabled"] is True
+    mv = _settings("--mode", "stem", "--srtio3", "3,3,2", "--res", "256", "--det", "ADF,60,200",
+                   "--stem-pixels", "8", "--moving-frame", "--dump-settings", "ADF")
+    assert mv["stem"]["static area"]["enabled"] is False
     # CBED diffraction: imageReturned drops the objective-lens entries and the aberrations
     cb = _settings("--mode", "cbed", "--srtio3", "2,2,2", "--res", "256", "--area", "3,3,4,4",
                    "--plasmon", "50,1.5,12,2", "--dump-settings", "Diff")
@@ -200,6 +204,40 @@ def test_driver_stem_phonon_average_matches_oracle(tmp_path, oracle):
         srt = oracle.sort_atoms(og, s.x, s.y, s.z, s.Z)
         tr = oracle.transmission(og, srt, p.values, 0, 3)
         acc += oracle.run_stem(og, tr, oracle.propagator(og), pos, np.array([[60, 200, 0, 0], [0, 15, 0, 0.]]))
+    acc /= 2
+    for d, name in enumerate(("ADF", "BF")):
+        got = out[name]
+        assert np.max(np.abs(got - acc[:, d]) / np.abs(acc[:, d])) <= 1e-3, name
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not have_driver(), reason="mse_run not built")
+def test_driver_stem_moving_frame_matches_oracle(tmp_path, oracle):
+    """The static area off ("static area enabled": false): every pixel job in its own field
+    with its own transmission functions (currentAreaBase(pixel), simulationmanager.cpp:485-501),
+    averaged over two phonon configurations, against the FP64 oracle run pixel by pixel."""
+    prefix = str(tmp_path / "stem_moving")
+    x0, x1 = 2 * 3.905, 3 * 3.905
+    subprocess.run([RUN, "--mode", "stem", "--srtio3", "5,5,3", "--res", "256", "--dz", "1.9525",
+                    "--area", f"{x0},{x1},{x0},{x1}", "--stem-pixels", "4", "--det", "ADF,60,200",
+                    "--det", "BF,0,15", "--alpha", "20", "--batch", "5", "--phonons", "2", "--moving-frame",
+                    "--urms", "0.08", "--seed", "99", "--params", os.path.join(PARAMS, "kirkland.dat"),
+                    "--out", prefix], check=True)
+    meta, out = _load(prefix)
+    assert meta["jobs"] == 2
+    w = wl.stem_srtio3(nx=5, ny=5, nz=3, resolution=256, pixels=4, window_cells=(2, 3))
+    s_or = oracle.scenario(**dict(w["scenario"], moving_stem_frame=1))
+    p = cltem_amd.load_params("kirkland")
+    pos = wl.stem_positions(oracle.geometry(s_or))
+    acc = np.zeros((16, 2))
+    for c in range(2):
+        s = w["specimen"].displaced(0.08, 99, c)
+        for px in range(16):
+            gp = oracle.geometry_pixel(s_or, px)
+            srt = oracle.sort_atoms(gp, s.x, s.y, s.z, s.Z)
+            tr = oracle.transmission(gp, srt, p.values, 0, 3)
+            acc[px] += oracle.run_stem(gp, tr, oracle.propagator(gp), pos[px:px + 1],
+                                       np.array([[60, 200, 0, 0], [0, 15, 0, 0.]]))[0]
     acc /= 2
     for d, name in enumerate(("ADF", "BF")):
         got = out[name]
