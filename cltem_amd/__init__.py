@@ -410,3 +410,36 @@ class Engine:
 
     def synchronize(self):
         _check(self.lib.mse_synchronize(self.h))
+
+
+class EngineGroup:
+    """Engines of one device (one per grid size) whose whole slice loops run as one graph
+    launch (mse_group_multislice).  Close the group before its engines."""
+
+    def __init__(self, engines):
+        self.lib = _abi.load()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = C.c_void_p()
+        _check(self.lib.mse_group_create(arr, len(self.engines), C.byref(h)))
+        self.h = h
+
+    def multislice(self):
+        _check(self.lib.mse_group_multislice(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.mse_group_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -168,6 +168,9 @@ SIGNATURES = [
     ("mse_exit_intensity_sum", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_diffraction_sum", C.c_int, [_eng, _ip, C.c_int, C.c_int, _dp]),
     ("mse_multislice_graph", C.c_int, [_eng, C.c_int, C.c_int]),
+    ("mse_group_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    ("mse_group_multislice", C.c_int, [C.c_void_p]),
+    ("mse_group_destroy", C.c_int, [C.c_void_p]),
     ("mse_time_sweeps", C.c_int, [_eng, C.c_int, C.c_int, _dp, _dp]),
     ("mse_set_focus_spread", C.c_int, [_eng, _dp, C.c_int]),
     ("mse_set_reference_offset", C.c_int, [_eng, C.c_int, C.c_double, C.c_double]),

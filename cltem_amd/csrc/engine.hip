@@ -1178,6 +1178,46 @@ struct LoopGraphKey {
     const void *waves, *cfg_ptrs, *cfg_nslices, *wave_cfg, *tw, *px, *py, *wave_prop, *lim, *mix_key;
 };
 
+}  // extern "C"
+
+namespace mse_rt {
+// The key of a captured slice loop over [first, first + count) for every wave; plans the
+// per-slice wave sets (host work and uploads that must happen before a capture).
+std::vector<unsigned char> loop_graph_key(mse_engine* e, int first, int count) {
+    LoopGraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.res = e->res;
+    k.nw = e->n_waves;
+    k.first = first;
+    k.count = count;
+    k.row_var = e->row_var;
+    k.col_var = e->col_var;
+    k.num_cus = e->num_cus;
+    k.waves = e->waves.p;
+    k.cfg_ptrs = e->cfg_ptrs.p;
+    k.cfg_nslices = e->cfg_nslices.p;
+    k.wave_cfg = e->wave_cfg.p;
+    k.tw = e->tw.p;
+    k.px = e->px.p;
+    k.py = e->py.p;
+    k.wave_prop = e->any_beam ? e->wave_prop.p : nullptr;
+    k.mix_first = e->mix_first;
+    k.mix_sets = e->mix_sets;
+    k.mix_seed = e->mix_seed;
+    k.mix_key = e->mix_key.p;
+    k.lim = e->prop_lim.p;
+    std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
+    active_waves(e, e->n_waves);  // the per-slice wave sets are baked into the graph
+    const void* mp = e->wave_map.p;
+    key.insert(key.end(), (const unsigned char*)&mp, (const unsigned char*)&mp + sizeof(mp));
+    key.insert(key.end(), (const unsigned char*)e->map_key.data(),
+               (const unsigned char*)(e->map_key.data() + e->map_key.size()));
+    return key;
+}
+}  // namespace mse_rt
+
+extern "C" {
+
 int mse_multislice_graph(mse_engine* e, int first, int count) {
     return run_guarded("mse_multislice_graph", e, [&]() {
         REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
@@ -1189,34 +1229,7 @@ int mse_multislice_graph(mse_engine* e, int first, int count) {
             multislice(e, e->n_waves, first, count, false, nullptr);
             return;
         }
-        LoopGraphKey k;
-        std::memset(&k, 0, sizeof(k));
-        k.res = e->res;
-        k.nw = e->n_waves;
-        k.first = first;
-        k.count = count;
-        k.row_var = e->row_var;
-        k.col_var = e->col_var;
-        k.num_cus = e->num_cus;
-        k.waves = e->waves.p;
-        k.cfg_ptrs = e->cfg_ptrs.p;
-        k.cfg_nslices = e->cfg_nslices.p;
-        k.wave_cfg = e->wave_cfg.p;
-        k.tw = e->tw.p;
-        k.px = e->px.p;
-        k.py = e->py.p;
-        k.wave_prop = e->any_beam ? e->wave_prop.p : nullptr;
-        k.mix_first = e->mix_first;
-        k.mix_sets = e->mix_sets;
-        k.mix_seed = e->mix_seed;
-        k.mix_key = e->mix_key.p;
-        k.lim = e->prop_lim.p;
-        std::vector<unsigned char> key((unsigned char*)&k, (unsigned char*)&k + sizeof(k));
-        active_waves(e, e->n_waves);  // the per-slice wave sets are baked into the graph
-        const void* mp = e->wave_map.p;
-        key.insert(key.end(), (const unsigned char*)&mp, (const unsigned char*)&mp + sizeof(mp));
-        key.insert(key.end(), (const unsigned char*)e->map_key.data(),
-                   (const unsigned char*)(e->map_key.data() + e->map_key.size()));
+        const std::vector<unsigned char> key = loop_graph_key(e, first, count);
         if (!e->loop_exec || key != e->loop_key) {
             if (e->loop_exec) (void)hipGraphExecDestroy(e->loop_exec);
             e->loop_exec = nullptr;
@@ -1240,6 +1253,137 @@ int mse_multislice_graph(mse_engine* e, int first, int count) {
             e->loop_key = key;
         }
         HIPX(hipGraphLaunch(e->loop_exec, e->stream));
+    });
+}
+
+}  // extern "C"
+
+// Engines of one device whose slice loops run as one graph (mixed grid sizes).
+struct mse_group {
+    std::vector<mse_engine*> engines;
+    int device = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> join, pre;  // per engine: capture join, launch-time dependency
+};
+
+namespace mse_rt {
+void destroy_group_events(mse_group* g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+    for (hipEvent_t ev : g->join) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : g->pre) (void)hipEventDestroy(ev);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->done) (void)hipEventDestroy(g->done);
+    g->join.clear();
+    g->pre.clear();
+    g->fork = g->done = nullptr;
+}
+}  // namespace mse_rt
+
+extern "C" {
+
+int mse_group_create(mse_engine* const* engines, int n, mse_group** out) {
+    return run_guarded("mse_group_create", nullptr, [&]() {
+        REQUIRE(engines && out && n >= 1, MSE_ERR_ARG, "NULL argument or empty group");
+        for (int i = 0; i < n; ++i) {
+            REQUIRE(engines[i], MSE_ERR_ARG, "NULL engine in the group");
+            REQUIRE(engines[i]->device == engines[0]->device, MSE_ERR_ARG, "a group's engines share one device");
+            for (int j = 0; j < i; ++j) REQUIRE(engines[j] != engines[i], MSE_ERR_ARG, "an engine appears twice");
+        }
+        auto* g = new mse_group();
+        g->engines.assign(engines, engines + n);
+        g->device = engines[0]->device;
+        try {
+            DeviceGuard dg(g->device);
+            HIPX(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+            HIPX(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+            for (int i = 0; i < n; ++i) {
+                hipEvent_t a = nullptr, b = nullptr;
+                HIPX(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+                g->join.push_back(a);
+                HIPX(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+                g->pre.push_back(b);
+            }
+        } catch (...) {
+            destroy_group_events(g);
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int mse_group_destroy(mse_group* g) {
+    if (!g) return MSE_OK;
+    return run_guarded("mse_group_destroy", nullptr, [&]() {
+        DeviceGuard dg(g->device);
+        (void)hipStreamSynchronize(g->engines[0]->stream);
+        destroy_group_events(g);
+        delete g;
+    });
+}
+
+int mse_group_multislice(mse_group* g) {
+    return run_guarded("mse_group_multislice", nullptr, [&]() {
+        REQUIRE(g, MSE_ERR_ARG, "group is NULL");
+        DeviceGuard dg(g->device);
+        const int n = (int)g->engines.size();
+        std::vector<unsigned char> key;
+        bool eager = false;
+        for (mse_engine* e : g->engines) {
+            check_waves(e);
+            check_trans(e);
+            eager = eager || e->prof;
+            const std::vector<unsigned char> k = loop_graph_key(e, 0, e->g.n_slices);
+            key.insert(key.end(), (const unsigned char*)&e, (const unsigned char*)&e + sizeof(e));
+            key.insert(key.end(), k.begin(), k.end());
+        }
+        if (eager) {  // per-kernel profiling events need eager launches on each engine's stream
+            for (mse_engine* e : g->engines) multislice(e, e->n_waves, 0, e->g.n_slices, false, nullptr);
+            return;
+        }
+        hipStream_t s0 = g->engines[0]->stream;
+        if (!g->exec || key != g->key) {
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            g->exec = nullptr;
+            g->key.clear();
+            hipGraph_t gr = nullptr;
+            HIPX(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+            try {
+                // every other engine's stream joins the capture, runs its own slice loop as a
+                // branch of the graph, and joins back into the first stream
+                HIPX(hipEventRecord(g->fork, s0));
+                for (int i = 1; i < n; ++i) HIPX(hipStreamWaitEvent(g->engines[i]->stream, g->fork, 0));
+                for (mse_engine* e : g->engines) multislice(e, e->n_waves, 0, e->g.n_slices, false, nullptr);
+                for (int i = 1; i < n; ++i) {
+                    HIPX(hipEventRecord(g->join[i], g->engines[i]->stream));
+                    HIPX(hipStreamWaitEvent(s0, g->join[i], 0));
+                }
+            } catch (...) {  // leave the streams out of capture mode for later calls
+                (void)hipStreamEndCapture(s0, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            HIPX(hipStreamEndCapture(s0, &gr));
+            hipError_t st = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (st != hipSuccess) {
+                g->exec = nullptr;
+                throw Error(MSE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(st));
+            }
+            g->key = key;
+        }
+        // the graph starts after the work queued on every engine (probe set-ups) and every
+        // engine's later work waits for it
+        for (int i = 1; i < n; ++i) {
+            HIPX(hipEventRecord(g->pre[i], g->engines[i]->stream));
+            HIPX(hipStreamWaitEvent(s0, g->pre[i], 0));
+        }
+        HIPX(hipGraphLaunch(g->exec, s0));
+        HIPX(hipEventRecord(g->done, s0));
+        for (int i = 1; i < n; ++i) HIPX(hipStreamWaitEvent(g->engines[i]->stream, g->done, 0));
     });
 }
 

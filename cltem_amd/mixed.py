@@ -5,21 +5,20 @@ clTEM runs one simulation per job on one grid (SimulationCbed::simulate,
 microscope/simulationcbed.cpp:122-210, merged by SimulationManager::updateImages,
 simulationmanager.cpp:274-350).  Here every instance of one grid size shares an engine:
 the engine's geometry is that of the thickest instance of the size and thinner instances
-get their own z geometry (mse_set_config_geometry).  Their waves, ordered by thickness,
-go through one graph-captured slice loop in which each wave stops at its own exit
-surface.  Each size has its own engine, stream and autotuned kernel variants, and the
-sizes run concurrently from one host thread each (the engine calls release the
-interpreter lock).
+get their own z geometry (mse_set_config_geometry); their waves, ordered by thickness, stop
+at their own exit surfaces.  The engines of all sizes form one group (mse_group_*), and the
+slice loops of every size run from one captured graph launched once per job: each size's
+kernels (its own FFT plan and autotuned variants) are a branch of that graph, and the
+branches run concurrently on the one GPU.
 """
 from __future__ import annotations
 
-import threading
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Engine, load_params, make_probe, make_scenario, plan_geometry
+from . import Engine, EngineGroup, load_params, make_probe, make_scenario, plan_geometry
 
 
 @dataclass
@@ -38,112 +37,136 @@ class SizeGroupResult:
     resolution: int
     instances: list
     tuning: dict | None
-    build_s: float
-    loop_s: float                # last timed repetition (probe init, slice loop, FP64 sums)
-    wave_slices: int             # slices actually run per repetition, summed over waves
-    t_start: float = 0.0         # perf_counter at the start / end of the timed repetitions
-    t_end: float = 0.0
+    wave_slices: int             # slices actually run per job, summed over the size's waves
     patterns: dict = field(default_factory=dict)  # instance index -> mean diffraction (res, res)
 
 
-def _run_group(res, items, params_name, device, autotune, graph, warmup, repeats, barrier, out, errors):
-    try:
-        _run_group_body(res, items, params_name, device, autotune, graph, warmup, repeats, barrier, out)
-    except Exception as ex:  # reported by run_mixed_cbed
-        errors.append((res, ex))
-        if barrier is not None:
-            barrier.abort()
+@dataclass
+class MixedTiming:
+    setup_s: float               # engines, geometries, first builds and autotuning (once)
+    job_s: float                 # one whole job: binning, potentials, probes, slice loops, sums
+    build_s: float               # of which binning and potentials
+    loop_s: float                # of which probe set-up, the group's slice loops and FP64 sums
+    wave_slices: int             # per job, all sizes
 
 
-def _run_group_body(res, items, params_name, device, autotune, graph, warmup, repeats, barrier, out):
-    eng = Engine(device)
-    p = load_params(params_name)
-    geoms = [plan_geometry(make_scenario(**inst.scenario)) for _, inst in items]
-    # waves ordered by thickness: each slice's active waves are then a prefix of the batch
-    order = sorted(range(len(items)), key=lambda k: -geoms[k].n_slices)
-    items = [items[k] for k in order]
-    geoms = [geoms[k] for k in order]
-    eng.set_geometry(geoms[0])
-    eng.set_parameterisation(p)
-    t0 = time.perf_counter()
-    wave_cfg, wave_group = [], []
-    c = 0
-    for gi, ((idx, inst), g) in enumerate(zip(items, geoms)):
-        for ph in range(inst.n_configs):
-            if gi != 0:
-                eng.set_config_geometry(c, g)
-            d = inst.specimen.displaced(inst.u_rms, inst.seed, ph)
-            eng.sort_atoms(c, d.x, d.y, d.z, d.Z, want_ids=False)
-            eng.build_transmission(c)
-            wave_cfg.append(c)
-            wave_group.append(gi)
-            c += 1
-    eng.synchronize()
-    build_s = time.perf_counter() - t0
-    tuning = eng.autotune(0, min(len(wave_cfg), 64), reps=3) if autotune else None
-    eng.set_wave_count(len(wave_cfg))
-    eng.set_wave_configs(wave_cfg)
-    probes = [make_probe(inst.alpha_mrad) for _, inst in items]
+class _SizeEngine:
+    """The instances of one grid size on one engine, waves ordered by thickness."""
 
-    def one_pass():
+    def __init__(self, res, items, params, device):
+        self.res = res
+        self.eng = Engine(device)
+        geoms = [plan_geometry(make_scenario(**inst.scenario)) for _, inst in items]
+        order = sorted(range(len(items)), key=lambda k: -geoms[k].n_slices)
+        self.items = [items[k] for k in order]
+        self.geoms = [geoms[k] for k in order]
+        self.eng.set_geometry(self.geoms[0])
+        self.eng.set_parameterisation(params)
+        self.wave_cfg, self.wave_group, self.jobs = [], [], []
+        c = 0
+        for gi, ((idx, inst), g) in enumerate(zip(self.items, self.geoms)):
+            for ph in range(inst.n_configs):
+                if gi != 0:
+                    self.eng.set_config_geometry(c, g)
+                # the displaced coordinates of every configuration, generated once
+                self.jobs.append((c, inst.specimen.displaced(inst.u_rms, inst.seed, ph)))
+                self.wave_cfg.append(c)
+                self.wave_group.append(gi)
+                c += 1
+        self.probes = [make_probe(inst.alpha_mrad) for _, inst in self.items]
+        self.wave_slices = sum(inst.n_configs * g.n_slices for (_, inst), g in zip(self.items, self.geoms))
+
+    def sort(self):
+        for c, d in self.jobs:
+            self.eng.sort_atoms(c, d.x, d.y, d.z, d.Z, want_ids=False)
+
+    def build(self):
+        for c, _ in self.jobs:
+            self.eng.build_transmission(c)
+
+    def set_waves(self):
+        self.eng.set_wave_count(len(self.wave_cfg))
+        self.eng.set_wave_configs(self.wave_cfg)
+
+    def init_probes(self):
         w0 = 0
-        for (idx, inst), pr in zip(items, probes):
+        for (idx, inst), pr in zip(self.items, self.probes):
             pos = np.tile(np.asarray(inst.position, dtype=np.float64), (inst.n_configs, 1))
-            eng.init_probes_range(w0, pos, pr)
+            self.eng.init_probes_range(w0, pos, pr)
             w0 += inst.n_configs
-        eng.multislice(graph=graph)
-        return eng.diffraction_sum(wave_group, len(items))
 
-    for _ in range(warmup):
-        one_pass()
-    eng.synchronize()
-    if barrier is not None:
-        barrier.wait()
-    t_start = time.perf_counter()
-    loop_s = 0.0
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        sums = one_pass()
-        loop_s = time.perf_counter() - t0
-    t_end = time.perf_counter()
-    wave_slices = sum(inst.n_configs * g.n_slices for (_, inst), g in zip(items, geoms))
-    r = SizeGroupResult(res, [idx for idx, _ in items], tuning, build_s, loop_s, wave_slices, t_start, t_end)
-    for gi, (idx, inst) in enumerate(items):
-        r.patterns[idx] = sums[gi] / inst.n_configs
-    eng.close()
-    out[res] = r
+    def sums(self):
+        return self.eng.diffraction_sum(self.wave_group, len(self.items))
 
 
 def run_mixed_cbed(instances, params_name="kirkland", device=0, autotune=True, graph=True, warmup=0,
-                   repeats=1, concurrent=True):
+                   repeats=1, concurrent=True, timing=None):
     """Phonon-averaged CBED patterns |Psi(k)|^2 (fft-shifted, FP64) of every instance.
-    Each size group builds its transmission functions, runs `warmup` untimed passes, then
-    `repeats` timed passes (probe init, slice loop, FP64 sums): concurrently with the other
-    groups after a common barrier, or (concurrent=False) one group after another, each with
-    the whole GPU.  Returns (patterns in instance order, per-size results)."""
-    groups: dict = {}
+
+    After the set-up (one engine per grid size, the displaced coordinates of every
+    configuration, first builds, autotuning), `warmup` untimed and `repeats` timed whole jobs
+    run: binning and potentials of every instance and configuration, probe set-up, the slice
+    loops of all sizes and the FP64 sums.  With
+    concurrent=True the slice loops of all sizes are one graph launch (mse_group_multislice);
+    otherwise each size runs its own loop after the previous one (graph per size when graph=True,
+    eager launches otherwise).  When `timing` is a list, a MixedTiming of the last timed job is
+    appended.  Returns (patterns in instance order, per-size results)."""
+    t_setup = time.perf_counter()
+    by_size: dict = {}
     for i, inst in enumerate(instances):
-        groups.setdefault(inst.scenario["resolution"], []).append((i, inst))
-    out, errors = {}, []
-    if concurrent:
-        barrier = threading.Barrier(len(groups)) if len(groups) > 1 else None
-        threads = [threading.Thread(target=_run_group, args=(res, items, params_name, device, autotune, graph,
-                                                              warmup, max(1, repeats), barrier, out, errors))
-                   for res, items in sorted(groups.items())]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-    else:
-        for res, items in sorted(groups.items()):
-            _run_group(res, items, params_name, device, autotune, graph, warmup, max(1, repeats), None, out, errors)
-            if errors:
-                break
-    if errors:
-        real = [e for e in errors if not isinstance(e[1], threading.BrokenBarrierError)] or errors
-        raise RuntimeError(f"size group {real[0][0]} failed: {real[0][1]}") from real[0][1]
-    patterns = [None] * len(instances)
-    for r in out.values():
-        for idx, pat in r.patterns.items():
-            patterns[idx] = pat
-    return patterns, out
+        by_size.setdefault(inst.scenario["resolution"], []).append((i, inst))
+    params = load_params(params_name)
+    sizes = [_SizeEngine(res, items, params, device) for res, items in sorted(by_size.items())]
+    tuning = {}
+    for s in sizes:
+        s.sort()
+        s.build()
+        if autotune:
+            tuning[s.res] = s.eng.autotune(0, min(len(s.wave_cfg), 64), reps=3)
+        s.set_waves()
+    for s in sizes:
+        s.eng.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    group = EngineGroup([s.eng for s in sizes]) if concurrent else None
+
+    def job():
+        t0 = time.perf_counter()
+        for s in sizes:  # binning synchronises each engine; the potential launches are queued
+            s.sort()
+        for s in sizes:
+            s.build()
+        for s in sizes:
+            s.eng.synchronize()
+        t1 = time.perf_counter()
+        for s in sizes:
+            s.init_probes()
+        if group is not None:
+            group.multislice()
+        else:
+            for s in sizes:
+                s.eng.multislice(graph=graph)
+                s.eng.synchronize()
+        out = [s.sums() for s in sizes]
+        t2 = time.perf_counter()
+        return out, t2 - t0, t1 - t0, t2 - t1
+
+    try:
+        for _ in range(warmup):
+            job()
+        for _ in range(max(1, repeats)):
+            sums, job_s, build_s, loop_s = job()
+    finally:
+        if group is not None:
+            group.close()
+    wave_slices = sum(s.wave_slices for s in sizes)
+    if timing is not None:
+        timing.append(MixedTiming(t_setup, job_s, build_s, loop_s, wave_slices))
+    results, patterns = {}, [None] * len(instances)
+    for s, sm in zip(sizes, sums):
+        r = SizeGroupResult(s.res, [idx for idx, _ in s.items], tuning.get(s.res), s.wave_slices)
+        for gi, (idx, inst) in enumerate(s.items):
+            r.patterns[idx] = sm[gi] / inst.n_configs
+            patterns[idx] = r.patterns[idx]
+        results[s.res] = r
+        s.eng.close()
+    return patterns, results

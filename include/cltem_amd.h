@@ -226,6 +226,18 @@ int mse_multislice(mse_engine* e, int first, int count);
  * Captured on first use and again whenever the wave set, slice range, kernel variants or
  * engine buffers change; results are identical to mse_multislice. */
 int mse_multislice_graph(mse_engine* e, int first, int count);
+/* ---- mixed grid sizes in one launch.  No clTEM counterpart (clTEM runs one simulation
+ *      per grid).  A group holds engines of one device, typically one per grid size;
+ *      mse_group_multislice runs every engine's whole slice loop (all its slices and waves,
+ *      as mse_multislice) from one captured hipGraph launched once on the first engine's
+ *      stream: each engine's kernels form a branch of the graph and the branches run
+ *      concurrently.  The launch waits for the work already queued on every engine, and
+ *      every engine's later calls wait for it.  Captured again when any engine's wave set,
+ *      slice plan, kernel variants or buffers change.  Destroy a group before its engines. */
+typedef struct mse_group mse_group;
+int mse_group_create(mse_engine* const* engines, int n, mse_group** out);
+int mse_group_multislice(mse_group* g);
+int mse_group_destroy(mse_group* g);
 /* SimulationStem::getStemPixel for every wave and detector (microscope/simulationstem.cpp:42-131):
  * out[wave*n_det + d] = (sqrt2 ? sqrt(2) : 1) * sum over the annulus of |Psi_shifted|^2, FP64 */
 int mse_stem_detect(mse_engine* e, const mse_detector* dets, int n_det, int sqrt2, double* out);

@@ -245,15 +245,19 @@ def test_thinner_specimens_stop_at_their_exit_surface(engine, oracle, order):
 
 
 def test_mixed_size_cbed_driver(oracle):
-    """cltem_amd.mixed: instances of two grid sizes and several thicknesses and convergence
-    angles, one engine per size running concurrently, each size graph-captured with its
-    own autotuned kernels; every phonon-averaged pattern against the FP64 oracle."""
+    """cltem_amd.mixed: instances of all four grid sizes (256, 384, 512, 768) and several
+    thicknesses and convergence angles, the slice loops of every size in one graph launch
+    (each size with its own autotuned kernels), two timed whole jobs; every phonon-averaged
+    pattern against the FP64 oracle."""
     from cltem_amd.mixed import run_mixed_cbed
-    inst = wl.mixed_cbed(n_instances=4, cells_xy=3, thickness_cells=(2, 5), sizes=(256, 384), n_configs=2)
-    inst[0].scenario["resolution"] = 256
-    inst[1].scenario["resolution"] = 384
-    patterns, groups = run_mixed_cbed(inst)
-    assert set(groups) <= {256, 384} and len(groups) == 2
+    inst = wl.mixed_cbed(n_instances=6, cells_xy=3, thickness_cells=(2, 5), sizes=(256, 384, 512, 768),
+                         n_configs=2)
+    for i, res in enumerate((256, 384, 512, 768)):
+        inst[i].scenario["resolution"] = res
+    timing = []
+    patterns, groups = run_mixed_cbed(inst, warmup=1, repeats=1, timing=timing)
+    assert set(groups) == {256, 384, 512, 768}
+    assert timing[0].job_s > 0 and timing[0].wave_slices == sum(r.wave_slices for r in groups.values())
     p = cltem_amd.load_params("kirkland")
     for i, it in enumerate(inst):
         og = oracle.geometry(oracle.scenario(**it.scenario))
