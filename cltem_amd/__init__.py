@@ -366,6 +366,10 @@ class Engine:
         return accum
 
     # -- tuning -------------------------------------------------------------------------
+    def set_potential_tables(self, enable: bool = True):
+        """Tabulated potentials (default) or the formulas evaluated directly, from the next build."""
+        _check(self.lib.mse_set_potential_tables(self.h, int(enable)))
+
     def set_kernel_variant(self, kernel: int, variant: int):
         _check(self.lib.mse_set_kernel_variant(self.h, kernel, variant))
 
@@ -443,3 +447,29 @@ class EngineGroup:
             self.close()
         except Exception:
             pass
+
+
+# layout of the tabulated potentials (potential_table.hpp)
+TAB_OCTAVE_LO, TAB_OCTAVE_HI, TAB_SUB_BITS = -20, 8, 5
+TAB_BUCKETS = (TAB_OCTAVE_HI - TAB_OCTAVE_LO) << TAB_SUB_BITS
+TAB_MANT_BITS = 23 - TAB_SUB_BITS
+
+
+def potential_table(p: Parameterisation, Z: int, full3d: bool = False) -> np.ndarray:
+    """The engine's tabulated potential of element Z: (TAB_BUCKETS, 4) cubic coefficients per
+    bucket of r^2 (see mse_potential_table), evaluated by potential_table_eval."""
+    out = np.zeros(4 * TAB_BUCKETS, dtype=np.float32)
+    v = np.ascontiguousarray(p.values, dtype=np.float64)
+    _check(_abi.load().mse_potential_table(p.form, int(full3d), _dp(v), len(v), p.i_per_atom, Z,
+                                           out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out.reshape(TAB_BUCKETS, 4)
+
+
+def potential_table_eval(tab: np.ndarray, r2) -> np.ndarray:
+    """V at squared distances r2 from a table of potential_table, as the kernels evaluate it
+    (float32 r^2 bits pick the bucket and its fraction t; cubic in t)."""
+    bits = np.asarray(r2, dtype=np.float32).view(np.uint32).astype(np.int64)
+    idx = np.clip((bits >> TAB_MANT_BITS) - ((127 + TAB_OCTAVE_LO) << TAB_SUB_BITS), 0, TAB_BUCKETS - 1)
+    t = (bits & ((1 << TAB_MANT_BITS) - 1)).astype(np.float32) * np.float32(1.0 / (1 << TAB_MANT_BITS))
+    c = tab[idx].astype(np.float32)
+    return ((c[:, 3] * t + c[:, 2]) * t + c[:, 1]) * t + c[:, 0]

@@ -180,6 +180,8 @@ SIGNATURES = [
                                  C.POINTER(MseDetector), C.c_int, C.c_int, C.c_int, _dp]),
     ("mse_stem_batch_frames", C.c_int, [_eng, C.c_int, _dp, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
                                         C.POINTER(MseDetector), C.c_int, C.c_int, _dp]),
+    ("mse_set_potential_tables", C.c_int, [_eng, C.c_int]),
+    ("mse_potential_table", C.c_int, [C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _fp]),
     ("mse_set_kernel_variant", C.c_int, [_eng, C.c_int, C.c_int]),
     ("mse_autotune", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _ip]),
     ("mse_bandwidth_probe", C.c_int, [_eng, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),

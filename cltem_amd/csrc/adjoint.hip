@@ -94,7 +94,10 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         e->adj_w.ensure(plane);
         HIPX(hipMemcpyAsync(e->adj_w.p, dl_di, plane * sizeof(double), hipMemcpyHostToDevice, e->stream));
         e->adj_phase.ensure((size_t)std::max(n, 1) * N);
-        const mse::PotArgs pa = pot_args(e, c);
+        mse::PotArgs pa = pot_args(e, c);
+        // the phases from the same (tabulated or direct) potentials as the forward build
+        pa.vtab = potential_tables(e, c, g);
+        const bool tab = pa.vtab != nullptr;
         for (int s0 = 0; s0 < n; s0 += 16) {
             const int cs = std::min(16, n - s0);
             dim3 grid(e->res / 16, e->res / 16, cs);
@@ -102,17 +105,23 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
             launch(e, PK_POT, [&]() {
                 if (g.full3d) {
                     if (e->form == 0)
-                        mse::k_phase_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                        tab ? mse::k_phase_full3d<0, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0)
+                            : mse::k_phase_full3d<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
                     else if (e->form == 1)
-                        mse::k_phase_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                        tab ? mse::k_phase_full3d<1, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0)
+                            : mse::k_phase_full3d<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
                     else
-                        mse::k_phase_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
+                        tab ? mse::k_phase_full3d<2, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0)
+                            : mse::k_phase_full3d<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, pa, s0);
                 } else if (e->form == 0)
-                    mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
+                    tab ? mse::k_phase_projected<0, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0)
+                        : mse::k_phase_projected<0><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
                 else if (e->form == 1)
-                    mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
+                    tab ? mse::k_phase_projected<1, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0)
+                        : mse::k_phase_projected<1><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
                 else
-                    mse::k_phase_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
+                    tab ? mse::k_phase_projected<2, true><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0)
+                        : mse::k_phase_projected<2><<<grid, 256, 0, e->stream>>>(o, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, pa, s0);
             });
         }
         const size_t ng = 3 * (size_t)std::max(n_atoms, 1);

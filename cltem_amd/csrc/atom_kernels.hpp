@@ -1,5 +1,6 @@
 // Atom binning and transmission-function kernels (gfx950).
 #pragma once
+#include "potential_table.hpp"
 #include <hip/hip_runtime.h>
 
 namespace mse {
@@ -307,7 +308,21 @@ struct PotArgs {
     // moving STEM frames: when set, blockIdx.z selects a frame whose potential-grid origin is
     // (frame_xy[2z], frame_xy[2z+1]) and every frame builds the same slice `first`
     const double* frame_xy;
+    // tabulated potentials (potential_table.hpp), kBuckets cubic pieces per element from
+    // element 1; read by the kernels instantiated with TAB = true
+    const float4* vtab;
 };
+
+// V(r2) of one element from its table (r2 within the table's range; the bucket index is
+// clamped so that no r2 can read outside the element's table)
+__device__ __forceinline__ float vtab_eval(const float4* __restrict__ tz, float r2) {
+    const unsigned bits = __float_as_uint(r2);
+    int i = (int)(bits >> mse_tab::kMantBits) - ((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits);
+    i = min(max(i, 0), mse_tab::kBuckets - 1);
+    const float t = (float)(bits & ((1u << mse_tab::kMantBits) - 1u)) * (1.0f / (float)(1u << mse_tab::kMantBits));
+    const float4 c = tz[i];
+    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+}
 
 // The arguments of the plane blockIdx.z of a launch over slices [first, first + gridDim.z),
 // or over frames of slice `first` (frame_xy set); returns the slice.
@@ -347,7 +362,7 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
 // Projected potential sum (sumz) at one pixel of slice `slice`: one 16x16 pixel
 // tile per block; the atoms of each block row are staged through LDS in chunks of 256,
 // with no limit on how many a row holds.
-template <int FORM>
+template <int FORM, bool TAB>
 __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                  const int* __restrict__ aZ, const int* __restrict__ bsp,
                                                  const float* __restrict__ ptab, const PotArgs& a, int slice,
@@ -379,9 +394,12 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
             }
             __syncthreads();
             for (int l = 0; l < cnt; ++l) {
-                // the element's constants, loaded ahead of the distance arithmetic
-                float pt[kPotProjStride];
-                {
+                // the element's constants (or table), loaded ahead of the distance arithmetic
+                float pt[TAB ? 1 : kPotProjStride];
+                const float4* tz = nullptr;
+                if constexpr (TAB) {
+                    tz = a.vtab + (size_t)(sZ[l] - 1) * mse_tab::kBuckets;
+                } else {
                     const float* row = ptab + (size_t)(sZ[l] - 1) * kPotProjStride;
 #pragma unroll
                     for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
@@ -403,8 +421,12 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                 // rad = max(sqrt(r2), r_min) <= 8: certain for r2 <= 64; the double square root
                 // decides only the sliver just above 64.  The potential takes rad in float.
                 if (r2 <= 64.0000000001 && (r2 <= 64.0 || sqrt(r2) <= 8.0)) {
-                    const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
-                    sumz += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                    if constexpr (TAB) {
+                        sumz += (double)vtab_eval(tz, r2 < r_min2 ? r_minf * r_minf : (float)r2);
+                    } else {
+                        const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
+                        sumz += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                    }
                 }
             }
         }
@@ -415,7 +437,7 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
 // Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
 // (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
 // slice (relative to first).
-template <int FORM>
+template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ ptab,
@@ -424,7 +446,7 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     __shared__ int sZ[256];
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ);
+    const double sumz = projected_sumz<FORM, TAB>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
@@ -433,14 +455,14 @@ __global__ __launch_bounds__(256) void k_potential_projected(
 // The phase sigma V of the same slices, reduced to [-pi, pi] as phase_to_t reduces it, so
 // that sincosf of the stored float reproduces the transmission function's exp(i sigma V)
 // before the band limit (used by the adjoint).
-template <int FORM>
+template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_phase_projected(
     float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const int* __restrict__ aZ, const int* __restrict__ bsp, const float* __restrict__ ptab,
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ);
+    const double sumz = projected_sumz<FORM, TAB>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
@@ -451,7 +473,7 @@ __global__ __launch_bounds__(256) void k_phase_projected(
 // slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
 // kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Returns sumz at one
 // pixel of slice `slice` (16x16 pixel tile per block).
-template <int FORM>
+template <int FORM, bool TAB = false>
 __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                               const double* __restrict__ az, const int* __restrict__ aZ,
                                               const int* __restrict__ bsp, const float* __restrict__ tab3d,
@@ -495,9 +517,12 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                 for (int l = 0; l < cnt; ++l) {
                     double rad_x = im_pos_x - sx[l];
                     double rad_y = im_pos_y - sy[l];
-                    // the element's constants, loaded once per atom (scalar registers)
-                    float pz[kPot3dStride];
-                    {
+                    // the element's constants (scalar registers) or table, loaded once per atom
+                    float pz[TAB ? 1 : kPot3dStride];
+                    const float4* tz = nullptr;
+                    if constexpr (TAB) {
+                        tz = a.vtab + (size_t)(sZ[l] - 1) * mse_tab::kBuckets;
+                    } else {
                         const float* row = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
 #pragma unroll
                         for (int q = 0; q < kPot3dStride; ++q) pz[q] = row[q];
@@ -525,7 +550,11 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         for (int h = h0; h <= a.integrals; ++h) {
                             const float rz = fmaf(-(float)(h - h0), stepf, rz0);
                             const float r2 = fmaf(rz, rz, xy2f);
-                            const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
+                            float p1;
+                            if constexpr (TAB)
+                                p1 = vtab_eval(tz, r2 < r_min2f ? r_min2f : r2);
+                            else
+                                p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
                             if (h != 0) acc += p1 + p2f;
                             p2f = p1;
                         }
@@ -540,7 +569,12 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
                         if (xyrad2 <= 64.0 && rad_z <= 3.0) {
                             const double r2 = xyrad2 + rad_z * rad_z;
-                            const double p1 = (double)v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
+                            float p1f;
+                            if constexpr (TAB)
+                                p1f = vtab_eval(tz, r2 < r_min2 ? r_min2f : (float)r2);
+                            else
+                                p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
+                            const double p1 = (double)p1f;
                             sumz += (h != 0) * (p1 + p2) * 0.5;
                             p2 = p1;
                         }
@@ -552,7 +586,7 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     return sumz;
 }
 
-template <int FORM>
+template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_full3d(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
@@ -561,7 +595,7 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     __shared__ int sZ[256];
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ);
+    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);
@@ -569,14 +603,14 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
 }
 
 // The full-3D phase sst sigma sumz of the same slices, reduced to [-pi, pi] (used by the adjoint).
-template <int FORM>
+template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_phase_full3d(
     float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
     const float* __restrict__ tab3d, PotArgs a, int first) {
     __shared__ double sx[256], sy[256], sz[256];
     __shared__ int sZ[256];
-    const double sumz = full3d_sumz<FORM>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
+    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);

@@ -21,6 +21,7 @@
 #include "../../include/cltem_amd.h"
 #include "atom_kernels.hpp"
 #include "engine_internal.hpp"
+#include "potential_table.hpp"
 
 namespace mse_geom {
 void kgrid(int res, double real_scale, double* kx, double* ky, double* bandwidthkmax);
@@ -625,6 +626,32 @@ int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const doub
         e->i_count = i_per_atom;
         e->per_atom = per;
         e->max_Z = n_params / per;
+        // tabulated potentials: built per element on first use (potential_tables)
+        e->h_params.assign(params, params + n_params);
+        e->h_vtab.assign((size_t)2 * e->max_Z * mse_tab::kBuckets * 4, 0.f);
+        e->vtab_on.assign((size_t)2 * e->max_Z, 0);
+        e->vtab.ensure((size_t)2 * e->max_Z * mse_tab::kBuckets);
+    });
+}
+
+int mse_set_potential_tables(mse_engine* e, int enable) {
+    return run_guarded("mse_set_potential_tables", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        e->use_vtab = enable != 0;
+    });
+}
+
+int mse_potential_table(int form, int full3d, const double* params, int n_params, int i_per_atom, int Z,
+                        float* out) {
+    return run_guarded("mse_potential_table", nullptr, [&]() {
+        REQUIRE(params && out, MSE_ERR_ARG, "NULL argument");
+        REQUIRE(form >= 0 && form <= 2, MSE_ERR_ARG, "form must be 0 (Kirkland), 1 (Peng) or 2 (Lobato)");
+        REQUIRE(form == 0 ? i_per_atom == 3 : i_per_atom == 5, MSE_ERR_UNSUPPORTED,
+                "parameter layout expects 3 Kirkland or 5 Peng/Lobato terms");
+        const int per = i_per_atom * (form == 0 ? 4 : 2);
+        REQUIRE(n_params > 0 && n_params % per == 0 && Z >= 1 && Z <= n_params / per, MSE_ERR_ARG,
+                "atomic number outside the parameterisation");
+        mse_tab::build(form, full3d != 0, params, i_per_atom, Z, out);
     });
 }
 
@@ -711,6 +738,9 @@ int mse_sort_atoms(mse_engine* e, int cfg, const double* x, const double* y, con
         HIPX(hipStreamSynchronize(e->stream));
         if (n_in_range) *n_in_range = c.n_in_range;
         c.n_input = n;
+        c.has_Z.assign((size_t)std::max(e->max_Z, 0) + 1, 0);
+        for (int i = 0; i < n; ++i)
+            if (Z[i] >= 1 && Z[i] < (int)c.has_Z.size()) c.has_Z[Z[i]] = 1;
         ++c.version;
         c.sorted = true;
     });
@@ -739,7 +769,7 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         const int chunk = g.full3d ? 1 : 16;
         for (int s0 = 0; s0 < count; s0 += chunk) {
             const int cs = std::min(chunk, count - s0);
-            potential_planes(e, c, g.full3d, a, outp + (size_t)s0 * e->N, cs, first + s0);
+            potential_planes(e, c, g, a, outp + (size_t)s0 * e->N, cs, first + s0);
         }
         band_limit_planes(e, outp, count);
     });
@@ -1051,24 +1081,54 @@ void active_waves(mse_engine* e, int nw) {
     e->map_key = key;
 }
 
-void potential_planes(mse_engine* e, const ConfigData& c, bool full3d, const mse::PotArgs& a, float2* out,
-                      int planes, int first) {
+const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g) {
+    if (!e->use_vtab || e->max_Z <= 0 || e->h_params.empty()) return nullptr;
+    // the distances the kernels evaluate must lie inside the tables: r >= r_min = px / 4, and
+    // r^2 <= 64 + (depth reach)^2 with the full-3D depth below (load_z + 1) slices
+    const double r_min = 0.25 * g.real_scale;
+    if (r_min * r_min < std::ldexp(1.0, mse_tab::kOctaveLo)) return nullptr;
+    const double reach = (g.load_z + 2) * g.slice_thickness;
+    if (g.full3d && 64.0 + reach * reach >= std::ldexp(1.0, mse_tab::kOctaveHi)) return nullptr;
+    const int t = g.full3d ? 1 : 0;
+    const size_t per = (size_t)mse_tab::kBuckets * 4;
+    for (int Z = 1; Z <= e->max_Z && Z < (int)c.has_Z.size(); ++Z) {
+        if (!c.has_Z[Z] || e->vtab_on[(size_t)t * e->max_Z + Z - 1]) continue;
+        float* h = e->h_vtab.data() + ((size_t)t * e->max_Z + Z - 1) * per;
+        mse_tab::build(e->form, g.full3d != 0, e->h_params.data(), e->i_count, Z, h);
+        HIPX(hipMemcpyAsync(e->vtab.p + ((size_t)t * e->max_Z + Z - 1) * mse_tab::kBuckets, h,
+                            per * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        e->vtab_on[(size_t)t * e->max_Z + Z - 1] = 1;
+    }
+    return e->vtab.p + (size_t)t * e->max_Z * mse_tab::kBuckets;
+}
+
+void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g, const mse::PotArgs& args,
+                      float2* out, int planes, int first) {
     const dim3 grid(e->res / 16, e->res / 16, planes);
+    mse::PotArgs a = args;
+    a.vtab = potential_tables(e, c, g);
+    const bool tab = a.vtab != nullptr;
     launch(e, PK_POT, [&]() {
-        if (full3d) {
+        if (g.full3d) {
             if (e->form == 0)
-                mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                tab ? mse::k_potential_full3d<0, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first)
+                    : mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
             else if (e->form == 1)
-                mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                tab ? mse::k_potential_full3d<1, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first)
+                    : mse::k_potential_full3d<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
             else
-                mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                tab ? mse::k_potential_full3d<2, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first)
+                    : mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
         } else {
             if (e->form == 0)
-                mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<0, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
             else if (e->form == 1)
-                mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<1, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
             else
-                mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<2, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
         }
     });
 }
@@ -1711,7 +1771,7 @@ int mse_stem_batch_frames(mse_engine* e, int cfg, const double* pos_xy, const do
             }
             probes_into_waves(e, nb, probe);
             for (int s = 0; s < ns; ++s) {
-                potential_planes(e, c, g.full3d, pa, e->frame_trans.p, nb, s);
+                potential_planes(e, c, g, pa, e->frame_trans.p, nb, s);
                 band_limit_planes(e, e->frame_trans.p, nb);
                 row_pass(e, mse::ROW_IFFT_T_FFT, e->waves.p, e->waves.p, nb, e->N, ra);
                 if (s == ns - 1)

@@ -92,6 +92,7 @@ struct ConfigData {
     unsigned version = 0;   // bumped whenever the atoms or transmission functions change
     double ref_off[2] = {0.0, 0.0};  // potential grid origin shift (source-size incoherence)
     int n_in_range = 0;
+    std::vector<unsigned char> has_Z;  // atomic numbers present in the last sort (index Z)
     bool sorted = false;
     DevBuf<float2> trans;  // n_slices * N
     bool built = false;     // trans holds the transmission functions of the current geometry
@@ -126,6 +127,13 @@ struct mse_engine {
     DevBuf<float> params3d;  // per-element constants of the full-3D potentials (kPot3dStride per Z)
     DevBuf<float> params_proj;  // per-element constants of the projected potentials (kPotProjStride per Z)
     int form = -1, i_count = 0, per_atom = 0, max_Z = 0;
+    // tabulated potentials (potential_table.hpp): the parameterisation's values, and per
+    // table (0 projected, 1 full 3D) and element the host copy and whether it is on the device
+    std::vector<double> h_params;
+    std::vector<float> h_vtab;           // [2][max_Z][kBuckets][4]
+    std::vector<unsigned char> vtab_on;  // [2][max_Z]
+    DevBuf<float4> vtab;                 // [2][max_Z][kBuckets]
+    bool use_vtab = true;                // mse_set_potential_tables
     std::vector<ConfigData> configs;
     DevBuf<const float2*> cfg_ptrs;
     DevBuf<int> cfg_nslices;
@@ -241,10 +249,13 @@ int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, 
 void slice_step(mse_engine* e, float2* waves, int nw, int slice);
 // exp(i sigma V) of `planes` planes into out from the sorted atoms of c: slices
 // first..first+planes-1, or with a.frame_xy set one plane per frame of slice `first`
-void potential_planes(mse_engine* e, const ConfigData& c, bool full3d, const mse::PotArgs& a, float2* out,
-                      int planes, int first);
+void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g, const mse::PotArgs& a,
+                      float2* out, int planes, int first);
 // transmission band limit of `planes` planes in place: FFT, |k| <= 2/3 kmax, IFFT
 void band_limit_planes(mse_engine* e, float2* planes_p, int planes);
+// the tables of configuration c's elements for the geometry g (uploaded on first use), or
+// NULL when tabulation is off or the geometry's distances fall outside the tables' range
+const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g);
 // refresh wave_prop / any_beam after the waves' configurations or a configuration's beam changed
 void sync_wave_prop(mse_engine* e);
 

@@ -303,6 +303,20 @@ int mse_stem_batch_frames(mse_engine* e, int config, const double* pos_xy, const
                           int n_probes, int batch, const mse_probe* probe, const mse_detector* dets,
                           int n_det, int sqrt2, double* accum);
 
+/* ---- potential evaluation.  The potential kernels read each element's potential from a
+ *      table of cubic pieces in r^2 (32 per octave of r^2 from 2^-20 to 2^8 A^2, indexed by the
+ *      float bits of r^2), filled on the host from the reference formulas in double
+ *      (kernels/transmission_potentials_projected_d.cl:71-181,
+ *      kernels/transmission_potentials_full_3d_d.cl:57-134).  Geometries whose distances leave
+ *      that range (pixels below 0.004 A, full-3D depth reach above 13.8 A) evaluate the
+ *      formulas directly.  enable = 0 always evaluates them directly; either way the next
+ *      mse_build_transmission uses the choice.  No clTEM counterpart. ---- */
+int mse_set_potential_tables(mse_engine* e, int enable);
+/* host only: the table of element Z (1-based) as 4 floats (c0..c3 of V = ((c3 t + c2) t + c1) t
+ * + c0) per bucket into out[4 * 896], for form 0/1/2 (Kirkland, Peng, Lobato), full3d 0/1 */
+int mse_potential_table(int form, int full3d, const double* params, int n_params, int i_per_atom, int Z,
+                        float* out);
+
 /* ---- tuning: kernel variants of the slice sweeps.  kernel 0 = row pass (variants 0..6):
  *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at
  *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes,

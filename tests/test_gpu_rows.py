@@ -64,6 +64,32 @@ def test_full3d_amorphous(engine, oracle, params):
     ctem_compare(engine, oracle, g, og, p, o)
 
 
+@pytest.mark.parametrize("params,full3d", [("kirkland", 0), ("peng", 0), ("lobato", 0),
+                                           ("kirkland", 1), ("peng", 1), ("lobato", 1)])
+def test_tabulated_and_direct_potentials(engine, oracle, params, full3d):
+    """The tabulated potentials (default) and the formulas evaluated directly
+    (mse_set_potential_tables(e, 0)) give the same transmission functions, and both match the
+    FP64 oracle to rel-L2 1e-5 per slice."""
+    spec = sp.random_box({"Sr": 8, "Ti": 8, "O": 24}, (12.0, 12.0, 6.0), 1.5, seed=21)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.5, struct_min=tuple(lo),
+                struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]), full3d=full3d, full3d_integrals=10)
+    try:
+        g, og, p, o = prepare(engine, oracle, scen, spec, params)
+        tab = [engine.read_transmission(0, s) for s in range(g.n_slices)]
+        engine.set_potential_tables(False)
+        engine.build_transmission(0)
+        direct = [engine.read_transmission(0, s) for s in range(g.n_slices)]
+    finally:
+        engine.set_potential_tables(True)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    e_td = max(rel_l2(a, b) for a, b in zip(tab, direct))
+    e_t = max(rel_l2(a, b) for a, b in zip(tab, trans))
+    e_d = max(rel_l2(a, b) for a, b in zip(direct, trans))
+    print(f"{params} full3d={full3d}: tabulated vs direct {e_td:.2e}, vs FP64 {e_t:.2e} / {e_d:.2e}")
+    assert e_td <= 2e-6 and e_t <= 1e-5 and e_d <= 1e-5
+
+
 def test_beam_tilt(engine, oracle):
     w = wl.ctem_srtio3(3, 3, 5, resolution=256)
     scen = dict(w["scenario"], beam_tilt_mrad=8.0, beam_azimuth=0.7)
