@@ -123,6 +123,8 @@ def roofline(engine, g, pos, batch, probe, dets):
     name = max(kern, key=lambda k: kern[k][0])
     us, b = kern[name]
     achieved = b / (us * 1e-6) / 1e9
+    # the same access pattern with no arithmetic: what this part's memory system delivers to it
+    copy_us = engine.bandwidth_probe(1 if name == "column_pass" else 0, batch, reps=10)
     return {
         "kernel": f"{name} (512^2, B={batch})",
         "bound": "hbm",
@@ -139,7 +141,11 @@ def roofline(engine, g, pos, batch, probe, dets):
         "column_pass_avg_us": round(col_us, 2),
         "row_pass_GBs": round(row_bytes / (row_us * 1e-6) / 1e9, 1),
         "column_pass_GBs": round(col_bytes / (col_us * 1e-6) / 1e9, 1),
-        "timing": "hipEvents around 20 back-to-back launches of each sweep on the engine stream",
+        "copy_pattern_avg_us": round(copy_us, 2),
+        "copy_pattern_GBs": round(batch * 16 * N / (copy_us * 1e-6) / 1e9, 1),
+        "frac_of_copy_pattern": round(copy_us / us, 4),
+        "timing": "hipEvents around 20 back-to-back launches of each sweep on the engine stream; the copy "
+                  "pattern is the same kernel's tile loads and stores without arithmetic (10 launches)",
     }
 
 

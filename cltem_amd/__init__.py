@@ -381,7 +381,7 @@ class Engine:
         _check(self.lib.mse_autotune(self.h, config, n_waves, reps, _dp(row), _dp(col), C.byref(br),
                                      C.byref(bc)))
         self.n_waves = max(self.n_waves, n_waves)
-        nr, nc = 7, 12  # row / column kernel variants
+        nr, nc = 7, 14  # row / column kernel variants
         return dict(row_us=row[:nr].tolist(), col_us=col[:nc].tolist(), best_row=br.value,
                     best_col=bc.value)
 

@@ -276,7 +276,9 @@ void sync_wave_prop(mse_engine* e);
 //          9: as 7 with twiddles read at use
 //         10: as 7 with 8-column tiles (512 threads: no register spills at 1024 points)
 constexpr int kRowVariants = 7;
-constexpr int kColVariants = 12;
+//         12: as 10 (8-column tiles) with XCD-aware item order, up to four blocks per CU
+//         13: as 7 (16-column tiles) with XCD-aware item order
+constexpr int kColVariants = 14;
 
 // Sweep launchers, one translation unit per resolution (sweeps_<N>.hip).
 template <int N>

@@ -494,7 +494,11 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
 // loads of item i+1 are already in flight into registers, and the stores of item i drain
 // during the next item's transforms.  Same arithmetic as k_col<..., COL_PROP>.
 // --------------------------------------------------------------------------------------
-template <class PL, int TC, class SYNC = WaveSync, bool TWREG = true, int WPE = 1>
+// XCD: blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8), each with
+// its own L2.  With XCD set (and gridDim.x a multiple of 8) the blocks of one XCD take
+// consecutive items in every round, so tiles that share 128-byte lines (8-column tiles: two
+// per line) are read and written through the same L2 at about the same time.
+template <class PL, int TC, class SYNC = WaveSync, bool TWREG = true, int WPE = 1, bool XCD = false>
 __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE))) void k_col_prop_pipe(float2* __restrict__ data, size_t wave_stride,
                                                               const float2* __restrict__ tw, ColArgs a,
                                                               int nsets) {
@@ -558,7 +562,8 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
             for (int i = tid; i < N; i += NT) spy[i] = a.py[i];  // ordered by the first tile's barrier
     }
     const float2* pyt = a.py;
-    int item = blockIdx.x, prev = -1;
+    int item = XCD ? (int)((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8) : (int)blockIdx.x;
+    int prev = -1;
     if (item < nitems) fetch(item);
     for (; item < nitems; item += gridDim.x) {
         const int wv = a.wave_map ? cload(a.wave_map + item / ntiles) : item / ntiles;

@@ -164,6 +164,28 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
                 return tiles;
             }
         }
+        if ((variant == 12 || variant == 13) && mode == mse::COL_PROP && in == out) {
+            // persistent tiles of 8 (12) or 16 (13) columns with XCD-aware item order; the grid
+            // is a multiple of the 8 XCDs
+            auto go_xcd = [&](auto tc) {
+                constexpr int TCX = decltype(tc)::value;
+                constexpr int NT = TCX * P0::L;
+                const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, TCX, mse::WaveSync, true, 1, true>>(
+                    NT, 2048 / NT);
+                int blocks = std::min(e->num_cus * per_cu, (N / TCX) * nsets);
+                blocks = blocks >= 8 ? blocks / 8 * 8 : 8;
+                tiles = N / TCX;
+                launch(e, kind, [&]() {
+                    mse::k_col_prop_pipe<P0, TCX, mse::WaveSync, true, 1, true><<<blocks, NT, 0, e->stream>>>(
+                        out, wave_stride, e->tw.p, args, nsets);
+                });
+            };
+            if (variant == 12)
+                go_xcd(C8{});
+            else
+                go_xcd(C16{});
+            return tiles;
+        }
         if (variant == 10 && mode == mse::COL_PROP && in == out) {  // 8-column tiles (no spills at 1024)
             constexpr int NT = 8 * P0::L;
             const int per_cu = resident_per_cu<mse::k_col_prop_pipe<P0, 8, mse::WaveSync>>(NT, 1024 / NT);

@@ -321,13 +321,14 @@ int mse_potential_table(int form, int full3d, const double* params, int n_params
  *      0 wave-level LDS sync, 1 half lanes per line, 2 block barriers, 3 twiddles read at
  *      use, 4 persistent with the next row prefetched (default), 5 as 4 with half lanes,
  *      6 as 4 with twiddles read at use.
- *      kernel 1 = column pass (variants 0..11): 0 16-column tiles, 1 half lanes, 2 8-column
+ *      kernel 1 = column pass (variants 0..13): 0 16-column tiles, 1 half lanes, 2 8-column
  *      tiles, 3 block barriers, 4/5 twiddles read at use (16/8 columns), 6 32-column tiles,
  *      7 persistent with the next tile prefetched (default), 8 as 7 with half lanes, 9 as 7
  *      with twiddles read at use, 10 as 7 with 8-column tiles, 11 one tile per block at two
  *      blocks per CU, twiddles and propagator rows from LDS (16-column tiles at 512 points,
  *      8-column tiles for the four-pass 768-point plan; every other size, 384 included,
- *      runs variant 7).
+ *      runs variant 7), 12 / 13 as 10 / 7 with the persistent blocks of each XCD taking
+ *      consecutive tiles, so that 8-column tiles sharing 128-byte lines meet in one L2.
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
  *      (reps launches each; times in microseconds per launch), selects the fastest and
  *      leaves the waves' contents undefined. ---- */

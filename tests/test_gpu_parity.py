@@ -120,7 +120,7 @@ def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
         np.testing.assert_array_equal(replay, eager)
     # every kernel variant computes the same detector values
     for rv in range(7):
-        for cv in range(12):
+        for cv in range(14):
             engine.set_kernel_variant(0, rv)
             engine.set_kernel_variant(1, cv)
             out = engine.stem_batch(0, pos[:8], 8, probe, dets)
@@ -131,8 +131,8 @@ def test_graph_replays_match_eager_and_variants_agree(engine, oracle):
 
 @pytest.mark.parametrize("res", [512, 384, 768, 1024])
 def test_column_variants_agree(engine, oracle, res):
-    """The column kernels (persistent: prefetching, half lanes, 8-column tiles; lean: two
-    blocks per CU, 512 points) do the same butterflies with the same twiddles as the default; only the compiler's fusing of
+    """The column kernels (persistent: prefetching, half lanes, 8-column tiles, XCD-aware 8- and
+    16-column tiles; lean: two blocks per CU, 512 points) do the same butterflies with the same twiddles as the default; only the compiler's fusing of
     multiplies and adds into FMAs may differ between kernels, so the exit waves of a small
     specimen agree to float rounding (rel-L2 1e-6)."""
     w = wl.stem_srtio3(nx=4, ny=4, nz=3, resolution=res, pixels=4, window_cells=(1, 2))
@@ -144,7 +144,7 @@ def test_column_variants_agree(engine, oracle, res):
     engine.set_wave_count(len(pos))
     engine.set_wave_configs([0] * len(pos))
     ref = None
-    for cv in (7, 8, 10, 11):
+    for cv in (7, 8, 10, 11, 12, 13):
         engine.set_kernel_variant(1, cv)
         engine.init_probes(pos, probe)
         engine.multislice()
