@@ -599,9 +599,13 @@ def main():
 
     # slice-loop rate against B (transmission functions resident)
     sweep = {}
+    sweep_tuning = {}
     for B in [int(b) for b in args.sweep.split(",") if b]:
         npr = max(B, args.sweep_probes if B > 1 else min(args.sweep_probes, 64))
         sub = pos[:npr]
+        if B > 1 and B != args.batch:  # kernel variants tuned for this batch size too
+            t = engine.autotune(0, B, reps=3)
+            sweep_tuning[str(B)] = {"best_row": t["best_row"], "best_col": t["best_col"]}
         engine.stem_batch(0, sub[:B], B, probe, dets)
         engine.synchronize()
         t0 = time.perf_counter()
@@ -650,7 +654,7 @@ def main():
             "phonon_configs_timed": args.steps, "atoms": len(spec), "device": dev,
         },
         "kernel_tuning": tune,
-        "slice_loop_rate_by_batch": sweep,
+        "slice_loop_rate_by_batch": sweep, "kernel_tuning_by_batch": sweep_tuning,
         "plain_rate": round(plain, 1),
         # whole job at B (value) over the plain form with a host sync after every slice
         "batched_over_plain": round(value / plain, 2) if plain > 0 else None,

@@ -103,9 +103,16 @@ void dispatch_res(int res, F&& f) {
 }
 
 // ---- sweeps: dispatch on the resolution to the per-size launchers ---------------------------
+// the variant of a sweep over nsets waves: the one tuned for that wave count, else the default
+int tuned_variant(const mse_engine* e, int nsets, bool column) {
+    for (const auto& t : e->tuned)
+        if (t.first == nsets) return column ? t.second.second : t.second.first;
+    return column ? e->col_var : e->row_var;
+}
+
 void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines_sets,
               size_t wave_stride, const mse::RowArgs& args, int variant) {
-    if (variant < 0) variant = e->row_var;
+    if (variant < 0) variant = tuned_variant(e, nlines_sets, false);
     switch (e->res) {
         case 256: row_pass_n<256>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
         case 384: row_pass_n<384>(e, mode, in, out, nlines_sets, wave_stride, args, variant); break;
@@ -119,7 +126,7 @@ void row_pass(mse_engine* e, int mode, const float2* in, float2* out, int nlines
 // returns the number of column tiles of the launch (detector partials per wave)
 int col_pass(mse_engine* e, int mode, const float2* in, float2* out, int nsets, size_t wave_stride,
              const mse::ColArgs& args, int kind, int variant) {
-    if (variant < 0) variant = e->col_var;
+    if (variant < 0) variant = tuned_variant(e, nsets, true);
     switch (e->res) {
         case 256: return col_pass_n<256>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
         case 384: return col_pass_n<384>(e, mode, in, out, nsets, wave_stride, args, kind, variant);
@@ -1189,6 +1196,7 @@ void reduce_detectors(mse_engine* e, int nw, int n_det, int sqrt2, double* dev_o
 // memory and are updated (outside the graph) before each replay.
 struct StemGraphKey {
     int res, cfg, nb, n_det, sqrt2, row_var, col_var, n_slices, mix_first, mix_sets;
+    unsigned tune_version;
     unsigned long long mix_seed;
     const void *waves, *partials, *det_out, *det_bits, *trans, *probe_pos, *probe_df, *mix_key;
     const void *cfg_ptrs, *cfg_nslices, *wave_cfg, *wave_map, *tw, *px, *py, *wave_prop, *lim, *kx, *ky;
@@ -1234,6 +1242,7 @@ int mse_multislice(mse_engine* e, int first, int count) {
 // device pointer baked into its kernel arguments.
 struct LoopGraphKey {
     int res, nw, first, count, row_var, col_var, num_cus, mix_first, mix_sets;
+    unsigned tune_version;
     unsigned long long mix_seed;
     const void *waves, *cfg_ptrs, *cfg_nslices, *wave_cfg, *tw, *px, *py, *wave_prop, *lim, *mix_key;
 };
@@ -1252,6 +1261,7 @@ std::vector<unsigned char> loop_graph_key(mse_engine* e, int first, int count) {
     k.count = count;
     k.row_var = e->row_var;
     k.col_var = e->col_var;
+    k.tune_version = e->tune_version;
     k.num_cus = e->num_cus;
     k.waves = e->waves.p;
     k.cfg_ptrs = e->cfg_ptrs.p;
@@ -1602,6 +1612,7 @@ int mse_stem_batch(mse_engine* e, int cfg, const double* pos_xy, int n_probes, i
             k.sqrt2 = sqrt2;
             k.row_var = e->row_var;
             k.col_var = e->col_var;
+            k.tune_version = e->tune_version;
             k.n_slices = ns;
             k.waves = e->waves.p;
             k.partials = e->partials.p;
@@ -2028,6 +2039,8 @@ int mse_set_kernel_variant(mse_engine* e, int kernel, int variant) {
             e->row_var = variant;
         else
             e->col_var = variant;
+        e->tuned.clear();  // an explicit choice holds for every wave count
+        ++e->tune_version;
     });
 }
 
@@ -2089,6 +2102,15 @@ int mse_autotune(mse_engine* e, int cfg, int n_waves, int reps, double* row_us, 
             if (us < best) { best = us; bi = v; }
         }
         e->col_var = bi;
+        // remembered for this wave count; other wave counts use the latest choice
+        bool found = false;
+        for (auto& t : e->tuned)
+            if (t.first == n_waves) {
+                t.second = {e->row_var, e->col_var};
+                found = true;
+            }
+        if (!found) e->tuned.push_back({n_waves, {e->row_var, e->col_var}});
+        ++e->tune_version;
         e->prof = prof;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);

@@ -169,6 +169,10 @@ struct mse_engine {
     DevBuf<unsigned> keys, keys_sorted, idx, idx_sorted;
     DevBuf<unsigned char> cub_tmp;
     int row_var = 4, col_var = 7;  // kernel variants (see row_pass / col_pass); pipelined by default
+    // per wave count: the (row, column) variants mse_autotune found fastest for that many waves;
+    // sweeps over a tuned wave count use them, any other count the defaults above
+    std::vector<std::pair<int, std::pair<int, int>>> tuned;
+    unsigned tune_version = 0;     // bumped by every change of variants (captured graphs depend on them)
     hipGraphExec_t stem_exec = nullptr;
     std::vector<unsigned char> stem_key;
     hipGraphExec_t loop_exec = nullptr;   // captured slice loop of mse_multislice_graph

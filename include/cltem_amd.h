@@ -330,8 +330,10 @@ int mse_potential_table(int form, int full3d, const double* params, int n_params
  *      runs variant 7), 12 / 13 as 10 / 7 with the persistent blocks of each XCD taking
  *      consecutive tiles, so that 8-column tiles sharing 128-byte lines meet in one L2.
  *      mse_autotune times every variant of both sweeps on n_waves waves of `config`
- *      (reps launches each; times in microseconds per launch), selects the fastest and
- *      leaves the waves' contents undefined. ---- */
+ *      (reps launches each; times in microseconds per launch) and selects the fastest: sweeps
+ *      over that number of waves use it from then on, sweeps over any other number of waves the
+ *      latest choice.  It leaves the waves' contents undefined.  mse_set_kernel_variant
+ *      forces a variant for every number of waves. ---- */
 int mse_set_kernel_variant(mse_engine* e, int kernel, int variant);
 int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_us, double* col_us,
                  int* best_row, int* best_col);
