@@ -366,9 +366,10 @@ class Engine:
         return accum
 
     # -- tuning -------------------------------------------------------------------------
-    def set_potential_tables(self, enable: bool = True):
-        """Tabulated potentials (default) or the formulas evaluated directly, from the next build."""
-        _check(self.lib.mse_set_potential_tables(self.h, int(enable)))
+    def set_potential_tables(self, enable: bool = True, lds: bool = True):
+        """Tabulated potentials (default) or the formulas evaluated directly, from the next build;
+        lds=False keeps the full-3D kernels' table reads in global memory (same values)."""
+        _check(self.lib.mse_set_potential_tables(self.h, (1 if lds else 2) if enable else 0))
 
     def set_kernel_variant(self, kernel: int, variant: int):
         _check(self.lib.mse_set_kernel_variant(self.h, kernel, variant))

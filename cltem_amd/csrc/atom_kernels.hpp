@@ -311,17 +311,51 @@ struct PotArgs {
     // tabulated potentials (potential_table.hpp), kBuckets cubic pieces per element from
     // element 1; read by the kernels instantiated with TAB = true
     const float4* vtab;
+    // the full-3D kernels with LDS = true first copy into LDS, for each of the v_nel elements
+    // present, the buckets whose index bits(r^2) >> kMantBits lies in [vb_lo, vb_lo + vb_n):
+    // all that the slice's distances reach.  vslot[Z] is element Z's slot there and
+    // vslot[vsel_at + s] the element of slot s.
+    const int* vslot;
+    int v_nel, vb_lo, vb_n, vsel_at;
 };
 
-// V(r2) of one element from its table (r2 within the table's range; the bucket index is
-// clamped so that no r2 can read outside the element's table)
-__device__ __forceinline__ float vtab_eval(const float4* __restrict__ tz, float r2) {
+// An element's device table as an address biased so that the bucket of r2 sits at byte
+// offset (bits(r2) >> (kMantBits - 4)) with the low four bits cleared: the bucket index plus
+// (127 + kOctaveLo) << kSubBits, times 16.
+__device__ __forceinline__ const char* vtab_base(const float4* tz) {
+    return reinterpret_cast<const char*>(tz) - ((size_t)((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits) << 4);
+}
+
+// Squared distances are clamped below by at least this before a table lookup, so that the
+// bucket offset never falls below the element's table.
+constexpr float kVtabFloor = 1.0f / (float)(1u << -mse_tab::kOctaveLo);
+
+// V(r2) of one element from its device table (base from vtab_base), for r2 >= kVtabFloor.
+// Above the table's range the offset is clamped to the last bucket, so that no r2 (NaN
+// included) reads past the element's table.  The device table holds each bucket's cubic in
+// the integer m = t 2^kMantBits of the low mantissa bits, as (c0, c1 2^-kMantBits,
+// c2 2^-2kMantBits, c3 2^-3kMantBits): scaling by powers of two is exact in every rounding
+// step, so the Horner sum in m equals ((c3 t + c2) t + c1) t + c0 bit for bit while saving
+// the multiply that forms t.
+__device__ __forceinline__ float vtab_eval(const char* __restrict__ base, float r2) {
+    constexpr unsigned top = (unsigned)(((127 + mse_tab::kOctaveHi) << mse_tab::kSubBits) - 1) << 4;
     const unsigned bits = __float_as_uint(r2);
-    int i = (int)(bits >> mse_tab::kMantBits) - ((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits);
-    i = min(max(i, 0), mse_tab::kBuckets - 1);
-    const float t = (float)(bits & ((1u << mse_tab::kMantBits) - 1u)) * (1.0f / (float)(1u << mse_tab::kMantBits));
-    const float4 c = tz[i];
-    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    const unsigned off = min((bits >> (mse_tab::kMantBits - 4)) & ~15u, top);
+    const float4 c = *reinterpret_cast<const float4*>(base + off);
+    const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
+    return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
+}
+
+// V(r2) as vtab_eval computes it, from the LDS copy of an element's buckets
+// [vb_lo, vb_lo + vb_n), which starts at lt[off + vb_lo]; r2 must lie in those buckets
+// (the caller clamps it to them).
+__device__ __forceinline__ float vtab_eval_lds(const float4* __restrict__ lt, int off, float r2) {
+    const unsigned bits = __float_as_uint(r2);
+    // bits >> kMantBits as a bit-field extract, which keeps the index and address to two
+    // instructions (extract, then shift-and-add)
+    const float4 c = lt[(int)__builtin_amdgcn_ubfe(bits, mse_tab::kMantBits, 32 - mse_tab::kMantBits) + off];
+    const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
+    return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
 }
 
 // The arguments of the plane blockIdx.z of a launch over slices [first, first + gridDim.z),
@@ -352,6 +386,31 @@ __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int g
     endj = (int)fmin(ceil(a.blocks_y * (group_end_y - a.min_y) * rry) + a.load_y, (double)(a.blocks_y - 1));
 }
 
+// The same range cut down to the blocks that can hold an atom within `reach` A of the tile's
+// pixels: an atom at x >= x_lo is binned (k_bin_atoms: floor((x - min_x) * (1 / (max_x -
+// min_x)) * blocks_x)) into a block >= floor((x_lo - min_x) * (1 / (max_x - min_x)) *
+// blocks_x), each rounding step being monotone, and likewise at the other ends.  Atoms of the
+// blocks left out lie beyond the reach of every pixel of the tile: they add nothing and, in the
+// full-3D trapezoid, carry nothing to the next atom.
+__device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx, int gy, double reach,
+                                                        int& starti, int& endi, int& startj, int& endj) {
+    tile_block_range(a, gx, gy, starti, endi, startj, endj);
+    const double rrx = 1.0 / (a.max_x - a.min_x);
+    const double rry = 1.0 / (a.max_y - a.min_y);
+    const double x_lo = a.startx + (gx * 16) * a.pixelscale - reach;
+    const double x_hi = a.startx + (gx * 16 + 15) * a.pixelscale + reach;
+    const double y_lo = a.starty + (gy * 16) * a.pixelscale - reach;
+    const double y_hi = a.starty + (gy * 16 + 15) * a.pixelscale + reach;
+    starti = max(starti, (int)fmax(floor((x_lo - a.min_x) * rrx * a.blocks_x), -1.0));
+    endi = min(endi, (int)fmin(floor((x_hi - a.min_x) * rrx * a.blocks_x), (double)a.blocks_x));
+    startj = max(startj, (int)fmax(floor((y_lo - a.min_y) * rry * a.blocks_y), -1.0));
+    endj = min(endj, (int)fmin(floor((y_hi - a.min_y) * rry * a.blocks_y), (double)a.blocks_y));
+    if (starti > endi) {  // no block in reach: no rows (and no row reads)
+        startj = 1;
+        endj = 0;
+    }
+}
+
 __device__ __forceinline__ float2 phase_to_t(double ph) {
     ph = remainder(ph, 2.0 * M_PI);
     float s, c;
@@ -372,7 +431,11 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
     const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
     int starti, endi, startj, endj;
-    tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+    // untilted, a pixel reaches r <= 8 A (r2 <= 64.0000000001) laterally
+    if (a.untilted)
+        tile_block_range_within(a, blockIdx.x, blockIdx.y, 8.01, starti, endi, startj, endj);
+    else
+        tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
     int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
     const double im_pos_x = a.startx + xid * a.pixelscale;
     const double im_pos_y = a.starty + yid * a.pixelscale;
@@ -396,9 +459,9 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
             for (int l = 0; l < cnt; ++l) {
                 // the element's constants (or table), loaded ahead of the distance arithmetic
                 float pt[TAB ? 1 : kPotProjStride];
-                const float4* tz = nullptr;
+                const char* tz = nullptr;
                 if constexpr (TAB) {
-                    tz = a.vtab + (size_t)(sZ[l] - 1) * mse_tab::kBuckets;
+                    tz = vtab_base(a.vtab + (size_t)(__builtin_amdgcn_readfirstlane(sZ[l]) - 1) * mse_tab::kBuckets);
                 } else {
                     const float* row = ptab + (size_t)(sZ[l] - 1) * kPotProjStride;
 #pragma unroll
@@ -422,7 +485,7 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                 // decides only the sliver just above 64.  The potential takes rad in float.
                 if (r2 <= 64.0000000001 && (r2 <= 64.0 || sqrt(r2) <= 8.0)) {
                     if constexpr (TAB) {
-                        sumz += (double)vtab_eval(tz, r2 < r_min2 ? r_minf * r_minf : (float)r2);
+                        sumz += (double)vtab_eval(tz, fmaxf(r2 < r_min2 ? r_minf * r_minf : (float)r2, kVtabFloor));
                     } else {
                         const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
                         sumz += (double)vproj_tab<FORM>(pt, a.i_count, rf);
@@ -473,18 +536,20 @@ __global__ __launch_bounds__(256) void k_phase_projected(
 // slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
 // kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Returns sumz at one
 // pixel of slice `slice` (16x16 pixel tile per block).
-template <int FORM, bool TAB = false>
+template <int FORM, bool TAB = false, bool LDS = false>
 __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                               const double* __restrict__ az, const int* __restrict__ aZ,
                                               const int* __restrict__ bsp, const float* __restrict__ tab3d,
                                               const PotArgs& a, int slice, double* sx, double* sy, double* sz,
-                                              int* sZ) {
+                                              int* sZ, int* sh0, float* srz, const float4* lt = nullptr) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
     const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
     int starti, endi, startj, endj;
-    tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+    // a pixel reaches 8 A laterally, plus the beam-tilt shift over the integration points
+    const double drift = (a.integrals + 1) * fmax(fabs(a.shift_x), fabs(a.shift_y));
+    tile_block_range_within(a, blockIdx.x, blockIdx.y, 8.01 + drift, starti, endi, startj, endj);
     int topz = slice - a.load_z, bottomz = slice + a.load_z;
     if (topz < 0) topz = 0;
     if (bottomz >= a.n_slices) bottomz = a.n_slices - 1;
@@ -497,6 +562,12 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     const double r_min2 = r_min * r_min;
     const float r_min2f = (float)r_min * (float)r_min;
     const bool no_shift = (a.shift_x == 0.0 && a.shift_y == 0.0);
+    const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
+    // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
+    const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
+    const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
+    const float inv_ps = (float)(1.0 / a.pixelscale);
+    const float reach_px2 = (float)(64.01 / (a.pixelscale * a.pixelscale));
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int k = topz; k <= bottomz; ++k) {
@@ -510,18 +581,62 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                 if (lid < cnt) {
                     sx[lid] = ax[c0 + lid];
                     sy[lid] = ay[c0 + lid];
-                    sz[lid] = az[c0 + lid];
-                    sZ[lid] = aZ[c0 + lid];
+                    const double zl = az[c0 + lid];
+                    sz[lid] = zl;
+                    // with the tables in LDS, the element's slot there
+                    sZ[lid] = LDS ? a.vslot[aZ[c0 + lid]] : aZ[c0 + lid];
+                    if (no_shift) {
+                        // rad_z(h) = (current_z - h dz / integrals) - z falls with h (each
+                        // rounding step is monotone), so the points with rad_z <= 3 are
+                        // h0..integrals; h0 is found with the very expression of the per-point
+                        // test, once per atom for the whole tile, with the depth rad_z(h0)
+                        auto within = [&](int h) { return (current_z - h * a.dz * int_r) - zl <= 3.0; };
+                        const double h_est = ceil((current_z - zl - 3.0) / (a.dz * int_r));
+                        int h0 = (int)fmin(fmax(h_est, 0.0), (double)(a.integrals + 1));
+#pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
+                        while (h0 > 0 && within(h0 - 1)) --h0;
+#pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
+                        while (h0 <= a.integrals && !within(h0)) ++h0;
+                        // bit 16 + w: wave w (pixel rows 4w..4w+3 of the tile) may have a pixel
+                        // within 8 A.  The atom's distance to the wave's pixel rectangle, in
+                        // pixels and in float, is tested against 64.01 A^2: the float error
+                        // (~1e-6 A) is far below that margin, so a clear bit means every lane
+                        // of the wave fails its own exact test xyrad2 <= 64.
+                        unsigned waves = 0;
+                        if (h0 <= a.integrals) {
+                            const float u = (float)(sx[lid] - tile_x0) * inv_ps;
+                            const float v = (float)(sy[lid] - tile_y0) * inv_ps;
+                            const float du = fmaxf(fmaxf(-u, u - 15.f), 0.f);
+#pragma unroll
+                            for (int w = 0; w < 4; ++w) {
+                                const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
+                                if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
+                            }
+                        }
+                        sh0[lid] = h0 | (int)(waves << 16);
+                        srz[lid] = (float)((current_z - h0 * a.dz * int_r) - zl);
+                    }
                 }
                 __syncthreads();
                 for (int l = 0; l < cnt; ++l) {
+                    // the atom's first point (the same in every lane); atoms out of this
+                    // wave's reach, in depth or laterally, are passed over by the whole wave
+                    int h0 = 0;
+                    if (no_shift) {
+                        const int info = __builtin_amdgcn_readfirstlane(sh0[l]);
+                        if (!((info >> (16 + wave)) & 1)) continue;
+                        h0 = info & 0xffff;
+                    }
                     double rad_x = im_pos_x - sx[l];
                     double rad_y = im_pos_y - sy[l];
                     // the element's constants (scalar registers) or table, loaded once per atom
                     float pz[TAB ? 1 : kPot3dStride];
-                    const float4* tz = nullptr;
-                    if constexpr (TAB) {
-                        tz = a.vtab + (size_t)(sZ[l] - 1) * mse_tab::kBuckets;
+                    const char* tz = nullptr;
+                    int loff = 0;
+                    if constexpr (LDS) {
+                        loff = __builtin_amdgcn_readfirstlane(sZ[l]) * a.vb_n - a.vb_lo;
+                    } else if constexpr (TAB) {
+                        tz = vtab_base(a.vtab + (size_t)(__builtin_amdgcn_readfirstlane(sZ[l]) - 1) * mse_tab::kBuckets);
                     } else {
                         const float* row = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
 #pragma unroll
@@ -531,30 +646,47 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         // the xy distance is the same at every integration point
                         const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
                         if (xyrad2 > 64.0) continue;
-                        // rad_z(h) = (current_z - h dz / integrals) - z falls with h (each
-                        // rounding step is monotone), so the points with rad_z <= 3 are
-                        // h0..integrals; h0 is found with the very expression of the per-point test
-                        const double zl = sz[l];
-                        auto within = [&](int h) { return (current_z - h * a.dz * int_r) - zl <= 3.0; };
-                        const double h_est = ceil((current_z - zl - 3.0) / (a.dz * int_r));
-                        int h0 = (int)fmin(fmax(h_est, 0.0), (double)(a.integrals + 1));
-                        while (h0 > 0 && within(h0 - 1)) --h0;
-                        while (h0 <= a.integrals && !within(h0)) ++h0;
-                        if (h0 > a.integrals) continue;
                         // from there the depth steps in float and the atom's trapezoid terms are
                         // summed in float, then added to the pixel's double sum once
                         const float xy2f = (float)xyrad2;
                         const float stepf = (float)(a.dz * int_r);
-                        const float rz0 = (float)((current_z - h0 * a.dz * int_r) - zl);
+                        const float rz0 = srz[l];
                         float p2f = (float)p2, acc = 0.f;
+                        if constexpr (TAB) {
+                            // points h0..integrals at depths rz0 - (h - h0) step; the trapezoid
+                            //   sum_{h > 0} (p(h) + p(h - 1)) = 2 sum_h p(h) - p(integrals) + c,
+                            // c = p2 (carried from the row's previous atom) when h0 > 0 and
+                            // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
+                            // with s the sum over h0 + 1..integrals
+                            // r_min^2 >= kVtabFloor on every geometry the tables accept
+                            const float r_lo2 = fmaxf(r_min2f, kVtabFloor);
+                            // with the tables in LDS, r^2 is also held below the top of the last
+                            // bucket there, which lies above every distance the slice reaches
+                            const float r_hi2 = __uint_as_float(((unsigned)(a.vb_lo + a.vb_n) << mse_tab::kMantBits) - 1u);
+                            auto eval = [&](float r2) {
+                                if constexpr (LDS)
+                                    return vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2));
+                                else
+                                    return vtab_eval(tz, fmaxf(r2, r_lo2));
+                            };
+                            float rz = rz0;
+                            const float pf = eval(fmaf(rz, rz, xy2f));
+                            float s = 0.f, pl = pf;
+#pragma unroll 2
+                            for (int h = h0; h < a.integrals; ++h) {
+                                rz -= stepf;
+                                pl = eval(fmaf(rz, rz, xy2f));
+                                s += pl;
+                            }
+                            acc = (2.f * s - pl) + (h0 > 0 ? 2.f * pf + p2f : pf);
+                            sumz += 0.5 * (double)acc;
+                            p2 = (double)pl;
+                            continue;
+                        }
                         for (int h = h0; h <= a.integrals; ++h) {
                             const float rz = fmaf(-(float)(h - h0), stepf, rz0);
                             const float r2 = fmaf(rz, rz, xy2f);
-                            float p1;
-                            if constexpr (TAB)
-                                p1 = vtab_eval(tz, r2 < r_min2f ? r_min2f : r2);
-                            else
-                                p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
+                            const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
                             if (h != 0) acc += p1 + p2f;
                             p2f = p1;
                         }
@@ -571,7 +703,7 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                             const double r2 = xyrad2 + rad_z * rad_z;
                             float p1f;
                             if constexpr (TAB)
-                                p1f = vtab_eval(tz, r2 < r_min2 ? r_min2f : (float)r2);
+                                p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
                             else
                                 p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
                             const double p1 = (double)p1f;
@@ -586,16 +718,30 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     return sumz;
 }
 
-template <int FORM, bool TAB = false>
+// LDS = true (untilted beams only): the elements' tables are first copied into dynamic LDS
+// (v_nel * vb_n * 16 bytes), where the integration points read them.
+template <int FORM, bool TAB = false, bool LDS = false>
 __global__ __launch_bounds__(256) void k_potential_full3d(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
     const float* __restrict__ tab3d, PotArgs a, int first) {
     __shared__ double sx[256], sy[256], sz[256];
-    __shared__ int sZ[256];
+    __shared__ int sZ[256], sh0[256];
+    __shared__ float srz[256];
+    extern __shared__ float4 lt[];
+    if constexpr (LDS) {
+        const int total = a.v_nel * a.vb_n;
+        const int base = a.vb_lo - ((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits);
+        for (int i = threadIdx.x; i < total; i += 256) {
+            const int s = i / a.vb_n;
+            const int Z = a.vslot[a.vsel_at + s];
+            lt[i] = a.vtab[(size_t)(Z - 1) * mse_tab::kBuckets + base + (i - s * a.vb_n)];
+        }
+        // the first staging round of full3d_sumz synchronises before any table read
+    }
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ);
+    const double sumz = full3d_sumz<FORM, TAB, LDS>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ, sh0, srz, lt);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);
@@ -609,8 +755,9 @@ __global__ __launch_bounds__(256) void k_phase_full3d(
     const double* __restrict__ az, const int* __restrict__ aZ, const int* __restrict__ bsp,
     const float* __restrict__ tab3d, PotArgs a, int first) {
     __shared__ double sx[256], sy[256], sz[256];
-    __shared__ int sZ[256];
-    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ);
+    __shared__ int sZ[256], sh0[256];
+    __shared__ float srz[256];
+    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ, sh0, srz);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);

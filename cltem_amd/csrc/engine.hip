@@ -644,7 +644,9 @@ int mse_set_parameterisation(mse_engine* e, int form, int i_per_atom, const doub
 int mse_set_potential_tables(mse_engine* e, int enable) {
     return run_guarded("mse_set_potential_tables", e, [&]() {
         REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        REQUIRE(enable >= 0 && enable <= 2, MSE_ERR_ARG, "enable must be 0, 1 or 2");
         e->use_vtab = enable != 0;
+        e->vtab_lds = enable == 1;
     });
 }
 
@@ -1102,11 +1104,60 @@ const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geo
         if (!c.has_Z[Z] || e->vtab_on[(size_t)t * e->max_Z + Z - 1]) continue;
         float* h = e->h_vtab.data() + ((size_t)t * e->max_Z + Z - 1) * per;
         mse_tab::build(e->form, g.full3d != 0, e->h_params.data(), e->i_count, Z, h);
+        // the kernels' form of the cubics: in the integer m = t 2^kMantBits (exact rescaling)
+        for (int b = 0; b < mse_tab::kBuckets; ++b)
+            for (int q = 1; q < 4; ++q) h[4 * b + q] = std::ldexp(h[4 * b + q], -q * mse_tab::kMantBits);
         HIPX(hipMemcpyAsync(e->vtab.p + ((size_t)t * e->max_Z + Z - 1) * mse_tab::kBuckets, h,
                             per * sizeof(float), hipMemcpyHostToDevice, e->stream));
         e->vtab_on[(size_t)t * e->max_Z + Z - 1] = 1;
     }
     return e->vtab.p + (size_t)t * e->max_Z * mse_tab::kBuckets;
+}
+
+// The full-3D kernels' LDS copy of the tables (PotArgs::vslot .. vsel_at): the elements of the
+// configuration and the buckets an untilted slice reaches, from r_min^2 (the kernels' floor)
+// to 64 + ((load_z + 2) dz)^2.  Returns the bytes of LDS it takes, or 0 where the global
+// tables are used instead: a tilted beam, or tables over 32 KB.
+static size_t lds_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g, mse::PotArgs& a) {
+    if (a.shift_x != 0.0 || a.shift_y != 0.0) return 0;
+    const float r_min = (float)(0.25 * a.pixelscale);
+    const float r_lo2 = std::max(r_min * r_min, mse::kVtabFloor);
+    const double reach = (g.load_z + 2) * g.slice_thickness;
+    const float r_hi2 = (float)((64.0 + reach * reach) * 1.001);
+    const auto bucket = [](float r2) {
+        unsigned u;
+        std::memcpy(&u, &r2, sizeof(u));
+        return (int)(u >> mse_tab::kMantBits);
+    };
+    const int top = ((127 + mse_tab::kOctaveHi) << mse_tab::kSubBits) - 1;
+    const int lo = bucket(r_lo2), hi = std::min(bucket(r_hi2), top);
+    if (hi < lo) return 0;
+    const int n_z = (int)c.has_Z.size();
+    std::vector<int> hv((size_t)n_z, 0);
+    int n_el = 0;
+    for (int Z = 1; Z < n_z; ++Z) {
+        if (!c.has_Z[Z]) continue;
+        if (Z > e->max_Z) return 0;  // no table for it
+        hv[Z] = n_el++;
+    }
+    for (int Z = 1; Z < n_z; ++Z)
+        if (c.has_Z[Z]) hv.push_back(Z);
+    const size_t bytes = (size_t)n_el * (hi - lo + 1) * sizeof(float4);
+    if (n_el == 0 || bytes > 32768) return 0;
+    if (hv != e->h_vslot) {
+        // the previous contents may still be read or uploaded
+        HIPX(hipStreamSynchronize(e->stream));
+        e->vslot.ensure(hv.size());
+        e->h_vslot = hv;
+        HIPX(hipMemcpyAsync(e->vslot.p, e->h_vslot.data(), hv.size() * sizeof(int), hipMemcpyHostToDevice,
+                            e->stream));
+    }
+    a.vslot = e->vslot.p;
+    a.v_nel = n_el;
+    a.vb_lo = lo;
+    a.vb_n = hi - lo + 1;
+    a.vsel_at = n_z;
+    return bytes;
 }
 
 void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g, const mse::PotArgs& args,
@@ -1115,8 +1166,16 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);
     const bool tab = a.vtab != nullptr;
+    const size_t lds = tab && g.full3d && e->vtab_lds ? lds_tables(e, c, g, a) : 0;
     launch(e, PK_POT, [&]() {
-        if (g.full3d) {
+        if (g.full3d && lds) {
+            if (e->form == 0)
+                mse::k_potential_full3d<0, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+            else if (e->form == 1)
+                mse::k_potential_full3d<1, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+            else
+                mse::k_potential_full3d<2, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+        } else if (g.full3d) {
             if (e->form == 0)
                 tab ? mse::k_potential_full3d<0, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first)
                     : mse::k_potential_full3d<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);

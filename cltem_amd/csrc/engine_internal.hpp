@@ -130,10 +130,13 @@ struct mse_engine {
     // tabulated potentials (potential_table.hpp): the parameterisation's values, and per
     // table (0 projected, 1 full 3D) and element the host copy and whether it is on the device
     std::vector<double> h_params;
-    std::vector<float> h_vtab;           // [2][max_Z][kBuckets][4]
+    std::vector<float> h_vtab;           // [2][max_Z][kBuckets][4], as uploaded (cubics in m)
     std::vector<unsigned char> vtab_on;  // [2][max_Z]
     DevBuf<float4> vtab;                 // [2][max_Z][kBuckets]
+    std::vector<int> h_vslot;            // element slots of the LDS tables (PotArgs::vslot)
+    DevBuf<int> vslot;
     bool use_vtab = true;                // mse_set_potential_tables
+    bool vtab_lds = true;                // full-3D tables copied into LDS where they fit
     std::vector<ConfigData> configs;
     DevBuf<const float2*> cfg_ptrs;
     DevBuf<int> cfg_nslices;

@@ -62,6 +62,7 @@ void frame_position(const mse_geometry* g, int pixel, double* xp, double* yp) {
 extern "C" int mse_plan_geometry(const mse_scenario* s, mse_geometry* g) {
     if (!s || !g || s->resolution < 16 || !(s->slice_thickness > 0) || !(s->voltage_kv > 0))
         return MSE_ERR_ARG;
+    if (s->full3d && (s->full3d_integrals < 1 || s->full3d_integrals > 4096)) return MSE_ERR_ARG;
     std::memset(g, 0, sizeof(*g));
     g->resolution = s->resolution;
     g->slice_thickness = s->slice_thickness;

@@ -46,7 +46,7 @@ typedef struct mse_scenario {
     double beam_azimuth;     /* MicroscopeParameters::BeamAzimuth (rad) */
     double limit_factor;     /* SimulationManager::max_inverse_factor, 2/3 */
     int full3d;              /* SimulationManager::use_full_3d */
-    int full3d_integrals;    /* SimulationManager::full_3d_integrals, 20 */
+    int full3d_integrals;    /* SimulationManager::full_3d_integrals, 20 (1..4096 with full3d) */
     int moving_stem_frame;   /* STEM only: 1 = !SimulationManager::parallel_stem, every pixel in a
                                 field of its own (StemArea::getPixelSimArea); 0 = clTEM's default */
 } mse_scenario;
@@ -309,8 +309,10 @@ int mse_stem_batch_frames(mse_engine* e, int config, const double* pos_xy, const
  *      (kernels/transmission_potentials_projected_d.cl:71-181,
  *      kernels/transmission_potentials_full_3d_d.cl:57-134).  Geometries whose distances leave
  *      that range (pixels below 0.004 A, full-3D depth reach above 13.8 A) evaluate the
- *      formulas directly.  enable = 0 always evaluates them directly; either way the next
- *      mse_build_transmission uses the choice.  No clTEM counterpart. ---- */
+ *      formulas directly.  The full-3D kernels of an untilted beam copy the buckets a slice
+ *      reaches into LDS when they take at most 32 KB; enable = 2 keeps every read in global
+ *      memory (same values; for tests), enable = 0 always evaluates the formulas directly.
+ *      The next mse_build_transmission uses the choice.  No clTEM counterpart. ---- */
 int mse_set_potential_tables(mse_engine* e, int enable);
 /* host only: the table of element Z (1-based) as 4 floats (c0..c3 of V = ((c3 t + c2) t + c1) t
  * + c0) per bucket into out[4 * 896], for form 0/1/2 (Kirkland, Peng, Lobato), full3d 0/1 */

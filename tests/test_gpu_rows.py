@@ -69,7 +69,8 @@ def test_full3d_amorphous(engine, oracle, params):
 def test_tabulated_and_direct_potentials(engine, oracle, params, full3d):
     """The tabulated potentials (default) and the formulas evaluated directly
     (mse_set_potential_tables(e, 0)) give the same transmission functions, and both match the
-    FP64 oracle to rel-L2 1e-5 per slice."""
+    FP64 oracle to rel-L2 1e-5 per slice.  The full-3D tables read from LDS (default) and from
+    global memory (mse_set_potential_tables(e, 2)) give identical bits."""
     spec = sp.random_box({"Sr": 8, "Ti": 8, "O": 24}, (12.0, 12.0, 6.0), 1.5, seed=21)
     lo, hi = spec.limits
     scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.5, struct_min=tuple(lo),
@@ -77,6 +78,10 @@ def test_tabulated_and_direct_potentials(engine, oracle, params, full3d):
     try:
         g, og, p, o = prepare(engine, oracle, scen, spec, params)
         tab = [engine.read_transmission(0, s) for s in range(g.n_slices)]
+        engine.set_potential_tables(True, lds=False)
+        engine.build_transmission(0)
+        for s in range(g.n_slices):
+            np.testing.assert_array_equal(engine.read_transmission(0, s), tab[s])
         engine.set_potential_tables(False)
         engine.build_transmission(0)
         direct = [engine.read_transmission(0, s) for s in range(g.n_slices)]
