@@ -351,9 +351,7 @@ __device__ __forceinline__ float vtab_eval(const char* __restrict__ base, float 
 // (the caller clamps it to them).
 __device__ __forceinline__ float vtab_eval_lds(const float4* __restrict__ lt, int off, float r2) {
     const unsigned bits = __float_as_uint(r2);
-    // bits >> kMantBits as a bit-field extract, which keeps the index and address to two
-    // instructions (extract, then shift-and-add)
-    const float4 c = lt[(int)__builtin_amdgcn_ubfe(bits, mse_tab::kMantBits, 32 - mse_tab::kMantBits) + off];
+    const float4 c = lt[(int)(bits >> mse_tab::kMantBits) + off];
     const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
     return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
 }
@@ -420,7 +418,8 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
 
 // Projected potential sum (sumz) at one pixel of slice `slice`: one 16x16 pixel
 // tile per block; the atoms of each block row are staged through LDS in chunks of 256,
-// with no limit on how many a row holds.
+// with no limit on how many a row holds.  (The tables stay in global memory here: one
+// evaluation per atom and pixel is too little work to repay an LDS copy per block.)
 template <int FORM, bool TAB>
 __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                  const int* __restrict__ aZ, const int* __restrict__ bsp,
@@ -442,6 +441,12 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
     const double r_min = 0.25 * a.pixelscale;
     const double r_min2 = r_min * r_min;
     const float r_minf = (float)r_min;
+    const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
+    // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
+    const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
+    const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
+    const float inv_ps = (float)(1.0 / a.pixelscale);
+    const float reach_px2 = (float)(64.01 / (a.pixelscale * a.pixelscale));
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int j = startj; j <= endj; ++j) {
@@ -451,19 +456,39 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
             const int cnt = min(256, end - c0);
             __syncthreads();
             if (lid < cnt) {
-                sx[lid] = ax[c0 + lid];
-                sy[lid] = ay[c0 + lid];
-                sZ[lid] = aZ[c0 + lid];
+                const double xa = ax[c0 + lid], ya = ay[c0 + lid];
+                sx[lid] = xa;
+                sy[lid] = ya;
+                // bits 16..19: the waves (pixel rows 4w..4w+3) that may have a pixel within
+                // 8 A; the distance to each wave's pixel rectangle, in pixels and in float,
+                // is tested against 64.01 A^2, a margin far above the float error.  Untilted
+                // only (a tilted metric is no lateral distance): all waves otherwise.
+                unsigned waves = 15u;
+                if (a.untilted) {
+                    waves = 0;
+                    const float u = (float)(xa - tile_x0) * inv_ps;
+                    const float v = (float)(ya - tile_y0) * inv_ps;
+                    const float du = fmaxf(fmaxf(-u, u - 15.f), 0.f);
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) {
+                        const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
+                        if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
+                    }
+                }
+                sZ[lid] = aZ[c0 + lid] | (int)(waves << 16);
             }
             __syncthreads();
             for (int l = 0; l < cnt; ++l) {
+                const int info = __builtin_amdgcn_readfirstlane(sZ[l]);
+                if (!((info >> (16 + wave)) & 1)) continue;
+                const int zs = info & 0xffff;
                 // the element's constants (or table), loaded ahead of the distance arithmetic
                 float pt[TAB ? 1 : kPotProjStride];
                 const char* tz = nullptr;
                 if constexpr (TAB) {
-                    tz = vtab_base(a.vtab + (size_t)(__builtin_amdgcn_readfirstlane(sZ[l]) - 1) * mse_tab::kBuckets);
+                    tz = vtab_base(a.vtab + (size_t)(zs - 1) * mse_tab::kBuckets);
                 } else {
-                    const float* row = ptab + (size_t)(sZ[l] - 1) * kPotProjStride;
+                    const float* row = ptab + (size_t)(zs - 1) * kPotProjStride;
 #pragma unroll
                     for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
                 }
@@ -481,9 +506,11 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                     const double y_prime = rad_y + zt * a.sin_phi;
                     r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
                 }
-                // rad = max(sqrt(r2), r_min) <= 8: certain for r2 <= 64; the double square root
-                // decides only the sliver just above 64.  The potential takes rad in float.
-                if (r2 <= 64.0000000001 && (r2 <= 64.0 || sqrt(r2) <= 8.0)) {
+                // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the reference's test,
+                // correctly rounded double sqrt): sqrt(64 + 2^-46) = 8 + 2^-50 - 2^-103... rounds
+                // to 8 while sqrt of the next double, 64 + 2^-45, rounds to 8 + 2^-49, so the test
+                // is exactly r2 <= 64 + 2^-46.  The potential takes rad in float.
+                if (r2 <= 0x1.0000000000001p+6) {
                     if constexpr (TAB) {
                         sumz += (double)vtab_eval(tz, fmaxf(r2 < r_min2 ? r_minf * r_minf : (float)r2, kVtabFloor));
                     } else {
@@ -500,6 +527,19 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
 // Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
 // (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
 // slice (relative to first).
+// The LDS copy of the tables a kernel instantiated with LDS = true reads: buckets
+// [vb_lo, vb_lo + vb_n) of each element present, slot after slot (see PotArgs::vslot).  The
+// first staging round of the sums synchronises the block before any table read.
+__device__ __forceinline__ void fill_lds_tables(const PotArgs& a, float4* lt) {
+    const int total = a.v_nel * a.vb_n;
+    const int base = a.vb_lo - ((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits);
+    for (int i = threadIdx.x; i < total; i += 256) {
+        const int s = i / a.vb_n;
+        const int Z = a.vslot[a.vsel_at + s];
+        lt[i] = a.vtab[(size_t)(Z - 1) * mse_tab::kBuckets + base + (i - s * a.vb_n)];
+    }
+}
+
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -729,16 +769,7 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     __shared__ int sZ[256], sh0[256];
     __shared__ float srz[256];
     extern __shared__ float4 lt[];
-    if constexpr (LDS) {
-        const int total = a.v_nel * a.vb_n;
-        const int base = a.vb_lo - ((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits);
-        for (int i = threadIdx.x; i < total; i += 256) {
-            const int s = i / a.vb_n;
-            const int Z = a.vslot[a.vsel_at + s];
-            lt[i] = a.vtab[(size_t)(Z - 1) * mse_tab::kBuckets + base + (i - s * a.vb_n)];
-        }
-        // the first staging round of full3d_sumz synchronises before any table read
-    }
+    if constexpr (LDS) fill_lds_tables(a, lt);
     PotArgs b;
     const int slice = plane_args(a, first, b);
     const double sumz = full3d_sumz<FORM, TAB, LDS>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ, sh0, srz, lt);

@@ -1114,12 +1114,14 @@ const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geo
     return e->vtab.p + (size_t)t * e->max_Z * mse_tab::kBuckets;
 }
 
-// The full-3D kernels' LDS copy of the tables (PotArgs::vslot .. vsel_at): the elements of the
-// configuration and the buckets an untilted slice reaches, from r_min^2 (the kernels' floor)
-// to 64 + ((load_z + 2) dz)^2.  Returns the bytes of LDS it takes, or 0 where the global
-// tables are used instead: a tilted beam, or tables over 32 KB.
+// The full-3D kernels' LDS copy of the tables (PotArgs::vslot .. vsel_at): the elements of
+// the configuration and the buckets an untilted slice reaches, from r_min^2 as the kernels
+// round it (at least the tables' floor) to 64 + ((load_z + 2) dz)^2.  Returns the bytes of
+// LDS it takes, or 0 where the global tables are used instead: a tilted beam, or tables over
+// 32 KB.  (The projected kernels, one evaluation per atom and pixel, read the global tables:
+// an LDS copy per block costs them more than it saves.)
 static size_t lds_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g, mse::PotArgs& a) {
-    if (a.shift_x != 0.0 || a.shift_y != 0.0) return 0;
+    if (!g.full3d || a.shift_x != 0.0 || a.shift_y != 0.0) return 0;
     const float r_min = (float)(0.25 * a.pixelscale);
     const float r_lo2 = std::max(r_min * r_min, mse::kVtabFloor);
     const double reach = (g.load_z + 2) * g.slice_thickness;
@@ -1166,7 +1168,7 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);
     const bool tab = a.vtab != nullptr;
-    const size_t lds = tab && g.full3d && e->vtab_lds ? lds_tables(e, c, g, a) : 0;
+    const size_t lds = tab && e->vtab_lds ? lds_tables(e, c, g, a) : 0;
     launch(e, PK_POT, [&]() {
         if (g.full3d && lds) {
             if (e->form == 0)

@@ -64,3 +64,17 @@ def test_table_rejects_bad_input():
         cltem_amd.potential_table(p, 0)
     with pytest.raises(cltem_amd.MseError):
         cltem_amd.potential_table(p, 10_000)
+
+
+def test_projected_cutoff_threshold():
+    """The projected kernels test r2 <= 64 + 2^-46 for the reference's
+    r2 <= 64.0000000001 && sqrt(r2) <= 8.0 (correctly rounded double sqrt): the two agree on
+    every double from 64 - 2^-40 up to 64.0000000001 and on both sides of it."""
+    t = float.fromhex("0x1.0000000000001p+6")
+    xs = [64.0]
+    for _ in range(64):
+        xs.append(np.nextafter(xs[-1], 100.0))
+    xs += [np.nextafter(64.0, 0.0), 64.0 - 2.0 ** -40, 64.0000000001, np.nextafter(64.0000000001, 100.0), 65.0]
+    for x in xs:
+        ref = x <= 64.0000000001 and np.sqrt(x) <= 8.0
+        assert (x <= t) == ref, x
