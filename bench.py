@@ -500,9 +500,62 @@ def bench_adjoint(args):
     return 0
 
 
+def bench_sweeps(args):
+    """Row and column passes alone at several grid sizes: every row and column kernel variant
+    timed with hipEvents over 10 back-to-back launches (mse_time_sweeps) on --batch waves of a
+    small SrTiO3 specimen (one configuration), against copy kernels of each variant's access
+    pattern (mse_bandwidth_probe).  Algorithmic bytes: B*16N + 8N per row pass, B*16N per
+    column pass."""
+    B = args.batch
+    out = {}
+    for res in [int(x) for x in args.sizes.split(",")]:
+        w = wl.stem_srtio3(nx=4, ny=4, nz=3, resolution=res, pixels=4, window_cells=(1, 2))
+        engine = cltem_amd.Engine(0)
+        g, p = setup(engine, w)
+        spec = w["specimen"]
+        engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+        engine.build_transmission(0)
+        engine.set_wave_count(B)
+        engine.set_wave_configs([0] * B)
+        N = res * res
+        rowb, colb = B * 16 * N + 8 * N, B * 16 * N
+        rows, cols = [], []
+        for rv in range(7):
+            engine.set_kernel_variant(0, rv)
+            rows.append(engine.time_sweeps(0, reps=10)[0])
+        engine.set_kernel_variant(0, 4)
+        for cv in range(14):
+            engine.set_kernel_variant(1, cv)
+            cols.append((engine.time_sweeps(0, reps=10)[1], engine.bandwidth_probe(1, B, reps=10, variant=cv)))
+        copy_row = engine.bandwidth_probe(0, B, reps=10)
+        engine.close()
+        br = int(np.argmin(rows))
+        bc = int(np.argmin([c[0] for c in cols]))
+        out[str(res)] = {
+            "row_us": [round(x, 2) for x in rows], "col_us": [round(c[0], 2) for c in cols],
+            "col_copy_pattern_us": [round(c[1], 2) for c in cols], "row_copy_pattern_us": round(copy_row, 2),
+            "best_row": br, "best_col": bc,
+            "row_GBs": round(rowb / rows[br] / 1e3, 1), "col_GBs": round(colb / cols[bc][0] / 1e3, 1),
+            "row_frac": round(rowb / rows[br] / 8e6, 4), "col_frac": round(colb / cols[bc][0] / 8e6, 4),
+            "row_copy_GBs": round(rowb / copy_row / 1e3, 1), "col_copy_GBs": round(colb / cols[bc][1] / 1e3, 1)}
+        log(f"{res}^2, B = {B}: row variant {br} {out[str(res)]['row_GBs']} GB/s "
+            f"(copy pattern {out[str(res)]['row_copy_GBs']}), column variant {bc} {out[str(res)]['col_GBs']} GB/s "
+            f"(copy pattern {out[str(res)]['col_copy_GBs']})")
+    best = max(out.values(), key=lambda d: d["col_frac"])
+    line = {"metric": "slice-sweep HBM bandwidth per grid size (best variants)", "value": best["col_GBs"],
+            "unit": "GB/s", "n_gpus": 1, "steps": 10, "warmup": 0, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (SrTiO3 4x4x3, one configuration)",
+            "config": {"workload": f"row and column passes, B = {B}", "sizes": args.sizes}, "sizes": out}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("stem", "sio2", "mixed", "adjoint", "potentials"), default="stem")
+    ap.add_argument("--workload", choices=("stem", "sio2", "mixed", "adjoint", "potentials", "sweeps"),
+                    default="stem")
+    ap.add_argument("--sizes", default="512,768,1024", help="sweeps: grid sizes")
     ap.add_argument("--configs", type=int, default=30, help="sio2: frozen-phonon configurations")
     ap.add_argument("--cpu-slices", type=int, default=2, help="sio2: slices in the CPU baseline sample")
     ap.add_argument("--mixed-sequential", action="store_true",
@@ -538,6 +591,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "potentials":
         return bench_potentials(args)
+    if args.workload == "sweeps":
+        return bench_sweeps(args)
 
     w = wl.stem_srtio3()
     spec = w["specimen"]
