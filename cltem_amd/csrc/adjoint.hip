@@ -94,6 +94,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         e->adj_w.ensure(plane);
         HIPX(hipMemcpyAsync(e->adj_w.p, dl_di, plane * sizeof(double), hipMemcpyHostToDevice, e->stream));
         e->adj_phase.ensure((size_t)std::max(n, 1) * N);
+        require_elements(e, c);
         mse::PotArgs pa = pot_args(e, c);
         // the phases from the same (tabulated or direct) potentials as the forward build
         pa.vtab = potential_tables(e, c, g);

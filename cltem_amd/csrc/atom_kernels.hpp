@@ -322,8 +322,14 @@ struct PotArgs {
 // An element's device table as an address biased so that the bucket of r2 sits at byte
 // offset (bits(r2) >> (kMantBits - 4)) with the low four bits cleared: the bucket index plus
 // (127 + kOctaveLo) << kSubBits, times 16.
-__device__ __forceinline__ const char* vtab_base(const float4* tz) {
-    return reinterpret_cast<const char*>(tz) - ((size_t)((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits) << 4);
+// The base is wave-uniform and held in a scalar register pair that the compiler treats as
+// opaque (an empty asm statement), so that it cannot split the bias off again: the table
+// loads take it as their scalar base with the bucket offset as the 32-bit vector offset.
+__device__ __forceinline__ unsigned long long vtab_base(const float4* tz) {
+    unsigned long long b = reinterpret_cast<unsigned long long>(tz) -
+                           ((unsigned long long)((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits) << 4);
+    asm volatile("" : "+s"(b));
+    return b;
 }
 
 // Squared distances are clamped below by at least this before a table lookup, so that the
@@ -337,11 +343,13 @@ constexpr float kVtabFloor = 1.0f / (float)(1u << -mse_tab::kOctaveLo);
 // c2 2^-2kMantBits, c3 2^-3kMantBits): scaling by powers of two is exact in every rounding
 // step, so the Horner sum in m equals ((c3 t + c2) t + c1) t + c0 bit for bit while saving
 // the multiply that forms t.
-__device__ __forceinline__ float vtab_eval(const char* __restrict__ base, float r2) {
+__device__ __forceinline__ float vtab_eval(unsigned long long base, float r2) {
     constexpr unsigned top = (unsigned)(((127 + mse_tab::kOctaveHi) << mse_tab::kSubBits) - 1) << 4;
     const unsigned bits = __float_as_uint(r2);
     const unsigned off = min((bits >> (mse_tab::kMantBits - 4)) & ~15u, top);
-    const float4 c = *reinterpret_cast<const float4*>(base + off);
+    // a global-memory load at base + off (the address space 1 pointer tells the compiler so)
+    using gfloat4 = const __attribute__((address_space(1))) float4;
+    const float4 c = *(const float4*)reinterpret_cast<gfloat4*>(base + off);
     const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
     return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
 }
@@ -484,7 +492,7 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                 const int zs = info & 0xffff;
                 // the element's constants (or table), loaded ahead of the distance arithmetic
                 float pt[TAB ? 1 : kPotProjStride];
-                const char* tz = nullptr;
+                unsigned long long tz = 0;
                 if constexpr (TAB) {
                     tz = vtab_base(a.vtab + (size_t)(zs - 1) * mse_tab::kBuckets);
                 } else {
@@ -671,7 +679,7 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                     double rad_y = im_pos_y - sy[l];
                     // the element's constants (scalar registers) or table, loaded once per atom
                     float pz[TAB ? 1 : kPot3dStride];
-                    const char* tz = nullptr;
+                    unsigned long long tz = 0;
                     int loff = 0;
                     if constexpr (LDS) {
                         loff = __builtin_amdgcn_readfirstlane(sZ[l]) * a.vb_n - a.vb_lo;

@@ -1090,6 +1090,16 @@ void active_waves(mse_engine* e, int nw) {
     e->map_key = key;
 }
 
+void require_elements(const mse_engine* e, const ConfigData& c) {
+    // the potential kernels read per-element rows of the parameterisation (and its tables)
+    // by atomic number: a configuration sorted under a larger parameterisation must not reach
+    // past the current one
+    for (size_t Z = (size_t)std::max(e->max_Z, 0) + 1; Z < c.has_Z.size(); ++Z)
+        REQUIRE(!c.has_Z[Z], MSE_ERR_STATE,
+                "the configuration holds atomic numbers beyond the current parameterisation "
+                "(sort its atoms again after mse_set_parameterisation)");
+}
+
 const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g) {
     if (!e->use_vtab || e->max_Z <= 0 || e->h_params.empty()) return nullptr;
     // the distances the kernels evaluate must lie inside the tables: r >= r_min = px / 4, and
@@ -1164,6 +1174,7 @@ static size_t lds_tables(mse_engine* e, const ConfigData& c, const mse_geometry&
 
 void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g, const mse::PotArgs& args,
                       float2* out, int planes, int first) {
+    require_elements(e, c);
     const dim3 grid(e->res / 16, e->res / 16, planes);
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);

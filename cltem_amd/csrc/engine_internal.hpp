@@ -263,6 +263,8 @@ void band_limit_planes(mse_engine* e, float2* planes_p, int planes);
 // the tables of configuration c's elements for the geometry g (uploaded on first use), or
 // NULL when tabulation is off or the geometry's distances fall outside the tables' range
 const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geometry& g);
+// throws MSE_ERR_STATE if the configuration holds elements beyond the parameterisation
+void require_elements(const mse_engine* e, const ConfigData& c);
 // refresh wave_prop / any_beam after the waves' configurations or a configuration's beam changed
 void sync_wave_prop(mse_engine* e);
 

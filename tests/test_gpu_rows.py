@@ -307,3 +307,26 @@ def test_full3d_depth_cutoff_exact(engine, oracle):
           f"{np.abs(d_gpu).max():.3e} rad (GPU); difference {err:.2e} rad")
     assert np.abs(d_ref).max() > 5e-6
     assert err <= 0.05 * np.abs(d_ref).max()
+
+
+def test_parameterisation_smaller_than_sorted_atoms(engine, oracle):
+    """Atoms sorted under one parameterisation, then a parameterisation with fewer elements
+    than they use: building refuses (MSE_ERR_STATE) instead of reading past the smaller
+    parameter rows and tables, sorting again refuses the atomic numbers outside it, and the
+    full parameterisation builds as before."""
+    import dataclasses
+    w = wl.ctem_srtio3(2, 2, 2, resolution=256)
+    g, og, p, o = prepare(engine, oracle, w["scenario"], w["specimen"])
+    ref = engine.read_transmission(0, 0)
+    small = dataclasses.replace(p, values=p.values[: 22 * p.per_atom])  # up to Ti; Sr is 38
+    engine.set_parameterisation(small)
+    with pytest.raises(cltem_amd.MseError) as err:
+        engine.build_transmission(0)
+    assert err.value.code == -2, err.value  # MSE_ERR_STATE
+    spec = w["specimen"]
+    with pytest.raises(cltem_amd.MseError):
+        engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.set_parameterisation(p)
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    np.testing.assert_array_equal(engine.read_transmission(0, 0), ref)
