@@ -543,6 +543,17 @@ int mse_set_geometry(mse_engine* e, const mse_geometry* g) {
         e->py.ensure(res);
         e->prop_lim.ensure(res);
         e->band_lim.ensure(res);
+        // the band limit's column pass runs on the propagation kernels with these factors:
+        // x the unitary scale of the 2D transform pair, y one
+        const float band_scale = (float)(1.0 / ((double)res * (double)res));
+        std::vector<float2> bxy(2 * (size_t)res);
+        for (int i = 0; i < res; ++i) {
+            bxy[i] = make_float2(band_scale, 0.f);
+            bxy[res + i] = make_float2(1.f, 0.f);
+        }
+        e->band_xy.ensure(2 * (size_t)res);
+        HIPX(hipMemcpyAsync(e->band_xy.p, bxy.data(), 2 * (size_t)res * sizeof(float2), hipMemcpyHostToDevice,
+                            e->stream));
         e->kx.ensure(res);
         e->ky.ensure(res);
         HIPX(hipMemcpyAsync(e->tw.p, tw.data(), res * sizeof(float2), hipMemcpyHostToDevice, e->stream));
@@ -1212,15 +1223,23 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
     });
 }
 
-// band limit: FFT -> keep |k| <= kmax*2/3 -> IFFT (simulationgeneral.cpp:596-602)
+// band limit: FFT -> keep |k| <= kmax*2/3 -> IFFT (simulationgeneral.cpp:596-602).  The
+// column pass is a propagation step whose factors are the constant scale (x) and one (y),
+// with the band's per-row limits as the mask: cmul(v, cmul((s, 0), (1, 0))) = cmul(v, (s, 0))
+// bit for bit, on the persistent column kernels.  Their variant is fixed per grid size (not
+// tuned), so that transmission functions do not depend on the tuning state.
 void band_limit_planes(mse_engine* e, float2* p, int planes) {
     mse::RowArgs ra = base_row_args(e);
     ra.scale = 1.0f;
     row_pass(e, mse::ROW_FFT, p, p, planes, e->N, ra);
     mse::ColArgs ca = base_col_args(e);
+    ca.wave_map = nullptr;
+    ca.wave_prop = nullptr;
+    ca.px = e->band_xy.p;
+    ca.py = e->band_xy.p + e->res;
     ca.lim = e->band_lim.p;
-    ca.scale = (float)(1.0 / ((double)e->res * (double)e->res));
-    col_pass(e, mse::COL_BAND, p, p, planes, e->N, ca);
+    const int variant = e->res == 1024 ? 12 : e->res == 768 ? 11 : 7;
+    col_pass(e, mse::COL_PROP, p, p, planes, e->N, ca, PK_COL, variant);
     row_pass(e, mse::ROW_IFFT, p, p, planes, e->N, ra);
 }
 

@@ -121,6 +121,7 @@ struct mse_engine {
     size_t N = 0;
     DevBuf<float2> tw, px, py;
     DevBuf<int> prop_lim, band_lim;
+    DevBuf<float2> band_xy;  // x then y factors of the band limit's column pass
     DevBuf<double> kx, ky;
     std::vector<double> h_kx, h_ky;
     DevBuf<float> params;
