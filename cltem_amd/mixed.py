@@ -14,6 +14,7 @@ branches run concurrently on the one GPU.
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -84,6 +85,14 @@ class _SizeEngine:
         for c, _ in self.jobs:
             self.eng.build_transmission(c)
 
+    def sort_and_build(self):
+        """Each configuration binned, then its potentials queued: a binning's synchronisation
+        waits for the previous configuration's build, which keeps the GPU busy meanwhile."""
+        for c, d in self.jobs:
+            self.eng.sort_atoms(c, d.x, d.y, d.z, d.Z, want_ids=False)
+            self.eng.build_transmission(c)
+        self.eng.synchronize()
+
     def set_waves(self):
         self.eng.set_wave_count(len(self.wave_cfg))
         self.eng.set_wave_configs(self.wave_cfg)
@@ -106,8 +115,9 @@ def run_mixed_cbed(instances, params_name="kirkland", device=0, autotune=True, g
     After the set-up (one engine per grid size, the displaced coordinates of every
     configuration, first builds, autotuning), `warmup` untimed and `repeats` timed whole jobs
     run: binning and potentials of every instance and configuration, probe set-up, the slice
-    loops of all sizes and the FP64 sums.  With
-    concurrent=True the slice loops of all sizes are one graph launch (mse_group_multislice);
+    loops of all sizes and the FP64 sums.  With concurrent=True each size's binning and
+    potentials run from a host thread of their own and the slice loops of all sizes are one
+    graph launch (mse_group_multislice);
     otherwise each size runs its own loop after the previous one (graph per size when graph=True,
     eager launches otherwise).  When `timing` is a list, a MixedTiming of the last timed job is
     appended.  Returns (patterns in instance order, per-size results)."""
@@ -129,14 +139,18 @@ def run_mixed_cbed(instances, params_name="kirkland", device=0, autotune=True, g
     t_setup = time.perf_counter() - t_setup
     group = EngineGroup([s.eng for s in sizes]) if concurrent else None
 
+    pool = ThreadPoolExecutor(len(sizes)) if concurrent else None
+
     def job():
         t0 = time.perf_counter()
-        for s in sizes:  # binning synchronises each engine; the potential launches are queued
-            s.sort()
-        for s in sizes:
-            s.build()
-        for s in sizes:
-            s.eng.synchronize()
+        if pool is not None:
+            # one host thread per size: one engine's binning round trips overlap the other
+            # engines' potential kernels (the ABI calls release the interpreter lock)
+            for f in [pool.submit(s.sort_and_build) for s in sizes]:
+                f.result()
+        else:
+            for s in sizes:
+                s.sort_and_build()
         t1 = time.perf_counter()
         for s in sizes:
             s.init_probes()
@@ -156,6 +170,8 @@ def run_mixed_cbed(instances, params_name="kirkland", device=0, autotune=True, g
         for _ in range(max(1, repeats)):
             sums, job_s, build_s, loop_s = job()
     finally:
+        if pool is not None:
+            pool.shutdown()
         if group is not None:
             group.close()
     wave_slices = sum(s.wave_slices for s in sizes)
