@@ -330,3 +330,31 @@ def test_parameterisation_smaller_than_sorted_atoms(engine, oracle):
     engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
     engine.build_transmission(0)
     np.testing.assert_array_equal(engine.read_transmission(0, 0), ref)
+
+
+def test_full3d_tables_beyond_the_lds_budget(engine, oracle):
+    """Seven elements: their full-3D tables exceed the 32 KB the kernels copy into LDS, so the
+    tables are read from global memory (same bits as mse_set_potential_tables(e, 2)); the
+    transmission functions match the formulas evaluated directly and the FP64 oracle."""
+    spec = sp.random_box({"H": 6, "C": 6, "O": 6, "Si": 6, "Ti": 6, "Sr": 6, "Au": 6}, (12.0, 12.0, 6.0), 1.5,
+                         seed=5)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.5, struct_min=tuple(lo),
+                struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]), full3d=1, full3d_integrals=10)
+    try:
+        g, og, p, o = prepare(engine, oracle, scen, spec, "lobato")
+        tab = [engine.read_transmission(0, s) for s in range(g.n_slices)]
+        engine.set_potential_tables(True, lds=False)
+        engine.build_transmission(0)
+        for s in range(g.n_slices):
+            np.testing.assert_array_equal(engine.read_transmission(0, s), tab[s])
+        engine.set_potential_tables(False)
+        engine.build_transmission(0)
+        direct = [engine.read_transmission(0, s) for s in range(g.n_slices)]
+    finally:
+        engine.set_potential_tables(True)
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    e_td = max(rel_l2(a, b) for a, b in zip(tab, direct))
+    e_t = max(rel_l2(a, b) for a, b in zip(tab, trans))
+    print(f"7 elements, full 3D: tabulated vs direct {e_td:.2e}, vs FP64 {e_t:.2e}")
+    assert e_td <= 2e-6 and e_t <= 1e-5
