@@ -317,6 +317,10 @@ struct PotArgs {
     // vslot[vsel_at + s] the element of slot s.
     const int* vslot;
     int v_nel, vb_lo, vb_n, vsel_at;
+    // per launch, from the host: 1 / (max - min) of the padded limits (the binning's own
+    // factors), 1 / pixel size and (8 A + a margin of 0.01 A^2)^2 in pixels^2
+    double rrx, rry;
+    float inv_ps, reach_px2;
 };
 
 // An element's device table as an address biased so that the bucket of r2 sits at byte
@@ -384,8 +388,7 @@ __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int g
     const double group_end_x = group_start_x + gsx;
     const double group_start_y = a.starty + gy * gsy;
     const double group_end_y = group_start_y + gsy;
-    const double rrx = 1.0 / (a.max_x - a.min_x);
-    const double rry = 1.0 / (a.max_y - a.min_y);
+    const double rrx = a.rrx, rry = a.rry;
     starti = (int)fmax(floor(a.blocks_x * (group_start_x - a.min_x) * rrx) - a.load_x, 0.0);
     endi = (int)fmin(ceil(a.blocks_x * (group_end_x - a.min_x) * rrx) + a.load_x, (double)(a.blocks_x - 1));
     startj = (int)fmax(floor(a.blocks_y * (group_start_y - a.min_y) * rry) - a.load_y, 0.0);
@@ -401,8 +404,7 @@ __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int g
 __device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx, int gy, double reach,
                                                         int& starti, int& endi, int& startj, int& endj) {
     tile_block_range(a, gx, gy, starti, endi, startj, endj);
-    const double rrx = 1.0 / (a.max_x - a.min_x);
-    const double rry = 1.0 / (a.max_y - a.min_y);
+    const double rrx = a.rrx, rry = a.rry;
     const double x_lo = a.startx + (gx * 16) * a.pixelscale - reach;
     const double x_hi = a.startx + (gx * 16 + 15) * a.pixelscale + reach;
     const double y_lo = a.starty + (gy * 16) * a.pixelscale - reach;
@@ -417,8 +419,17 @@ __device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx
     }
 }
 
+// ph - n 2pi with n the integer nearest ph / 2pi (2pi as the double 2.0 * M_PI): the same
+// value as remainder(ph, 2pi), which is exact, whenever n agrees (the fma rounds the exact
+// difference once); otherwise the result lies at +-pi either way.  Phases here stay small
+// (|ph| << 2^40), far inside the range where n * 2pi is exact within the fma.
+__device__ __forceinline__ double reduce_2pi(double ph) {
+    const double n = rint(ph * (1.0 / (2.0 * M_PI)));
+    return fma(-n, 2.0 * M_PI, ph);
+}
+
 __device__ __forceinline__ float2 phase_to_t(double ph) {
-    ph = remainder(ph, 2.0 * M_PI);
+    ph = reduce_2pi(ph);
     float s, c;
     sincosf((float)ph, &s, &c);
     return make_float2(c, s);
@@ -453,8 +464,8 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
     const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
     const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
-    const float inv_ps = (float)(1.0 / a.pixelscale);
-    const float reach_px2 = (float)(64.01 / (a.pixelscale * a.pixelscale));
+    const float inv_ps = a.inv_ps;
+    const float reach_px2 = a.reach_px2;
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int j = startj; j <= endj; ++j) {
@@ -577,7 +588,7 @@ __global__ __launch_bounds__(256) void k_phase_projected(
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
-        (float)remainder(a.sigma * sumz, 2.0 * M_PI);
+        (float)reduce_2pi(a.sigma * sumz);
 }
 
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the
@@ -614,8 +625,8 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
     const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
     const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
-    const float inv_ps = (float)(1.0 / a.pixelscale);
-    const float reach_px2 = (float)(64.01 / (a.pixelscale * a.pixelscale));
+    const float inv_ps = a.inv_ps;
+    const float reach_px2 = a.reach_px2;
     double sumz = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int k = topz; k <= bottomz; ++k) {
@@ -801,7 +812,7 @@ __global__ __launch_bounds__(256) void k_phase_full3d(
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
-        (float)remainder(sst * a.sigma * sumz, 2.0 * M_PI);
+        (float)reduce_2pi(sst * a.sigma * sumz);
 }
 
 }  // namespace mse

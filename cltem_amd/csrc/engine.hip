@@ -206,6 +206,10 @@ mse::PotArgs pot_args(mse_engine* e, const ConfigData& c) {
     a.max_y = g.full_y[1];
     a.blocks_x = g.blocks_x;
     a.blocks_y = g.blocks_y;
+    a.rrx = 1.0 / (a.max_x - a.min_x);
+    a.rry = 1.0 / (a.max_y - a.min_y);
+    a.inv_ps = (float)(1.0 / a.pixelscale);
+    a.reach_px2 = (float)(64.01 / (a.pixelscale * a.pixelscale));
     a.load_x = g.load_x;
     a.load_y = g.load_y;
     a.load_z = g.load_z;
