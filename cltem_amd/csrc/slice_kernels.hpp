@@ -764,11 +764,14 @@ __global__ __launch_bounds__(8 * PL::L) __attribute__((amdgpu_waves_per_eu(4))) 
 // --------------------------------------------------------------------------------------
 // Persistent, software-pipelined row slice step (ROW_IFFT_T_FFT).  Every line walks the
 // (wave, row) items independently; the next item's wave row and transmission row are
-// loaded into registers while the current row is transformed.
+// loaded into registers while the current row is transformed.  MODE = ROW_FFT or ROW_IFFT:
+// the one transform and the scale of k_row's same modes, in place (no transmission row).
 // --------------------------------------------------------------------------------------
-template <class PL, class SYNC = WaveSync, bool TWREG = true>
+template <class PL, class SYNC = WaveSync, bool TWREG = true, int MODE = ROW_IFFT_T_FFT>
 __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data, size_t wave_stride,
                                                        const float2* __restrict__ tw, RowArgs a, int nsets) {
+    static_assert(MODE == ROW_IFFT_T_FFT || MODE == ROW_FFT || MODE == ROW_IFFT, "row step modes");
+    constexpr bool STEP = MODE == ROW_IFFT_T_FFT;
     constexpr int N = PL::N, L = PL::L, E = PL::E;
     constexpr int LINES = 256 / L;
     __shared__ float2 lds[LINES][LdsLine<N>::size];
@@ -790,9 +793,10 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
         if constexpr (L == 64) item = __builtin_amdgcn_readfirstlane(item);
         const int wv = a.wave_map ? cload(a.wave_map + item / N) : item / N, row = item % N;
         const float2* src = data + (size_t)wv * wave_stride + (size_t)row * N;
-        const int cfg = row_config(a, wv);
 #pragma unroll
         for (int m = 0; m < E; ++m) qv[m] = src[t + L * m];
+        if constexpr (!STEP) return;
+        const int cfg = row_config(a, wv);
         if (a.slice < cload(a.cfg_nslices + cfg)) {
             const float2* tr = cload(a.cfg_trans + cfg) + (size_t)a.slice * N * N + (size_t)row * N;
 #pragma unroll
@@ -812,11 +816,17 @@ __global__ __launch_bounds__(256) void k_row_step_pipe(float2* __restrict__ data
             tr[m] = qt[m];
         }
         if (item + stride < nitems) fetch(item + stride);
-        fft_line<PL, true>(v, ln, t, tws, sync);
-        sync();
+        if constexpr (STEP) {
+            fft_line<PL, true>(v, ln, t, tws, sync);
+            sync();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[m]);
-        fft_line<PL, false>(v, ln, t, tws, sync);
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tr[m]);
+            fft_line<PL, false>(v, ln, t, tws, sync);
+        } else {
+            fft_line<PL, MODE == ROW_IFFT>(v, ln, t, tws, sync);
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+        }
         sync();
         int it = item;
         if constexpr (L == 64) it = __builtin_amdgcn_readfirstlane(it);

@@ -65,6 +65,25 @@ void row_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nlin
         using P1 = typename mse::PlanHalfLanes<N>::P;
         using REG = std::true_type;
         using MEM = std::false_type;
+        if ((mode == mse::ROW_FFT || mode == mse::ROW_IFFT) && in == out) {
+            // one transform in place (the band limit of transmission functions): the persistent
+            // pipelined kernel, whatever the tuned variant
+            auto go_pipe = [&](auto m) {
+                constexpr int MD = decltype(m)::value;
+                constexpr int LINES = 256 / P0::L;
+                const int per_cu = resident_per_cu<mse::k_row_step_pipe<P0, mse::WaveSync, true, MD>>(256, 4);
+                const int blocks = std::max(1, std::min(e->num_cus * per_cu, (N * nlines_sets + LINES - 1) / LINES));
+                launch(e, PK_ROW, [&]() {
+                    mse::k_row_step_pipe<P0, mse::WaveSync, true, MD><<<blocks, 256, 0, e->stream>>>(
+                        out, wave_stride, e->tw.p, args, nlines_sets);
+                });
+            };
+            if (mode == mse::ROW_FFT)
+                go_pipe(std::integral_constant<int, mse::ROW_FFT>{});
+            else
+                go_pipe(std::integral_constant<int, mse::ROW_IFFT>{});
+            return;
+        }
         if constexpr (N <= 512) {  // half-lane lines keep the prefetch registers within budget up to 512
             if (mode == mse::ROW_IFFT_T_FFT && variant == 5 && in == out) {
                 constexpr int LINES = 256 / P1::L;
