@@ -326,9 +326,11 @@ struct PotArgs {
 // An element's device table as an address biased so that the bucket of r2 sits at byte
 // offset (bits(r2) >> (kMantBits - 4)) with the low four bits cleared: the bucket index plus
 // (127 + kOctaveLo) << kSubBits, times 16.
-// The base is wave-uniform and held in a scalar register pair that the compiler treats as
-// opaque (an empty asm statement), so that it cannot split the bias off again: the table
-// loads take it as their scalar base with the bucket offset as the 32-bit vector offset.
+// The element must be the same in every lane of the wave (the callers take it with
+// readfirstlane from the staged atom): the base is then held in a scalar register pair that
+// the compiler treats as opaque (an empty asm statement), so that it cannot split the bias
+// off again, and the table loads take it as their scalar base with the bucket offset as the
+// 32-bit vector offset.
 __device__ __forceinline__ unsigned long long vtab_base(const float4* tz) {
     unsigned long long b = reinterpret_cast<unsigned long long>(tz) -
                            ((unsigned long long)((127 + mse_tab::kOctaveLo) << mse_tab::kSubBits) << 4);
@@ -419,10 +421,11 @@ __device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx
     }
 }
 
-// ph - n 2pi with n the integer nearest ph / 2pi (2pi as the double 2.0 * M_PI): the same
-// value as remainder(ph, 2pi), which is exact, whenever n agrees (the fma rounds the exact
-// difference once); otherwise the result lies at +-pi either way.  Phases here stay small
-// (|ph| << 2^40), far inside the range where n * 2pi is exact within the fma.
+// ph - n 2pi with n the integer nearest ph / 2pi (2pi as the double 2.0 * M_PI).  The fma
+// takes n 2pi exactly and rounds the difference once; remainder(ph, 2pi) is exactly
+// representable, so the two agree whenever n is its quotient, which holds except where
+// ph / 2pi lies within rounding of a half-integer (both results are then +-pi).  The phases
+// here are small (a few hundred radians at most), so ph / 2pi is formed to far below 1/2.
 __device__ __forceinline__ double reduce_2pi(double ph) {
     const double n = rint(ph * (1.0 / (2.0 * M_PI)));
     return fma(-n, 2.0 * M_PI, ph);
@@ -543,9 +546,6 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
     return sumz;
 }
 
-// Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
-// (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
-// slice (relative to first).
 // The LDS copy of the tables a kernel instantiated with LDS = true reads: buckets
 // [vb_lo, vb_lo + vb_n) of each element present, slot after slot (see PotArgs::vslot).  The
 // first staging round of the sums synchronises the block before any table read.
@@ -559,6 +559,9 @@ __device__ __forceinline__ void fill_lds_tables(const PotArgs& a, float4* lt) {
     }
 }
 
+// Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
+// (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
+// slice (relative to first).
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -621,6 +624,11 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     const double r_min2 = r_min * r_min;
     const float r_min2f = (float)r_min * (float)r_min;
     const bool no_shift = (a.shift_x == 0.0 && a.shift_y == 0.0);
+    // table lookups: r^2 at least r_min^2 (and the tables' floor, which r_min^2 meets on every
+    // geometry the tables accept); with the tables in LDS also at most the top of the last
+    // bucket there, above every distance an untilted slice reaches
+    const float r_lo2 = fmaxf(r_min2f, kVtabFloor);
+    const float r_hi2 = __uint_as_float(((unsigned)(a.vb_lo + a.vb_n) << mse_tab::kMantBits) - 1u);
     const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
     const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
@@ -717,11 +725,6 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                             // c = p2 (carried from the row's previous atom) when h0 > 0 and
                             // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
                             // with s the sum over h0 + 1..integrals
-                            // r_min^2 >= kVtabFloor on every geometry the tables accept
-                            const float r_lo2 = fmaxf(r_min2f, kVtabFloor);
-                            // with the tables in LDS, r^2 is also held below the top of the last
-                            // bucket there, which lies above every distance the slice reaches
-                            const float r_hi2 = __uint_as_float(((unsigned)(a.vb_lo + a.vb_n) << mse_tab::kMantBits) - 1u);
                             auto eval = [&](float r2) {
                                 if constexpr (LDS)
                                     return vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2));
@@ -761,7 +764,9 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         if (xyrad2 <= 64.0 && rad_z <= 3.0) {
                             const double r2 = xyrad2 + rad_z * rad_z;
                             float p1f;
-                            if constexpr (TAB)
+                            if constexpr (LDS)  // not launched with a tilt; kept within the copy
+                                p1f = vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2 < r_min2 ? r_min2f : (float)r2, r_lo2, r_hi2));
+                            else if constexpr (TAB)
                                 p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
                             else
                                 p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
