@@ -789,8 +789,9 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         if (c.trans.p != before) upload_cfg_ptrs(e);
         const mse::PotArgs a = pot_args(e, c);
         float2* outp = c.trans.p + (size_t)first * e->N;
-        // potential kernels chunk the slice range so one launch stays short
-        const int chunk = g.full3d ? 1 : 16;
+        // potential kernels build up to 16 slices per launch (a few ms at most), which also
+        // keeps each launch's tail small against its work
+        const int chunk = 16;
         for (int s0 = 0; s0 < count; s0 += chunk) {
             const int cs = std::min(chunk, count - s0);
             potential_planes(e, c, g, a, outp + (size_t)s0 * e->N, cs, first + s0);
