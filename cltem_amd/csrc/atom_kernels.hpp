@@ -365,7 +365,11 @@ __device__ __forceinline__ float vtab_eval(unsigned long long base, float r2) {
 // (the caller clamps it to them).
 __device__ __forceinline__ float vtab_eval_lds(const float4* __restrict__ lt, int off, float r2) {
     const unsigned bits = __float_as_uint(r2);
-    const float4 c = lt[(int)(bits >> mse_tab::kMantBits) + off];
+    // the bucket index, shifted in an asm statement of its own so that the compiler folds the
+    // byte scaling and the base into one shift-and-add rather than shift, mask and add
+    unsigned q;
+    asm("v_lshrrev_b32 %0, %1, %2" : "=v"(q) : "i"(mse_tab::kMantBits), "v"(bits));
+    const float4 c = lt[(int)q + off];
     const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
     return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
 }
@@ -734,8 +738,18 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                             float rz = rz0;
                             const float pf = eval(fmaf(rz, rz, xy2f));
                             float s = 0.f, pl = pf;
-#pragma unroll 2
-                            for (int h = h0; h < a.integrals; ++h) {
+                            // two points per trip (the lookups' asm keeps the compiler from
+                            // unrolling), added in order
+                            int h = h0;
+                            for (; h + 1 < a.integrals; h += 2) {
+                                const float rza = rz - stepf;
+                                rz = rza - stepf;
+                                const float pa = eval(fmaf(rza, rza, xy2f));
+                                pl = eval(fmaf(rz, rz, xy2f));
+                                s += pa;
+                                s += pl;
+                            }
+                            if (h < a.integrals) {
                                 rz -= stepf;
                                 pl = eval(fmaf(rz, rz, xy2f));
                                 s += pl;
