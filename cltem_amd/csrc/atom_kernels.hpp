@@ -386,12 +386,14 @@ __device__ __forceinline__ int plane_args(const PotArgs& a, int first, PotArgs& 
     return first + (int)blockIdx.z;
 }
 
+// The blocks of the 16x16 pixel groups gx..gx + nx - 1 (the union of their ranges: the
+// range's ends are monotone in the group)
 __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int gy, int& starti,
-                                                 int& endi, int& startj, int& endj) {
+                                                 int& endi, int& startj, int& endj, int nx = 1) {
     // kernels/transmission_potentials_projected_d.cl:227-246 with a 16x16 group
     const double gsx = 16 * a.pixelscale, gsy = 16 * a.pixelscale;
     const double group_start_x = a.startx + gx * gsx;
-    const double group_end_x = group_start_x + gsx;
+    const double group_end_x = (a.startx + (gx + nx - 1) * gsx) + gsx;
     const double group_start_y = a.starty + gy * gsy;
     const double group_end_y = group_start_y + gsy;
     const double rrx = a.rrx, rry = a.rry;
@@ -408,11 +410,12 @@ __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int g
 // blocks left out lie beyond the reach of every pixel of the tile: they add nothing and, in the
 // full-3D trapezoid, carry nothing to the next atom.
 __device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx, int gy, double reach,
-                                                        int& starti, int& endi, int& startj, int& endj) {
-    tile_block_range(a, gx, gy, starti, endi, startj, endj);
+                                                        int& starti, int& endi, int& startj, int& endj,
+                                                        int nx = 1) {
+    tile_block_range(a, gx, gy, starti, endi, startj, endj, nx);
     const double rrx = a.rrx, rry = a.rry;
     const double x_lo = a.startx + (gx * 16) * a.pixelscale - reach;
-    const double x_hi = a.startx + (gx * 16 + 15) * a.pixelscale + reach;
+    const double x_hi = a.startx + ((gx + nx) * 16 - 1) * a.pixelscale + reach;
     const double y_lo = a.starty + (gy * 16) * a.pixelscale - reach;
     const double y_hi = a.starty + (gy * 16 + 15) * a.pixelscale + reach;
     starti = max(starti, (int)fmax(floor((x_lo - a.min_x) * rrx * a.blocks_x), -1.0));
@@ -442,38 +445,44 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
     return make_float2(c, s);
 }
 
-// Projected potential sum (sumz) at one pixel of slice `slice`: one 16x16 pixel
-// tile per block; the atoms of each block row are staged through LDS in chunks of 256,
+// Projected potential sums (sumz) of slice `slice` at NX pixels per thread, 16 columns apart:
+// one (16 NX)x16 pixel tile per block, the 16x16 groups blockIdx.x * NX .. + NX - 1 of the
+// reference's tiling; the atoms of each block row are staged through LDS in chunks of 256,
 // with no limit on how many a row holds.  (The tables stay in global memory here: one
 // evaluation per atom and pixel is too little work to repay an LDS copy per block.)
-template <int FORM, bool TAB>
-__device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
-                                                 const int* __restrict__ aZ, const int* __restrict__ bsp,
-                                                 const float* __restrict__ ptab, const PotArgs& a, int slice,
-                                                 double* sx, double* sy, int* sZ) {
+template <int FORM, bool TAB, int NX = 1>
+__device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
+                                               const int* __restrict__ aZ, const int* __restrict__ bsp,
+                                               const float* __restrict__ ptab, const PotArgs& a, int slice,
+                                               double* sx, double* sy, int* sZ, double (&sumz)[NX]) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
-    const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
     int starti, endi, startj, endj;
     // untilted, a pixel reaches r <= 8 A (r2 <= 64.0000000001) laterally
     if (a.untilted)
-        tile_block_range_within(a, blockIdx.x, blockIdx.y, 8.01, starti, endi, startj, endj);
+        tile_block_range_within(a, blockIdx.x * NX, blockIdx.y, 8.01, starti, endi, startj, endj, NX);
     else
-        tile_block_range(a, blockIdx.x, blockIdx.y, starti, endi, startj, endj);
+        tile_block_range(a, blockIdx.x * NX, blockIdx.y, starti, endi, startj, endj, NX);
     int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
-    const double im_pos_x = a.startx + xid * a.pixelscale;
+    double im_pos_x[NX];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const int xid = (blockIdx.x * NX + i) * 16 + (lid & 15);
+        im_pos_x[i] = a.startx + xid * a.pixelscale;
+    }
     const double im_pos_y = a.starty + yid * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
     const double r_min2 = r_min * r_min;
     const float r_minf = (float)r_min;
     const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
-    const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
+    const double tile_x0 = a.startx + (int)(blockIdx.x * NX * 16) * a.pixelscale;
     const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
     const float inv_ps = a.inv_ps;
     const float reach_px2 = a.reach_px2;
-    double sumz = 0.0;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) sumz[i] = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int j = startj; j <= endj; ++j) {
         const int start = bsp[k * nb + a.blocks_x * j + starti];
@@ -485,16 +494,17 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
                 const double xa = ax[c0 + lid], ya = ay[c0 + lid];
                 sx[lid] = xa;
                 sy[lid] = ya;
-                // bits 16..19: the waves (pixel rows 4w..4w+3) that may have a pixel within
-                // 8 A; the distance to each wave's pixel rectangle, in pixels and in float,
-                // is tested against 64.01 A^2, a margin far above the float error.  Untilted
-                // only (a tilted metric is no lateral distance): all waves otherwise.
+                // bits 16..19: the waves (pixel rows 4w..4w+3 of the tile's 16 NX columns) that
+                // may have a pixel within 8 A; the distance to each wave's pixel rectangle, in
+                // pixels and in float, is tested against 64.01 A^2, a margin far above the float
+                // error.  Untilted only (a tilted metric is no lateral distance): all waves
+                // otherwise.
                 unsigned waves = 15u;
                 if (a.untilted) {
                     waves = 0;
                     const float u = (float)(xa - tile_x0) * inv_ps;
                     const float v = (float)(ya - tile_y0) * inv_ps;
-                    const float du = fmaxf(fmaxf(-u, u - 15.f), 0.f);
+                    const float du = fmaxf(fmaxf(-u, u - (16.f * NX - 1.f)), 0.f);
 #pragma unroll
                     for (int w = 0; w < 4; ++w) {
                         const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
@@ -518,36 +528,40 @@ __device__ __forceinline__ double projected_sumz(const double* __restrict__ ax, 
 #pragma unroll
                     for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
                 }
-                const double rad_x = im_pos_x - sx[l];
+                const double xl = sx[l];
                 const double rad_y = im_pos_y - sy[l];
-                double r2;
-                if (a.untilted) {
-                    // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad vanish
-                    // below the rounding of x'^2 + y'^2, which equals this exactly
-                    r2 = rad_x * rad_x + rad_y * rad_y;
-                } else {
-                    const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
-                    const double zt = z_prime / a.tan_theta;
-                    const double x_prime = rad_x + zt * a.cos_phi;
-                    const double y_prime = rad_y + zt * a.sin_phi;
-                    r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
-                }
-                // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the reference's test,
-                // correctly rounded double sqrt): sqrt(64 + 2^-46) = 8 + 2^-50 - 2^-103... rounds
-                // to 8 while sqrt of the next double, 64 + 2^-45, rounds to 8 + 2^-49, so the test
-                // is exactly r2 <= 64 + 2^-46.  The potential takes rad in float.
-                if (r2 <= 0x1.0000000000001p+6) {
-                    if constexpr (TAB) {
-                        sumz += (double)vtab_eval(tz, fmaxf(r2 < r_min2 ? r_minf * r_minf : (float)r2, kVtabFloor));
+#pragma unroll
+                for (int i = 0; i < NX; ++i) {
+                    const double rad_x = im_pos_x[i] - xl;
+                    double r2;
+                    if (a.untilted) {
+                        // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad vanish
+                        // below the rounding of x'^2 + y'^2, which equals this exactly
+                        r2 = rad_x * rad_x + rad_y * rad_y;
                     } else {
-                        const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
-                        sumz += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                        const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
+                        const double zt = z_prime / a.tan_theta;
+                        const double x_prime = rad_x + zt * a.cos_phi;
+                        const double y_prime = rad_y + zt * a.sin_phi;
+                        r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
+                    }
+                    // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the reference's
+                    // test, correctly rounded double sqrt): sqrt(64 + 2^-46) = 8 + 2^-50 -
+                    // 2^-103... rounds to 8 while sqrt of the next double, 64 + 2^-45, rounds to
+                    // 8 + 2^-49, so the test is exactly r2 <= 64 + 2^-46.  The potential takes rad
+                    // in float.
+                    if (r2 <= 0x1.0000000000001p+6) {
+                        if constexpr (TAB) {
+                            sumz[i] += (double)vtab_eval(tz, fmaxf(r2 < r_min2 ? r_minf * r_minf : (float)r2, kVtabFloor));
+                        } else {
+                            const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
+                            sumz[i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                        }
                     }
                 }
             }
         }
     }
-    return sumz;
 }
 
 // The LDS copy of the tables a kernel instantiated with LDS = true reads: buckets
@@ -565,7 +579,7 @@ __device__ __forceinline__ void fill_lds_tables(const PotArgs& a, float4* lt) {
 
 // Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
 // (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
-// slice (relative to first).
+// slice (relative to first).  Grid (res / 32, res / 16, planes): 32x16 pixel tiles.
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -575,10 +589,16 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     __shared__ int sZ[256];
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    const double sumz = projected_sumz<FORM, TAB>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ);
-    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    // two pixels per thread, 16 columns apart (32x16 pixel tiles): the atom's staging, loads
+    // and wave test serve both
+    double sumz[2];
+    projected_sumz<FORM, TAB, 2>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
-    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int xid = (blockIdx.x * 2 + i) * 16 + (threadIdx.x & 15);
+        out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz[i]);
+    }
 }
 
 // The phase sigma V of the same slices, reduced to [-pi, pi] as phase_to_t reduces it, so
@@ -591,11 +611,12 @@ __global__ __launch_bounds__(256) void k_phase_projected(
     PotArgs a, int first) {
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
-    const double sumz = projected_sumz<FORM, TAB>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ);
+    double sumz[1];
+    projected_sumz<FORM, TAB, 1>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ, sumz);
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
-        (float)reduce_2pi(a.sigma * sumz);
+        (float)reduce_2pi(a.sigma * sumz[0]);
 }
 
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the

@@ -1192,6 +1192,7 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
                       float2* out, int planes, int first) {
     require_elements(e, c);
     const dim3 grid(e->res / 16, e->res / 16, planes);
+    const dim3 grid_proj(e->res / 32, e->res / 16, planes);  // 32x16 pixel tiles
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);
     const bool tab = a.vtab != nullptr;
@@ -1216,14 +1217,14 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
                     : mse::k_potential_full3d<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
         } else {
             if (e->form == 0)
-                tab ? mse::k_potential_projected<0, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
-                    : mse::k_potential_projected<0><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<0, true><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<0><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
             else if (e->form == 1)
-                tab ? mse::k_potential_projected<1, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
-                    : mse::k_potential_projected<1><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<1, true><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<1><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
             else
-                tab ? mse::k_potential_projected<2, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
-                    : mse::k_potential_projected<2><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
+                tab ? mse::k_potential_projected<2, true><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first)
+                    : mse::k_potential_projected<2><<<grid_proj, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.Z.p, c.block_start.p, e->params_proj.p, a, first);
         }
     });
 }
