@@ -579,7 +579,9 @@ __device__ __forceinline__ void fill_lds_tables(const PotArgs& a, float4* lt) {
 
 // Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
 // (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
-// slice (relative to first).  Grid (res / 32, res / 16, planes): 32x16 pixel tiles.
+// slice (relative to first).  Grid (res / (16 kProjNX), res / 16, planes): (16 kProjNX)x16
+// pixel tiles.
+constexpr int kProjNX = 4;
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -589,14 +591,14 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     __shared__ int sZ[256];
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    // two pixels per thread, 16 columns apart (32x16 pixel tiles): the atom's staging, loads
-    // and wave test serve both
-    double sumz[2];
-    projected_sumz<FORM, TAB, 2>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
+    // kProjNX pixels per thread, 16 columns apart: the atom's staging, loads and wave test
+    // serve them all
+    double sumz[kProjNX];
+    projected_sumz<FORM, TAB, kProjNX>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int xid = (blockIdx.x * 2 + i) * 16 + (threadIdx.x & 15);
+    for (int i = 0; i < kProjNX; ++i) {
+        const int xid = (blockIdx.x * kProjNX + i) * 16 + (threadIdx.x & 15);
         out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz[i]);
     }
 }
