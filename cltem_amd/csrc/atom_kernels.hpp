@@ -623,28 +623,34 @@ __global__ __launch_bounds__(256) void k_phase_projected(
 
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the
 // slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
-// kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Returns sumz at one
-// pixel of slice `slice` (16x16 pixel tile per block).
-template <int FORM, bool TAB = false, bool LDS = false>
-__device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
-                                              const double* __restrict__ az, const int* __restrict__ aZ,
-                                              const int* __restrict__ bsp, const float* __restrict__ tab3d,
-                                              const PotArgs& a, int slice, double* sx, double* sy, double* sz,
-                                              int* sZ, int* sh0, float* srz, const float4* lt = nullptr) {
+// kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Sums sumz of slice
+// `slice` at NX pixels per thread, 16 columns apart: one (16 NX)x16 pixel tile per block, the
+// 16x16 groups blockIdx.x * NX .. + NX - 1 of the reference's tiling; each pixel keeps its own
+// carried p2.
+template <int FORM, bool TAB = false, bool LDS = false, int NX = 1>
+__device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
+                                            const double* __restrict__ az, const int* __restrict__ aZ,
+                                            const int* __restrict__ bsp, const float* __restrict__ tab3d,
+                                            const PotArgs& a, int slice, double* sx, double* sy, double* sz,
+                                            int* sZ, int* sh0, float* srz, const float4* lt, double (&sumz)[NX]) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
-    const int xid = blockIdx.x * 16 + (lid & 15);
     const int yid = blockIdx.y * 16 + (lid >> 4);
     int starti, endi, startj, endj;
     // a pixel reaches 8 A laterally, plus the beam-tilt shift over the integration points
     const double drift = (a.integrals + 1) * fmax(fabs(a.shift_x), fabs(a.shift_y));
-    tile_block_range_within(a, blockIdx.x, blockIdx.y, 8.01 + drift, starti, endi, startj, endj);
+    tile_block_range_within(a, blockIdx.x * NX, blockIdx.y, 8.01 + drift, starti, endi, startj, endj, NX);
     int topz = slice - a.load_z, bottomz = slice + a.load_z;
     if (topz < 0) topz = 0;
     if (bottomz >= a.n_slices) bottomz = a.n_slices - 1;
     const double int_r = 1.0 / a.integrals;
     const double current_z = a.z_min + (a.n_slices - slice) * a.dz;
-    const double im_pos_x = a.startx + xid * a.pixelscale;
+    double im_pos_x[NX];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const int xid = (blockIdx.x * NX + i) * 16 + (lid & 15);
+        im_pos_x[i] = a.startx + xid * a.pixelscale;
+    }
     const double im_pos_y = a.starty + yid * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
     // distances below r_min are evaluated at r_min (squared, as the potentials take r^2)
@@ -658,17 +664,20 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
     const float r_hi2 = __uint_as_float(((unsigned)(a.vb_lo + a.vb_n) << mse_tab::kMantBits) - 1u);
     const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
-    const double tile_x0 = a.startx + (int)(blockIdx.x * 16) * a.pixelscale;
+    const double tile_x0 = a.startx + (int)(blockIdx.x * NX * 16) * a.pixelscale;
     const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
     const float inv_ps = a.inv_ps;
     const float reach_px2 = a.reach_px2;
-    double sumz = 0.0;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) sumz[i] = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int k = topz; k <= bottomz; ++k) {
         for (int j = startj; j <= endj; ++j) {
             const int start = bsp[k * nb + a.blocks_x * j + starti];
             const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
-            double p2 = 0.0;
+            double p2[NX];
+#pragma unroll
+            for (int i = 0; i < NX; ++i) p2[i] = 0.0;
             for (int c0 = start; c0 < end; c0 += 256) {
                 const int cnt = min(256, end - c0);
                 __syncthreads();
@@ -700,7 +709,7 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         if (h0 <= a.integrals) {
                             const float u = (float)(sx[lid] - tile_x0) * inv_ps;
                             const float v = (float)(sy[lid] - tile_y0) * inv_ps;
-                            const float du = fmaxf(fmaxf(-u, u - 15.f), 0.f);
+                            const float du = fmaxf(fmaxf(-u, u - (16.f * NX - 1.f)), 0.f);
 #pragma unroll
                             for (int w = 0; w < 4; ++w) {
                                 const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
@@ -721,8 +730,8 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         if (!((info >> (16 + wave)) & 1)) continue;
                         h0 = info & 0xffff;
                     }
-                    double rad_x = im_pos_x - sx[l];
-                    double rad_y = im_pos_y - sy[l];
+                    const double xl = sx[l];
+                    const double rad_y0 = im_pos_y - sy[l];
                     // the element's constants (scalar registers) or table, loaded once per atom
                     float pz[TAB ? 1 : kPot3dStride];
                     unsigned long long tz = 0;
@@ -737,21 +746,31 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                         for (int q = 0; q < kPot3dStride; ++q) pz[q] = row[q];
                     }
                     if (no_shift) {
-                        // the xy distance is the same at every integration point
-                        const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
-                        if (xyrad2 > 64.0) continue;
+                        // the xy distance of each pixel is the same at every integration point;
+                        // a pixel beyond 8 A takes no part (no sum, no carried p2)
+                        bool in[NX];
+                        float xy2f[NX];
+                        bool any = false;
+#pragma unroll
+                        for (int i = 0; i < NX; ++i) {
+                            const double rad_x = im_pos_x[i] - xl;
+                            const double xyrad2 = rad_x * rad_x + rad_y0 * rad_y0;
+                            in[i] = xyrad2 <= 64.0;
+                            xy2f[i] = (float)xyrad2;
+                            any = any || in[i];
+                        }
+                        if (!any) continue;
                         // from there the depth steps in float and the atom's trapezoid terms are
                         // summed in float, then added to the pixel's double sum once
-                        const float xy2f = (float)xyrad2;
                         const float stepf = (float)(a.dz * int_r);
                         const float rz0 = srz[l];
-                        float p2f = (float)p2, acc = 0.f;
                         if constexpr (TAB) {
                             // points h0..integrals at depths rz0 - (h - h0) step; the trapezoid
                             //   sum_{h > 0} (p(h) + p(h - 1)) = 2 sum_h p(h) - p(integrals) + c,
                             // c = p2 (carried from the row's previous atom) when h0 > 0 and
                             // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
-                            // with s the sum over h0 + 1..integrals
+                            // with s the sum over h0 + 1..integrals.  The pixels share the depths;
+                            // one beyond 8 A is evaluated (within the table) and discarded.
                             auto eval = [&](float r2) {
                                 if constexpr (LDS)
                                     return vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2));
@@ -759,68 +778,95 @@ __device__ __forceinline__ double full3d_sumz(const double* __restrict__ ax, con
                                     return vtab_eval(tz, fmaxf(r2, r_lo2));
                             };
                             float rz = rz0;
-                            const float pf = eval(fmaf(rz, rz, xy2f));
-                            float s = 0.f, pl = pf;
+                            float pf[NX], s[NX], pl[NX];
+#pragma unroll
+                            for (int i = 0; i < NX; ++i) {
+                                pf[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                s[i] = 0.f;
+                                pl[i] = pf[i];
+                            }
                             // two points per trip (the lookups' asm keeps the compiler from
                             // unrolling), added in order
                             int h = h0;
                             for (; h + 1 < a.integrals; h += 2) {
                                 const float rza = rz - stepf;
                                 rz = rza - stepf;
-                                const float pa = eval(fmaf(rza, rza, xy2f));
-                                pl = eval(fmaf(rz, rz, xy2f));
-                                s += pa;
-                                s += pl;
+#pragma unroll
+                                for (int i = 0; i < NX; ++i) {
+                                    const float pa = eval(fmaf(rza, rza, xy2f[i]));
+                                    pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                    s[i] += pa;
+                                    s[i] += pl[i];
+                                }
                             }
                             if (h < a.integrals) {
                                 rz -= stepf;
-                                pl = eval(fmaf(rz, rz, xy2f));
-                                s += pl;
+#pragma unroll
+                                for (int i = 0; i < NX; ++i) {
+                                    pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                    s[i] += pl[i];
+                                }
                             }
-                            acc = (2.f * s - pl) + (h0 > 0 ? 2.f * pf + p2f : pf);
-                            sumz += 0.5 * (double)acc;
-                            p2 = (double)pl;
+#pragma unroll
+                            for (int i = 0; i < NX; ++i) {
+                                if (!in[i]) continue;
+                                const float acc = (2.f * s[i] - pl[i]) + (h0 > 0 ? 2.f * pf[i] + (float)p2[i] : pf[i]);
+                                sumz[i] += 0.5 * (double)acc;
+                                p2[i] = (double)pl[i];
+                            }
                             continue;
                         }
-                        for (int h = h0; h <= a.integrals; ++h) {
-                            const float rz = fmaf(-(float)(h - h0), stepf, rz0);
-                            const float r2 = fmaf(rz, rz, xy2f);
-                            const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
-                            if (h != 0) acc += p1 + p2f;
-                            p2f = p1;
+#pragma unroll
+                        for (int i = 0; i < NX; ++i) {
+                            if (!in[i]) continue;
+                            float p2f = (float)p2[i], acc = 0.f;
+                            for (int h = h0; h <= a.integrals; ++h) {
+                                const float rz = fmaf(-(float)(h - h0), stepf, rz0);
+                                const float r2 = fmaf(rz, rz, xy2f[i]);
+                                const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
+                                if (h != 0) acc += p1 + p2f;
+                                p2f = p1;
+                            }
+                            sumz[i] += 0.5 * (double)acc;
+                            p2[i] = (double)p2f;
                         }
-                        sumz += 0.5 * (double)acc;
-                        p2 = (double)p2f;
                         continue;
                     }
-                    for (int h = 0; h <= a.integrals; ++h) {
-                        rad_x -= a.shift_x;
-                        rad_y -= a.shift_y;
-                        const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
-                        const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
-                        if (xyrad2 <= 64.0 && rad_z <= 3.0) {
-                            const double r2 = xyrad2 + rad_z * rad_z;
-                            float p1f;
-                            if constexpr (LDS)  // not launched with a tilt; kept within the copy
-                                p1f = vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2 < r_min2 ? r_min2f : (float)r2, r_lo2, r_hi2));
-                            else if constexpr (TAB)
-                                p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
-                            else
-                                p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
-                            const double p1 = (double)p1f;
-                            sumz += (h != 0) * (p1 + p2) * 0.5;
-                            p2 = p1;
+#pragma unroll
+                    for (int i = 0; i < NX; ++i) {
+                        double rad_x = im_pos_x[i] - xl;
+                        double rad_y = rad_y0;
+                        for (int h = 0; h <= a.integrals; ++h) {
+                            rad_x -= a.shift_x;
+                            rad_y -= a.shift_y;
+                            const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
+                            const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
+                            if (xyrad2 <= 64.0 && rad_z <= 3.0) {
+                                const double r2 = xyrad2 + rad_z * rad_z;
+                                float p1f;
+                                if constexpr (LDS)  // not launched with a tilt; kept within the copy
+                                    p1f = vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2 < r_min2 ? r_min2f : (float)r2, r_lo2, r_hi2));
+                                else if constexpr (TAB)
+                                    p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
+                                else
+                                    p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
+                                const double p1 = (double)p1f;
+                                sumz[i] += (h != 0) * (p1 + p2[i]) * 0.5;
+                                p2[i] = p1;
+                            }
                         }
                     }
                 }
             }
         }
     }
-    return sumz;
 }
 
 // LDS = true (untilted beams only): the elements' tables are first copied into dynamic LDS
-// (v_nel * vb_n * 16 bytes), where the integration points read them.
+// (v_nel * vb_n * 16 bytes), where the integration points read them, and each thread sums
+// kFull3dNX pixels 16 columns apart: grid (res / (16 kFull3dNX), res / 16, planes) then,
+// (res / 16, res / 16, planes) otherwise.
+constexpr int kFull3dNX = 2;
 template <int FORM, bool TAB = false, bool LDS = false>
 __global__ __launch_bounds__(256) void k_potential_full3d(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -833,11 +879,16 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     if constexpr (LDS) fill_lds_tables(a, lt);
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    const double sumz = full3d_sumz<FORM, TAB, LDS>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ, sh0, srz, lt);
-    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
+    constexpr int NX = LDS ? kFull3dNX : 1;
+    double sumz[NX];
+    full3d_sumz<FORM, TAB, LDS, NX>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ, sh0, srz, lt, sumz);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);
-    out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(sst * a.sigma * sumz);
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const int xid = (blockIdx.x * NX + i) * 16 + (threadIdx.x & 15);
+        out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(sst * a.sigma * sumz[i]);
+    }
 }
 
 // The full-3D phase sst sigma sumz of the same slices, reduced to [-pi, pi] (used by the adjoint).
@@ -849,7 +900,9 @@ __global__ __launch_bounds__(256) void k_phase_full3d(
     __shared__ double sx[256], sy[256], sz[256];
     __shared__ int sZ[256], sh0[256];
     __shared__ float srz[256];
-    const double sumz = full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ, sh0, srz);
+    double sumz1[1];
+    full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ, sh0, srz, nullptr, sumz1);
+    const double sumz = sumz1[0];
     const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
     const double sst = a.dz * (1.0 / a.integrals);

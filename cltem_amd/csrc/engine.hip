@@ -1193,6 +1193,7 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
     require_elements(e, c);
     const dim3 grid(e->res / 16, e->res / 16, planes);
     const dim3 grid_proj(e->res / (16 * mse::kProjNX), e->res / 16, planes);  // (16 kProjNX)x16 pixel tiles
+    const dim3 grid_lds(e->res / (16 * mse::kFull3dNX), e->res / 16, planes);
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);
     const bool tab = a.vtab != nullptr;
@@ -1200,11 +1201,11 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
     launch(e, PK_POT, [&]() {
         if (g.full3d && lds) {
             if (e->form == 0)
-                mse::k_potential_full3d<0, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                mse::k_potential_full3d<0, true, true><<<grid_lds, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
             else if (e->form == 1)
-                mse::k_potential_full3d<1, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                mse::k_potential_full3d<1, true, true><<<grid_lds, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
             else
-                mse::k_potential_full3d<2, true, true><<<grid, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
+                mse::k_potential_full3d<2, true, true><<<grid_lds, 256, lds, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first);
         } else if (g.full3d) {
             if (e->form == 0)
                 tab ? mse::k_potential_full3d<0, true><<<grid, 256, 0, e->stream>>>(out, c.x.p, c.y.p, c.z.p, c.Z.p, c.block_start.p, e->params3d.p, a, first)
