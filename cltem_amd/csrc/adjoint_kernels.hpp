@@ -218,7 +218,8 @@ __device__ __forceinline__ void add_grad(double* __restrict__ grad, const int* _
 }
 
 // Position gradients of the full-3D phase of slices first + blockIdx.z: the same 16x16 pixel tiles, block
-// rows, atoms and integration points as full3d_sumz, in the same order.  With p1_e the
+// rows, atoms and integration points as full3d_sumz with one pixel per thread (k_phase_full3d),
+// in the same order.  With p1_e the
 // potential at evaluation e of a (k, j) row sequence, the trapezoid gives
 //   sumz = sum_e 0.5 [h_e != 0] (p1_e + p1_prev(e)),
 // so p1_e carries 0.5 [h_e != 0] from its own step and 0.5 from the next evaluation when
