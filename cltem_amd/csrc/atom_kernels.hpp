@@ -2,6 +2,7 @@
 #pragma once
 #include "potential_table.hpp"
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace mse {
 
@@ -750,14 +751,17 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                         // a pixel beyond 8 A takes no part (no sum, no carried p2)
                         bool in[NX];
                         float xy2f[NX];
-                        bool any = false;
+                        bool any = false, low = false;
 #pragma unroll
                         for (int i = 0; i < NX; ++i) {
                             const double rad_x = im_pos_x[i] - xl;
                             const double xyrad2 = rad_x * rad_x + rad_y0 * rad_y0;
                             in[i] = xyrad2 <= 64.0;
-                            xy2f[i] = (float)xyrad2;
+                            // a pixel that takes no part is evaluated at the smallest table
+                            // distance, so its discarded lookups stay within the tables
+                            xy2f[i] = in[i] ? (float)xyrad2 : r_lo2;
                             any = any || in[i];
+                            low = low || xy2f[i] < r_lo2;
                         }
                         if (!any) continue;
                         // from there the depth steps in float and the atom's trapezoid terms are
@@ -771,49 +775,66 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                             // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
                             // with s the sum over h0 + 1..integrals.  The pixels share the depths;
                             // one beyond 8 A is evaluated (within the table) and discarded.
-                            auto eval = [&](float r2) {
-                                if constexpr (LDS)
-                                    return vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2));
-                                else
-                                    return vtab_eval(tz, fmaxf(r2, r_lo2));
+                            //
+                            // With the tables in LDS the clamp to the staged range is left out
+                            // when no lane of the wave needs it: every r^2 = fma(rz, rz, xy2) is
+                            // at least xy2 >= r_lo2, and at most 64 + zmax^2, zmax bounding |rz|
+                            // over the atom's points (|rz0| + its depth span; the 1e-3 margin
+                            // covers the float steps' rounding, < 4096 * 2^-24 relative).
+                            bool clamp = true;
+                            if constexpr (LDS) {
+                                const float zmax = (fabsf(rz0) + (float)(a.integrals - h0) * stepf) * 1.001f;
+                                clamp = __builtin_amdgcn_ballot_w64(low || !(fmaf(zmax, zmax, 64.f) <= r_hi2)) != 0;
+                            }
+                            auto sweep = [&](auto clamped) {
+                                auto eval = [&](float r2) {
+                                    if constexpr (LDS)
+                                        return vtab_eval_lds(lt, loff, decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2);
+                                    else
+                                        return vtab_eval(tz, fmaxf(r2, r_lo2));
+                                };
+                                float rz = rz0;
+                                float pf[NX], s[NX], pl[NX];
+#pragma unroll
+                                for (int i = 0; i < NX; ++i) {
+                                    pf[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                    s[i] = 0.f;
+                                    pl[i] = pf[i];
+                                }
+                                // two points per trip (the lookups' asm keeps the compiler from
+                                // unrolling), added in order
+                                int h = h0;
+                                for (; h + 1 < a.integrals; h += 2) {
+                                    const float rza = rz - stepf;
+                                    rz = rza - stepf;
+#pragma unroll
+                                    for (int i = 0; i < NX; ++i) {
+                                        const float pa = eval(fmaf(rza, rza, xy2f[i]));
+                                        pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                        s[i] += pa;
+                                        s[i] += pl[i];
+                                    }
+                                }
+                                if (h < a.integrals) {
+                                    rz -= stepf;
+#pragma unroll
+                                    for (int i = 0; i < NX; ++i) {
+                                        pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                        s[i] += pl[i];
+                                    }
+                                }
+#pragma unroll
+                                for (int i = 0; i < NX; ++i) {
+                                    if (!in[i]) continue;
+                                    const float acc = (2.f * s[i] - pl[i]) + (h0 > 0 ? 2.f * pf[i] + (float)p2[i] : pf[i]);
+                                    sumz[i] += 0.5 * (double)acc;
+                                    p2[i] = (double)pl[i];
+                                }
                             };
-                            float rz = rz0;
-                            float pf[NX], s[NX], pl[NX];
-#pragma unroll
-                            for (int i = 0; i < NX; ++i) {
-                                pf[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                s[i] = 0.f;
-                                pl[i] = pf[i];
-                            }
-                            // two points per trip (the lookups' asm keeps the compiler from
-                            // unrolling), added in order
-                            int h = h0;
-                            for (; h + 1 < a.integrals; h += 2) {
-                                const float rza = rz - stepf;
-                                rz = rza - stepf;
-#pragma unroll
-                                for (int i = 0; i < NX; ++i) {
-                                    const float pa = eval(fmaf(rza, rza, xy2f[i]));
-                                    pl[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                    s[i] += pa;
-                                    s[i] += pl[i];
-                                }
-                            }
-                            if (h < a.integrals) {
-                                rz -= stepf;
-#pragma unroll
-                                for (int i = 0; i < NX; ++i) {
-                                    pl[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                    s[i] += pl[i];
-                                }
-                            }
-#pragma unroll
-                            for (int i = 0; i < NX; ++i) {
-                                if (!in[i]) continue;
-                                const float acc = (2.f * s[i] - pl[i]) + (h0 > 0 ? 2.f * pf[i] + (float)p2[i] : pf[i]);
-                                sumz[i] += 0.5 * (double)acc;
-                                p2[i] = (double)pl[i];
-                            }
+                            if (!LDS || clamp)
+                                sweep(std::true_type{});
+                            else
+                                sweep(std::false_type{});
                             continue;
                         }
 #pragma unroll
