@@ -553,7 +553,9 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
                     // in float.
                     if (r2 <= 0x1.0000000000001p+6) {
                         if constexpr (TAB) {
-                            sumz[i] += (double)vtab_eval(tz, fmaxf(r2 < r_min2 ? r_minf * r_minf : (float)r2, kVtabFloor));
+                            // r^2 >= +0, so the floor is the larger bit pattern
+                            const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
+                            sumz[i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
                         } else {
                             const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
                             sumz[i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
