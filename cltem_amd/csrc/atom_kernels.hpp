@@ -387,16 +387,16 @@ __device__ __forceinline__ int plane_args(const PotArgs& a, int first, PotArgs& 
     return first + (int)blockIdx.z;
 }
 
-// The blocks of the 16x16 pixel groups gx..gx + nx - 1 (the union of their ranges: the
-// range's ends are monotone in the group)
+// The blocks of the 16x16 pixel groups gx..gx + nx - 1, gy..gy + ny - 1 (the union of their
+// ranges: the range's ends are monotone in the group)
 __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int gy, int& starti,
-                                                 int& endi, int& startj, int& endj, int nx = 1) {
+                                                 int& endi, int& startj, int& endj, int nx = 1, int ny = 1) {
     // kernels/transmission_potentials_projected_d.cl:227-246 with a 16x16 group
     const double gsx = 16 * a.pixelscale, gsy = 16 * a.pixelscale;
     const double group_start_x = a.startx + gx * gsx;
     const double group_end_x = (a.startx + (gx + nx - 1) * gsx) + gsx;
     const double group_start_y = a.starty + gy * gsy;
-    const double group_end_y = group_start_y + gsy;
+    const double group_end_y = (a.starty + (gy + ny - 1) * gsy) + gsy;
     const double rrx = a.rrx, rry = a.rry;
     starti = (int)fmax(floor(a.blocks_x * (group_start_x - a.min_x) * rrx) - a.load_x, 0.0);
     endi = (int)fmin(ceil(a.blocks_x * (group_end_x - a.min_x) * rrx) + a.load_x, (double)(a.blocks_x - 1));
@@ -412,13 +412,13 @@ __device__ __forceinline__ void tile_block_range(const PotArgs& a, int gx, int g
 // full-3D trapezoid, carry nothing to the next atom.
 __device__ __forceinline__ void tile_block_range_within(const PotArgs& a, int gx, int gy, double reach,
                                                         int& starti, int& endi, int& startj, int& endj,
-                                                        int nx = 1) {
-    tile_block_range(a, gx, gy, starti, endi, startj, endj, nx);
+                                                        int nx = 1, int ny = 1) {
+    tile_block_range(a, gx, gy, starti, endi, startj, endj, nx, ny);
     const double rrx = a.rrx, rry = a.rry;
     const double x_lo = a.startx + (gx * 16) * a.pixelscale - reach;
     const double x_hi = a.startx + ((gx + nx) * 16 - 1) * a.pixelscale + reach;
     const double y_lo = a.starty + (gy * 16) * a.pixelscale - reach;
-    const double y_hi = a.starty + (gy * 16 + 15) * a.pixelscale + reach;
+    const double y_hi = a.starty + ((gy + ny) * 16 - 1) * a.pixelscale + reach;
     starti = max(starti, (int)fmax(floor((x_lo - a.min_x) * rrx * a.blocks_x), -1.0));
     endi = min(endi, (int)fmin(floor((x_hi - a.min_x) * rrx * a.blocks_x), (double)a.blocks_x));
     startj = max(startj, (int)fmax(floor((y_lo - a.min_y) * rry * a.blocks_y), -1.0));
@@ -446,44 +446,55 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
     return make_float2(c, s);
 }
 
-// Projected potential sums (sumz) of slice `slice` at NX pixels per thread, 16 columns apart:
-// one (16 NX)x16 pixel tile per block, the 16x16 groups blockIdx.x * NX .. + NX - 1 of the
-// reference's tiling; the atoms of each block row are staged through LDS in chunks of 256,
-// with no limit on how many a row holds.  (The tables stay in global memory here: one
-// evaluation per atom and pixel is too little work to repay an LDS copy per block.)
-template <int FORM, bool TAB, int NX = 1>
+// Projected potential sums (sumz) of slice `slice` at PX x PY pixels per thread: one
+// (16 PX)x(16 PY) pixel tile per block, the 16x16 groups of the reference's tiling
+// blockIdx.x * PX .. + PX - 1 by blockIdx.y * PY .. + PY - 1.  Wave w takes the
+// (8 PX)x(8 PY) quadrant (w & 1, w >> 1) of the tile, a square footprint for the waves' cull;
+// lane (lx, ly) = (l & 7, l >> 3) of it takes the pixels (lx + 8 i, ly + 8 j), sumz[j PX + i].
+// The atoms of each block row are staged through LDS in chunks of 256, with no limit on how
+// many a row holds.  (The tables stay in global memory here: one evaluation per atom and pixel
+// is too little work to repay an LDS copy per block.)
+template <int PX, int PY>
+__device__ __forceinline__ int proj_pixel_x(int i) {
+    const int w = threadIdx.x >> 6;
+    return (int)blockIdx.x * 16 * PX + (w & 1) * 8 * PX + (threadIdx.x & 7) + 8 * i;
+}
+template <int PX, int PY>
+__device__ __forceinline__ int proj_pixel_y(int j) {
+    const int w = threadIdx.x >> 6;
+    return (int)blockIdx.y * 16 * PY + (w >> 1) * 8 * PY + ((threadIdx.x >> 3) & 7) + 8 * j;
+}
+
+template <int FORM, bool TAB, int PX = 1, int PY = 1>
 __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                const int* __restrict__ aZ, const int* __restrict__ bsp,
                                                const float* __restrict__ ptab, const PotArgs& a, int slice,
-                                               double* sx, double* sy, int* sZ, double (&sumz)[NX]) {
+                                               double* sx, double* sy, int* sZ, double (&sumz)[PX * PY]) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
-    const int yid = blockIdx.y * 16 + (lid >> 4);
     int starti, endi, startj, endj;
     // untilted, a pixel reaches r <= 8 A (r2 <= 64.0000000001) laterally
     if (a.untilted)
-        tile_block_range_within(a, blockIdx.x * NX, blockIdx.y, 8.01, starti, endi, startj, endj, NX);
+        tile_block_range_within(a, blockIdx.x * PX, blockIdx.y * PY, 8.01, starti, endi, startj, endj, PX, PY);
     else
-        tile_block_range(a, blockIdx.x * NX, blockIdx.y, starti, endi, startj, endj, NX);
+        tile_block_range(a, blockIdx.x * PX, blockIdx.y * PY, starti, endi, startj, endj, PX, PY);
     int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
-    double im_pos_x[NX];
+    double im_pos_x[PX], im_pos_y[PY];
 #pragma unroll
-    for (int i = 0; i < NX; ++i) {
-        const int xid = (blockIdx.x * NX + i) * 16 + (lid & 15);
-        im_pos_x[i] = a.startx + xid * a.pixelscale;
-    }
-    const double im_pos_y = a.starty + yid * a.pixelscale;
+    for (int i = 0; i < PX; ++i) im_pos_x[i] = a.startx + proj_pixel_x<PX, PY>(i) * a.pixelscale;
+#pragma unroll
+    for (int j = 0; j < PY; ++j) im_pos_y[j] = a.starty + proj_pixel_y<PX, PY>(j) * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
     const double r_min2 = r_min * r_min;
     const float r_minf = (float)r_min;
     const int wave = __builtin_amdgcn_readfirstlane(lid >> 6);
     // the tile's first pixel, 1 / pixel size and (8 A + margin)^2 in pixels^2, for the waves' reach
-    const double tile_x0 = a.startx + (int)(blockIdx.x * NX * 16) * a.pixelscale;
-    const double tile_y0 = a.starty + (int)(blockIdx.y * 16) * a.pixelscale;
+    const double tile_x0 = a.startx + (int)(blockIdx.x * PX * 16) * a.pixelscale;
+    const double tile_y0 = a.starty + (int)(blockIdx.y * PY * 16) * a.pixelscale;
     const float inv_ps = a.inv_ps;
     const float reach_px2 = a.reach_px2;
 #pragma unroll
-    for (int i = 0; i < NX; ++i) sumz[i] = 0.0;
+    for (int i = 0; i < PX * PY; ++i) sumz[i] = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
     for (int j = startj; j <= endj; ++j) {
         const int start = bsp[k * nb + a.blocks_x * j + starti];
@@ -495,20 +506,20 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
                 const double xa = ax[c0 + lid], ya = ay[c0 + lid];
                 sx[lid] = xa;
                 sy[lid] = ya;
-                // bits 16..19: the waves (pixel rows 4w..4w+3 of the tile's 16 NX columns) that
-                // may have a pixel within 8 A; the distance to each wave's pixel rectangle, in
-                // pixels and in float, is tested against 64.01 A^2, a margin far above the float
-                // error.  Untilted only (a tilted metric is no lateral distance): all waves
-                // otherwise.
+                // bits 16..19: the waves (pixel quadrants of the tile) that may have a pixel
+                // within 8 A; the distance to each wave's pixel rectangle, in pixels and in
+                // float, is tested against 64.01 A^2, a margin far above the float error.
+                // Untilted only (a tilted metric is no lateral distance): all waves otherwise.
                 unsigned waves = 15u;
                 if (a.untilted) {
                     waves = 0;
                     const float u = (float)(xa - tile_x0) * inv_ps;
                     const float v = (float)(ya - tile_y0) * inv_ps;
-                    const float du = fmaxf(fmaxf(-u, u - (16.f * NX - 1.f)), 0.f);
 #pragma unroll
                     for (int w = 0; w < 4; ++w) {
-                        const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
+                        const float u0 = (float)((w & 1) * 8 * PX), v0 = (float)((w >> 1) * 8 * PY);
+                        const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * PX - 1.f))), 0.f);
+                        const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + (8.f * PY - 1.f))), 0.f);
                         if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
                     }
                 }
@@ -529,36 +540,39 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
 #pragma unroll
                     for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
                 }
-                const double xl = sx[l];
-                const double rad_y = im_pos_y - sy[l];
+                const double xl = sx[l], yl = sy[l];
 #pragma unroll
-                for (int i = 0; i < NX; ++i) {
-                    const double rad_x = im_pos_x[i] - xl;
-                    double r2;
-                    if (a.untilted) {
-                        // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad vanish
-                        // below the rounding of x'^2 + y'^2, which equals this exactly
-                        r2 = rad_x * rad_x + rad_y * rad_y;
-                    } else {
-                        const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
-                        const double zt = z_prime / a.tan_theta;
-                        const double x_prime = rad_x + zt * a.cos_phi;
-                        const double y_prime = rad_y + zt * a.sin_phi;
-                        r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
-                    }
-                    // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the reference's
-                    // test, correctly rounded double sqrt): sqrt(64 + 2^-46) = 8 + 2^-50 -
-                    // 2^-103... rounds to 8 while sqrt of the next double, 64 + 2^-45, rounds to
-                    // 8 + 2^-49, so the test is exactly r2 <= 64 + 2^-46.  The potential takes rad
-                    // in float.
-                    if (r2 <= 0x1.0000000000001p+6) {
-                        if constexpr (TAB) {
-                            // r^2 >= +0, so the floor is the larger bit pattern
-                            const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
-                            sumz[i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
+                for (int jy = 0; jy < PY; ++jy) {
+                    const double rad_y = im_pos_y[jy] - yl;
+#pragma unroll
+                    for (int i = 0; i < PX; ++i) {
+                        const double rad_x = im_pos_x[i] - xl;
+                        double r2;
+                        if (a.untilted) {
+                            // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad
+                            // vanish below the rounding of x'^2 + y'^2, which equals this exactly
+                            r2 = rad_x * rad_x + rad_y * rad_y;
                         } else {
-                            const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
-                            sumz[i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                            const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
+                            const double zt = z_prime / a.tan_theta;
+                            const double x_prime = rad_x + zt * a.cos_phi;
+                            const double y_prime = rad_y + zt * a.sin_phi;
+                            r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
+                        }
+                        // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the
+                        // reference's test, correctly rounded double sqrt): sqrt(64 + 2^-46) =
+                        // 8 + 2^-50 - 2^-103... rounds to 8 while sqrt of the next double,
+                        // 64 + 2^-45, rounds to 8 + 2^-49, so the test is exactly
+                        // r2 <= 64 + 2^-46.  The potential takes rad in float.
+                        if (r2 <= 0x1.0000000000001p+6) {
+                            if constexpr (TAB) {
+                                // r^2 >= +0, so the floor is the larger bit pattern
+                                const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
+                                sumz[jy * PX + i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
+                            } else {
+                                const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
+                                sumz[jy * PX + i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
+                            }
                         }
                     }
                 }
@@ -582,9 +596,9 @@ __device__ __forceinline__ void fill_lds_tables(const PotArgs& a, float4* lt) {
 
 // Transmission function exp(i sigma V) of slices [first, first + gridDim.z)
 // (kernels/transmission_potentials_projected_f.cl:183-317); out has one res*res plane per
-// slice (relative to first).  Grid (res / (16 kProjNX), res / 16, planes): (16 kProjNX)x16
-// pixel tiles.
-constexpr int kProjNX = 4;
+// slice (relative to first).  Grid (res / (16 kProjPX), res / (16 kProjPY), planes):
+// (16 kProjPX)x(16 kProjPY) pixel tiles, 16x16 pixels per wave.
+constexpr int kProjPX = 2, kProjPY = 2;
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_potential_projected(
     float2* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -594,21 +608,24 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     __shared__ int sZ[256];
     PotArgs b;
     const int slice = plane_args(a, first, b);
-    // kProjNX pixels per thread, 16 columns apart: the atom's staging, loads and wave test
-    // serve them all
-    double sumz[kProjNX];
-    projected_sumz<FORM, TAB, kProjNX>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
-    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    // kProjPX x kProjPY pixels per thread: the atom's staging, loads and wave test serve them all
+    double sumz[kProjPX * kProjPY];
+    projected_sumz<FORM, TAB, kProjPX, kProjPY>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
 #pragma unroll
-    for (int i = 0; i < kProjNX; ++i) {
-        const int xid = (blockIdx.x * kProjNX + i) * 16 + (threadIdx.x & 15);
-        out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(a.sigma * sumz[i]);
+    for (int j = 0; j < kProjPY; ++j) {
+        const int yid = proj_pixel_y<kProjPX, kProjPY>(j);
+#pragma unroll
+        for (int i = 0; i < kProjPX; ++i) {
+            const int xid = proj_pixel_x<kProjPX, kProjPY>(i);
+            out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
+                phase_to_t(a.sigma * sumz[j * kProjPX + i]);
+        }
     }
 }
 
 // The phase sigma V of the same slices, reduced to [-pi, pi] as phase_to_t reduces it, so
 // that sincosf of the stored float reproduces the transmission function's exp(i sigma V)
-// before the band limit (used by the adjoint).
+// before the band limit (used by the adjoint).  Grid (res / 16, res / 16, planes).
 template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_phase_projected(
     float* __restrict__ out, const double* __restrict__ ax, const double* __restrict__ ay,
@@ -617,9 +634,9 @@ __global__ __launch_bounds__(256) void k_phase_projected(
     __shared__ double sx[256], sy[256];
     __shared__ int sZ[256];
     double sumz[1];
-    projected_sumz<FORM, TAB, 1>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ, sumz);
-    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    projected_sumz<FORM, TAB, 1, 1>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ, sumz);
+    const int xid = proj_pixel_x<1, 1>(0);
+    const int yid = proj_pixel_y<1, 1>(0);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
         (float)reduce_2pi(a.sigma * sumz[0]);
 }
@@ -803,21 +820,61 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                                     s[i] = 0.f;
                                     pl[i] = pf[i];
                                 }
-                                // two points per trip (the lookups' asm keeps the compiler from
-                                // unrolling), added in order
+                                // four points per trip (the lookups' asm keeps the compiler from
+                                // unrolling), so that their table reads are in flight together;
+                                // added in order
                                 int h = h0;
-                                for (; h + 1 < a.integrals; h += 2) {
-                                    const float rza = rz - stepf;
-                                    rz = rza - stepf;
+                                for (; h + 3 < a.integrals; h += 4) {
+                                    float rzk[4];
+                                    rzk[0] = rz - stepf;
+                                    rzk[1] = rzk[0] - stepf;
+                                    rzk[2] = rzk[1] - stepf;
+                                    rzk[3] = rzk[2] - stepf;
+                                    rz = rzk[3];
+                                    if constexpr (LDS) {
+                                        // the same lookups as eval, in phases: every bucket index,
+                                        // then every table read, then the cubics
+                                        float r2k[4][NX];
+                                        unsigned qk[4][NX];
+                                        float4 ck[4][NX];
 #pragma unroll
-                                    for (int i = 0; i < NX; ++i) {
-                                        const float pa = eval(fmaf(rza, rza, xy2f[i]));
-                                        pl[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                        s[i] += pa;
-                                        s[i] += pl[i];
+                                        for (int k = 0; k < 4; ++k)
+#pragma unroll
+                                            for (int i = 0; i < NX; ++i) {
+                                                const float r2 = fmaf(rzk[k], rzk[k], xy2f[i]);
+                                                r2k[k][i] = decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2;
+                                                asm("v_lshrrev_b32 %0, %1, %2" : "=v"(qk[k][i]) : "i"(mse_tab::kMantBits), "v"(__float_as_uint(r2k[k][i])));
+                                            }
+#pragma unroll
+                                        for (int k = 0; k < 4; ++k)
+#pragma unroll
+                                            for (int i = 0; i < NX; ++i) ck[k][i] = lt[(int)qk[k][i] + loff];
+#pragma unroll
+                                        for (int i = 0; i < NX; ++i)
+#pragma unroll
+                                            for (int k = 0; k < 4; ++k) {
+                                                const float4 c = ck[k][i];
+                                                const float m = (float)(__float_as_uint(r2k[k][i]) & ((1u << mse_tab::kMantBits) - 1u));
+                                                const float p = fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
+                                                s[i] += p;
+                                                if (k == 3) pl[i] = p;
+                                            }
+                                    } else {
+#pragma unroll
+                                        for (int i = 0; i < NX; ++i) {
+                                            const float pa = eval(fmaf(rzk[0], rzk[0], xy2f[i]));
+                                            const float pb = eval(fmaf(rzk[1], rzk[1], xy2f[i]));
+                                            const float pc = eval(fmaf(rzk[2], rzk[2], xy2f[i]));
+                                            pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                            s[i] += pa;
+                                            s[i] += pb;
+                                            s[i] += pc;
+                                            s[i] += pl[i];
+                                        }
                                     }
                                 }
-                                if (h < a.integrals) {
+#pragma clang loop unroll(disable)
+                                for (; h < a.integrals; ++h) {
                                     rz -= stepf;
 #pragma unroll
                                     for (int i = 0; i < NX; ++i) {

@@ -1192,7 +1192,7 @@ void potential_planes(mse_engine* e, const ConfigData& c, const mse_geometry& g,
                       float2* out, int planes, int first) {
     require_elements(e, c);
     const dim3 grid(e->res / 16, e->res / 16, planes);
-    const dim3 grid_proj(e->res / (16 * mse::kProjNX), e->res / 16, planes);  // (16 kProjNX)x16 pixel tiles
+    const dim3 grid_proj(e->res / (16 * mse::kProjPX), e->res / (16 * mse::kProjPY), planes);
     const dim3 grid_lds(e->res / (16 * mse::kFull3dNX), e->res / 16, planes);
     mse::PotArgs a = args;
     a.vtab = potential_tables(e, c, g);
