@@ -446,25 +446,27 @@ __device__ __forceinline__ float2 phase_to_t(double ph) {
     return make_float2(c, s);
 }
 
-// Projected potential sums (sumz) of slice `slice` at PX x PY pixels per thread: one
-// (16 PX)x(16 PY) pixel tile per block, the 16x16 groups of the reference's tiling
-// blockIdx.x * PX .. + PX - 1 by blockIdx.y * PY .. + PY - 1.  Wave w takes the
-// (8 PX)x(8 PY) quadrant (w & 1, w >> 1) of the tile, a square footprint for the waves' cull;
-// lane (lx, ly) = (l & 7, l >> 3) of it takes the pixels (lx + 8 i, ly + 8 j), sumz[j PX + i].
-// The atoms of each block row are staged through LDS in chunks of 256, with no limit on how
-// many a row holds.  (The tables stay in global memory here: one evaluation per atom and pixel
-// is too little work to repay an LDS copy per block.)
+// The pixels of a thread in the builders' (16 PX)x(16 PY) pixel tiles (block (blockIdx.x,
+// blockIdx.y), the 16x16 groups of the reference's tiling blockIdx.x * PX .. + PX - 1 by
+// blockIdx.y * PY .. + PY - 1): wave w takes the (8 PX)x(8 PY) quadrant (w & 1, w >> 1) of
+// the tile, a compact footprint for the waves' cull, and lane (lx, ly) = (l & 7, l >> 3) of
+// it the pixels (lx + 8 i, ly + 8 j).
 template <int PX, int PY>
-__device__ __forceinline__ int proj_pixel_x(int i) {
+__device__ __forceinline__ int tile_pixel_x(int i) {
     const int w = threadIdx.x >> 6;
     return (int)blockIdx.x * 16 * PX + (w & 1) * 8 * PX + (threadIdx.x & 7) + 8 * i;
 }
 template <int PX, int PY>
-__device__ __forceinline__ int proj_pixel_y(int j) {
+__device__ __forceinline__ int tile_pixel_y(int j) {
     const int w = threadIdx.x >> 6;
     return (int)blockIdx.y * 16 * PY + (w >> 1) * 8 * PY + ((threadIdx.x >> 3) & 7) + 8 * j;
 }
 
+// Projected potential sums (sumz) of slice `slice` at PX x PY pixels per thread, sumz[j PX + i]
+// for pixel (tile_pixel_x(i), tile_pixel_y(j)).  The atoms of each block row are staged
+// through LDS in chunks of 256, with no limit on how many a row holds.  (The tables stay in
+// global memory here: one evaluation per atom and pixel is too little work to repay an LDS
+// copy per block.)
 template <int FORM, bool TAB, int PX = 1, int PY = 1>
 __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                const int* __restrict__ aZ, const int* __restrict__ bsp,
@@ -481,9 +483,9 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
     int k = slice < 0 ? 0 : (slice >= a.n_slices ? a.n_slices - 1 : slice);
     double im_pos_x[PX], im_pos_y[PY];
 #pragma unroll
-    for (int i = 0; i < PX; ++i) im_pos_x[i] = a.startx + proj_pixel_x<PX, PY>(i) * a.pixelscale;
+    for (int i = 0; i < PX; ++i) im_pos_x[i] = a.startx + tile_pixel_x<PX, PY>(i) * a.pixelscale;
 #pragma unroll
-    for (int j = 0; j < PY; ++j) im_pos_y[j] = a.starty + proj_pixel_y<PX, PY>(j) * a.pixelscale;
+    for (int j = 0; j < PY; ++j) im_pos_y[j] = a.starty + tile_pixel_y<PX, PY>(j) * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
     const double r_min2 = r_min * r_min;
     const float r_minf = (float)r_min;
@@ -613,10 +615,10 @@ __global__ __launch_bounds__(256) void k_potential_projected(
     projected_sumz<FORM, TAB, kProjPX, kProjPY>(ax, ay, aZ, bsp, ptab, b, slice, sx, sy, sZ, sumz);
 #pragma unroll
     for (int j = 0; j < kProjPY; ++j) {
-        const int yid = proj_pixel_y<kProjPX, kProjPY>(j);
+        const int yid = tile_pixel_y<kProjPX, kProjPY>(j);
 #pragma unroll
         for (int i = 0; i < kProjPX; ++i) {
-            const int xid = proj_pixel_x<kProjPX, kProjPY>(i);
+            const int xid = tile_pixel_x<kProjPX, kProjPY>(i);
             out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
                 phase_to_t(a.sigma * sumz[j * kProjPX + i]);
         }
@@ -635,8 +637,8 @@ __global__ __launch_bounds__(256) void k_phase_projected(
     __shared__ int sZ[256];
     double sumz[1];
     projected_sumz<FORM, TAB, 1, 1>(ax, ay, aZ, bsp, ptab, a, first + (int)blockIdx.z, sx, sy, sZ, sumz);
-    const int xid = proj_pixel_x<1, 1>(0);
-    const int yid = proj_pixel_y<1, 1>(0);
+    const int xid = tile_pixel_x<1, 1>(0);
+    const int yid = tile_pixel_y<1, 1>(0);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
         (float)reduce_2pi(a.sigma * sumz[0]);
 }
@@ -644,8 +646,8 @@ __global__ __launch_bounds__(256) void k_phase_projected(
 // transmission_potentials_full_3d: trapezoid over integrals+1 sub-slice points for the
 // slices k +- load_z; p2 carries between atoms of one block row exactly as the reference
 // kernel does (kernels/transmission_potentials_full_3d_f.cl:229-278).  Sums sumz of slice
-// `slice` at NX pixels per thread, 16 columns apart: one (16 NX)x16 pixel tile per block, the
-// 16x16 groups blockIdx.x * NX .. + NX - 1 of the reference's tiling; each pixel keeps its own
+// `slice` at NX pixels per thread, sumz[i] for pixel (tile_pixel_x<NX, 1>(i),
+// tile_pixel_y<NX, 1>(0)) of the block's (16 NX)x16 pixel tile; each pixel keeps its own
 // carried p2.
 template <int FORM, bool TAB = false, bool LDS = false, int NX = 1>
 __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
@@ -655,7 +657,7 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                                             int* sZ, int* sh0, float* srz, const float4* lt, double (&sumz)[NX]) {
 #pragma clang fp contract(off)
     const int lid = threadIdx.x;
-    const int yid = blockIdx.y * 16 + (lid >> 4);
+    const int yid = tile_pixel_y<NX, 1>(0);
     int starti, endi, startj, endj;
     // a pixel reaches 8 A laterally, plus the beam-tilt shift over the integration points
     const double drift = (a.integrals + 1) * fmax(fabs(a.shift_x), fabs(a.shift_y));
@@ -667,10 +669,7 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
     const double current_z = a.z_min + (a.n_slices - slice) * a.dz;
     double im_pos_x[NX];
 #pragma unroll
-    for (int i = 0; i < NX; ++i) {
-        const int xid = (blockIdx.x * NX + i) * 16 + (lid & 15);
-        im_pos_x[i] = a.startx + xid * a.pixelscale;
-    }
+    for (int i = 0; i < NX; ++i) im_pos_x[i] = a.startx + tile_pixel_x<NX, 1>(i) * a.pixelscale;
     const double im_pos_y = a.starty + yid * a.pixelscale;
     const double r_min = 0.25 * a.pixelscale;
     // distances below r_min are evaluated at r_min (squared, as the potentials take r^2)
@@ -720,19 +719,20 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                         while (h0 > 0 && within(h0 - 1)) --h0;
 #pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
                         while (h0 <= a.integrals && !within(h0)) ++h0;
-                        // bit 16 + w: wave w (pixel rows 4w..4w+3 of the tile) may have a pixel
-                        // within 8 A.  The atom's distance to the wave's pixel rectangle, in
-                        // pixels and in float, is tested against 64.01 A^2: the float error
-                        // (~1e-6 A) is far below that margin, so a clear bit means every lane
-                        // of the wave fails its own exact test xyrad2 <= 64.
+                        // bit 16 + w: wave w (its (8 NX)x8 pixel quadrant of the tile) may have
+                        // a pixel within 8 A.  The atom's distance to the wave's pixel
+                        // rectangle, in pixels and in float, is tested against 64.01 A^2: the
+                        // float error (~1e-6 A) is far below that margin, so a clear bit means
+                        // every lane of the wave fails its own exact test xyrad2 <= 64.
                         unsigned waves = 0;
                         if (h0 <= a.integrals) {
                             const float u = (float)(sx[lid] - tile_x0) * inv_ps;
                             const float v = (float)(sy[lid] - tile_y0) * inv_ps;
-                            const float du = fmaxf(fmaxf(-u, u - (16.f * NX - 1.f)), 0.f);
 #pragma unroll
                             for (int w = 0; w < 4; ++w) {
-                                const float dv = fmaxf(fmaxf(4.f * w - v, v - (4.f * w + 3.f)), 0.f);
+                                const float u0 = (float)((w & 1) * 8 * NX), v0 = (float)((w >> 1) * 8);
+                                const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * NX - 1.f))), 0.f);
+                                const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + 7.f)), 0.f);
                                 if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
                             }
                         }
@@ -962,11 +962,11 @@ __global__ __launch_bounds__(256) void k_potential_full3d(
     constexpr int NX = LDS ? kFull3dNX : 1;
     double sumz[NX];
     full3d_sumz<FORM, TAB, LDS, NX>(ax, ay, az, aZ, bsp, tab3d, b, slice, sx, sy, sz, sZ, sh0, srz, lt, sumz);
-    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int yid = tile_pixel_y<NX, 1>(0);
     const double sst = a.dz * (1.0 / a.integrals);
 #pragma unroll
     for (int i = 0; i < NX; ++i) {
-        const int xid = (blockIdx.x * NX + i) * 16 + (threadIdx.x & 15);
+        const int xid = tile_pixel_x<NX, 1>(i);
         out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] = phase_to_t(sst * a.sigma * sumz[i]);
     }
 }
@@ -983,8 +983,8 @@ __global__ __launch_bounds__(256) void k_phase_full3d(
     double sumz1[1];
     full3d_sumz<FORM, TAB>(ax, ay, az, aZ, bsp, tab3d, a, first + blockIdx.z, sx, sy, sz, sZ, sh0, srz, nullptr, sumz1);
     const double sumz = sumz1[0];
-    const int xid = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int yid = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int xid = tile_pixel_x<1, 1>(0);
+    const int yid = tile_pixel_y<1, 1>(0);
     const double sst = a.dz * (1.0 / a.integrals);
     out[(size_t)blockIdx.z * a.res * a.res + (size_t)yid * a.res + xid] =
         (float)reduce_2pi(sst * a.sigma * sumz);
