@@ -498,83 +498,101 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
 #pragma unroll
     for (int i = 0; i < PX * PY; ++i) sumz[i] = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
-    for (int j = startj; j <= endj; ++j) {
-        const int start = bsp[k * nb + a.blocks_x * j + starti];
-        const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
-        for (int c0 = start; c0 < end; c0 += 256) {
-            const int cnt = min(256, end - c0);
-            __syncthreads();
-            if (lid < cnt) {
-                const double xa = ax[c0 + lid], ya = ay[c0 + lid];
-                sx[lid] = xa;
-                sy[lid] = ya;
-                // bits 16..19: the waves (pixel quadrants of the tile) that may have a pixel
-                // within 8 A; the distance to each wave's pixel rectangle, in pixels and in
-                // float, is tested against 64.01 A^2, a margin far above the float error.
-                // Untilted only (a tilted metric is no lateral distance): all waves otherwise.
-                unsigned waves = 15u;
-                if (a.untilted) {
-                    waves = 0;
-                    const float u = (float)(xa - tile_x0) * inv_ps;
-                    const float v = (float)(ya - tile_y0) * inv_ps;
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) {
-                        const float u0 = (float)((w & 1) * 8 * PX), v0 = (float)((w >> 1) * 8 * PY);
-                        const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * PX - 1.f))), 0.f);
-                        const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + (8.f * PY - 1.f))), 0.f);
-                        if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
-                    }
-                }
-                sZ[lid] = aZ[c0 + lid] | (int)(waves << 16);
+    // The atoms of block rows startj..endj (row j: sorted positions [bsp(j, starti),
+    // bsp(j, endi + 1))) are staged as one sequence in row order, 256 at a time, so that rows
+    // holding a few atoms each share a staging round.  Nothing carries from one atom to the
+    // next, and every pixel sums its atoms in the reference's order.
+    const int* brow = bsp + (size_t)k * nb;
+    int j = startj, p = 0, re = 0;
+    if (j <= endj) {
+        p = brow[a.blocks_x * j + starti];
+        re = brow[a.blocks_x * j + endi + 1];
+    }
+    while (j <= endj) {
+        // the next (up to) 256 atoms: lane lid takes the one at position lid of the chunk
+        int mine = -1, cnt = 0;
+        while (cnt < 256 && j <= endj) {
+            const int take = min(256 - cnt, re - p);
+            if (lid >= cnt && lid < cnt + take) mine = p + (lid - cnt);
+            cnt += take;
+            p += take;
+            if (p == re && ++j <= endj) {
+                p = brow[a.blocks_x * j + starti];
+                re = brow[a.blocks_x * j + endi + 1];
             }
-            __syncthreads();
-            for (int l = 0; l < cnt; ++l) {
-                const int info = __builtin_amdgcn_readfirstlane(sZ[l]);
-                if (!((info >> (16 + wave)) & 1)) continue;
-                const int zs = info & 0xffff;
-                // the element's constants (or table), loaded ahead of the distance arithmetic
-                float pt[TAB ? 1 : kPotProjStride];
-                unsigned long long tz = 0;
-                if constexpr (TAB) {
-                    tz = vtab_base(a.vtab + (size_t)(zs - 1) * mse_tab::kBuckets);
-                } else {
-                    const float* row = ptab + (size_t)(zs - 1) * kPotProjStride;
+        }
+        __syncthreads();
+        if (mine >= 0) {
+            const int c = mine;
+            const double xa = ax[c], ya = ay[c];
+            sx[lid] = xa;
+            sy[lid] = ya;
+            // bits 16..19: the waves (pixel quadrants of the tile) that may have a pixel
+            // within 8 A; the distance to each wave's pixel rectangle, in pixels and in
+            // float, is tested against 64.01 A^2, a margin far above the float error.
+            // Untilted only (a tilted metric is no lateral distance): all waves otherwise.
+            unsigned waves = 15u;
+            if (a.untilted) {
+                waves = 0;
+                const float u = (float)(xa - tile_x0) * inv_ps;
+                const float v = (float)(ya - tile_y0) * inv_ps;
 #pragma unroll
-                    for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
+                for (int w = 0; w < 4; ++w) {
+                    const float u0 = (float)((w & 1) * 8 * PX), v0 = (float)((w >> 1) * 8 * PY);
+                    const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * PX - 1.f))), 0.f);
+                    const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + (8.f * PY - 1.f))), 0.f);
+                    if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
                 }
-                const double xl = sx[l], yl = sy[l];
+            }
+            sZ[lid] = aZ[c] | (int)(waves << 16);
+        }
+        __syncthreads();
+        for (int l = 0; l < cnt; ++l) {
+            const int info = __builtin_amdgcn_readfirstlane(sZ[l]);
+            if (!((info >> (16 + wave)) & 1)) continue;
+            const int zs = info & 0xffff;
+            // the element's constants (or table), loaded ahead of the distance arithmetic
+            float pt[TAB ? 1 : kPotProjStride];
+            unsigned long long tz = 0;
+            if constexpr (TAB) {
+                tz = vtab_base(a.vtab + (size_t)(zs - 1) * mse_tab::kBuckets);
+            } else {
+                const float* row = ptab + (size_t)(zs - 1) * kPotProjStride;
 #pragma unroll
-                for (int jy = 0; jy < PY; ++jy) {
-                    const double rad_y = im_pos_y[jy] - yl;
+                for (int q = 0; q < kPotProjStride; ++q) pt[q] = row[q];
+            }
+            const double xl = sx[l], yl = sy[l];
 #pragma unroll
-                    for (int i = 0; i < PX; ++i) {
-                        const double rad_x = im_pos_x[i] - xl;
-                        double r2;
-                        if (a.untilted) {
-                            // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad
-                            // vanish below the rounding of x'^2 + y'^2, which equals this exactly
-                            r2 = rad_x * rad_x + rad_y * rad_y;
+            for (int jy = 0; jy < PY; ++jy) {
+                const double rad_y = im_pos_y[jy] - yl;
+#pragma unroll
+                for (int i = 0; i < PX; ++i) {
+                    const double rad_x = im_pos_x[i] - xl;
+                    double r2;
+                    if (a.untilted) {
+                        // theta = pi/2: z' = O(1e-16) rad and z'/tan(theta) = O(1e-32) rad
+                        // vanish below the rounding of x'^2 + y'^2, which equals this exactly
+                        r2 = rad_x * rad_x + rad_y * rad_y;
+                    } else {
+                        const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
+                        const double zt = z_prime / a.tan_theta;
+                        const double x_prime = rad_x + zt * a.cos_phi;
+                        const double y_prime = rad_y + zt * a.sin_phi;
+                        r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
+                    }
+                    // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the
+                    // reference's test, correctly rounded double sqrt): sqrt(64 + 2^-46) =
+                    // 8 + 2^-50 - 2^-103... rounds to 8 while sqrt of the next double,
+                    // 64 + 2^-45, rounds to 8 + 2^-49, so the test is exactly
+                    // r2 <= 64 + 2^-46.  The potential takes rad in float.
+                    if (r2 <= 0x1.0000000000001p+6) {
+                        if constexpr (TAB) {
+                            // r^2 >= +0, so the floor is the larger bit pattern
+                            const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
+                            sumz[jy * PX + i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
                         } else {
-                            const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
-                            const double zt = z_prime / a.tan_theta;
-                            const double x_prime = rad_x + zt * a.cos_phi;
-                            const double y_prime = rad_y + zt * a.sin_phi;
-                            r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
-                        }
-                        // rad = max(sqrt(r2), r_min) <= 8 with r2 <= 64.0000000001 (the
-                        // reference's test, correctly rounded double sqrt): sqrt(64 + 2^-46) =
-                        // 8 + 2^-50 - 2^-103... rounds to 8 while sqrt of the next double,
-                        // 64 + 2^-45, rounds to 8 + 2^-49, so the test is exactly
-                        // r2 <= 64 + 2^-46.  The potential takes rad in float.
-                        if (r2 <= 0x1.0000000000001p+6) {
-                            if constexpr (TAB) {
-                                // r^2 >= +0, so the floor is the larger bit pattern
-                                const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
-                                sumz[jy * PX + i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
-                            } else {
-                                const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
-                                sumz[jy * PX + i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
-                            }
+                            const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
+                            sumz[jy * PX + i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);
                         }
                     }
                 }
