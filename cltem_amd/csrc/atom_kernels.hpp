@@ -708,251 +708,286 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
 #pragma unroll
     for (int i = 0; i < NX; ++i) sumz[i] = 0.0;
     const int nb = a.blocks_x * a.blocks_y;
-    for (int k = topz; k <= bottomz; ++k) {
-        for (int j = startj; j <= endj; ++j) {
-            const int start = bsp[k * nb + a.blocks_x * j + starti];
-            const int end = bsp[k * nb + a.blocks_x * j + endi + 1];
-            double p2[NX];
+    // The atoms of the rows (k, j), k = topz..bottomz, j = startj..endj in that order (row
+    // (k, j): sorted positions [bsp(k, j, starti), bsp(k, j, endi + 1))) are staged as one
+    // sequence, 256 at a time, so that rows holding a few atoms each share a staging round.
+    // Bit 24 of sh0 marks the first atom of a row, where p2 starts again from 0.
+    constexpr int kRowStart = 1 << 24;
+    double p2[NX];
 #pragma unroll
-            for (int i = 0; i < NX; ++i) p2[i] = 0.0;
-            for (int c0 = start; c0 < end; c0 += 256) {
-                const int cnt = min(256, end - c0);
-                __syncthreads();
-                if (lid < cnt) {
-                    sx[lid] = ax[c0 + lid];
-                    sy[lid] = ay[c0 + lid];
-                    const double zl = az[c0 + lid];
-                    sz[lid] = zl;
-                    // with the tables in LDS, the element's slot there
-                    sZ[lid] = LDS ? a.vslot[aZ[c0 + lid]] : aZ[c0 + lid];
-                    if (no_shift) {
-                        // rad_z(h) = (current_z - h dz / integrals) - z falls with h (each
-                        // rounding step is monotone), so the points with rad_z <= 3 are
-                        // h0..integrals; h0 is found with the very expression of the per-point
-                        // test, once per atom for the whole tile, with the depth rad_z(h0)
-                        auto within = [&](int h) { return (current_z - h * a.dz * int_r) - zl <= 3.0; };
-                        const double h_est = ceil((current_z - zl - 3.0) / (a.dz * int_r));
-                        int h0 = (int)fmin(fmax(h_est, 0.0), (double)(a.integrals + 1));
+    for (int i = 0; i < NX; ++i) p2[i] = 0.0;
+    int k = topz, j = startj, p = 0, re = 0;
+    bool more = topz <= bottomz && startj <= endj, fresh = true;
+    if (more) {
+        p = bsp[k * nb + a.blocks_x * j + starti];
+        re = bsp[k * nb + a.blocks_x * j + endi + 1];
+    }
+    while (more) {
+        // the next (up to) 256 atoms: lane lid takes the one at position lid of the chunk
+        int mine = -1, cnt = 0;
+        bool mine_first = false;
+        while (cnt < 256 && more) {
+            const int take = min(256 - cnt, re - p);
+            if (lid >= cnt && lid < cnt + take) {
+                mine = p + (lid - cnt);
+                mine_first = fresh && lid == cnt;
+            }
+            if (take > 0) fresh = false;
+            cnt += take;
+            p += take;
+            if (p == re) {
+                if (++j > endj) {
+                    j = startj;
+                    more = ++k <= bottomz;
+                }
+                if (more) {
+                    p = bsp[k * nb + a.blocks_x * j + starti];
+                    re = bsp[k * nb + a.blocks_x * j + endi + 1];
+                    fresh = true;
+                }
+            }
+        }
+        __syncthreads();
+        if (mine >= 0) {
+            const int c = mine;
+            sx[lid] = ax[c];
+            sy[lid] = ay[c];
+            const double zl = az[c];
+            sz[lid] = zl;
+            // with the tables in LDS, the element's slot there
+            sZ[lid] = LDS ? a.vslot[aZ[c]] : aZ[c];
+            int info_w = mine_first ? kRowStart : 0;
+            if (no_shift) {
+                // rad_z(h) = (current_z - h dz / integrals) - z falls with h (each
+                // rounding step is monotone), so the points with rad_z <= 3 are
+                // h0..integrals; h0 is found with the very expression of the per-point
+                // test, once per atom for the whole tile, with the depth rad_z(h0)
+                auto within = [&](int h) { return (current_z - h * a.dz * int_r) - zl <= 3.0; };
+                const double h_est = ceil((current_z - zl - 3.0) / (a.dz * int_r));
+                int h0 = (int)fmin(fmax(h_est, 0.0), (double)(a.integrals + 1));
 #pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
-                        while (h0 > 0 && within(h0 - 1)) --h0;
+                while (h0 > 0 && within(h0 - 1)) --h0;
 #pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
-                        while (h0 <= a.integrals && !within(h0)) ++h0;
-                        // bit 16 + w: wave w (its (8 NX)x8 pixel quadrant of the tile) may have
-                        // a pixel within 8 A.  The atom's distance to the wave's pixel
-                        // rectangle, in pixels and in float, is tested against 64.01 A^2: the
-                        // float error (~1e-6 A) is far below that margin, so a clear bit means
-                        // every lane of the wave fails its own exact test xyrad2 <= 64.
-                        unsigned waves = 0;
-                        if (h0 <= a.integrals) {
-                            const float u = (float)(sx[lid] - tile_x0) * inv_ps;
-                            const float v = (float)(sy[lid] - tile_y0) * inv_ps;
+                while (h0 <= a.integrals && !within(h0)) ++h0;
+                // bit 16 + w: wave w (its (8 NX)x8 pixel quadrant of the tile) may have
+                // a pixel within 8 A.  The atom's distance to the wave's pixel
+                // rectangle, in pixels and in float, is tested against 64.01 A^2: the
+                // float error (~1e-6 A) is far below that margin, so a clear bit means
+                // every lane of the wave fails its own exact test xyrad2 <= 64.
+                unsigned waves = 0;
+                if (h0 <= a.integrals) {
+                    const float u = (float)(sx[lid] - tile_x0) * inv_ps;
+                    const float v = (float)(sy[lid] - tile_y0) * inv_ps;
 #pragma unroll
-                            for (int w = 0; w < 4; ++w) {
-                                const float u0 = (float)((w & 1) * 8 * NX), v0 = (float)((w >> 1) * 8);
-                                const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * NX - 1.f))), 0.f);
-                                const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + 7.f)), 0.f);
-                                if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
-                            }
-                        }
-                        sh0[lid] = h0 | (int)(waves << 16);
-                        srz[lid] = (float)((current_z - h0 * a.dz * int_r) - zl);
+                    for (int w = 0; w < 4; ++w) {
+                        const float u0 = (float)((w & 1) * 8 * NX), v0 = (float)((w >> 1) * 8);
+                        const float du = fmaxf(fmaxf(u0 - u, u - (u0 + (8.f * NX - 1.f))), 0.f);
+                        const float dv = fmaxf(fmaxf(v0 - v, v - (v0 + 7.f)), 0.f);
+                        if (du * du + dv * dv <= reach_px2) waves |= 1u << w;
                     }
                 }
-                __syncthreads();
-                for (int l = 0; l < cnt; ++l) {
-                    // the atom's first point (the same in every lane); atoms out of this
-                    // wave's reach, in depth or laterally, are passed over by the whole wave
-                    int h0 = 0;
-                    if (no_shift) {
-                        const int info = __builtin_amdgcn_readfirstlane(sh0[l]);
-                        if (!((info >> (16 + wave)) & 1)) continue;
-                        h0 = info & 0xffff;
-                    }
-                    const double xl = sx[l];
-                    const double rad_y0 = im_pos_y - sy[l];
-                    // the element's constants (scalar registers) or table, loaded once per atom
-                    float pz[TAB ? 1 : kPot3dStride];
-                    unsigned long long tz = 0;
-                    int loff = 0;
-                    if constexpr (LDS) {
-                        loff = __builtin_amdgcn_readfirstlane(sZ[l]) * a.vb_n - a.vb_lo;
-                    } else if constexpr (TAB) {
-                        tz = vtab_base(a.vtab + (size_t)(__builtin_amdgcn_readfirstlane(sZ[l]) - 1) * mse_tab::kBuckets);
-                    } else {
-                        const float* row = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+                info_w |= h0 | (int)(waves << 16);
+                srz[lid] = (float)((current_z - h0 * a.dz * int_r) - zl);
+            }
+            sh0[lid] = info_w;
+        }
+        __syncthreads();
+        for (int l = 0; l < cnt; ++l) {
+            // the atom's first point (the same in every lane); atoms out of this
+            // wave's reach, in depth or laterally, are passed over by the whole wave
+            int h0 = 0;
+            const int info = __builtin_amdgcn_readfirstlane(sh0[l]);
+            if (info & kRowStart) {
 #pragma unroll
-                        for (int q = 0; q < kPot3dStride; ++q) pz[q] = row[q];
+                for (int i = 0; i < NX; ++i) p2[i] = 0.0;
+            }
+            if (no_shift) {
+                if (!((info >> (16 + wave)) & 1)) continue;
+                h0 = info & 0xffff;
+            }
+            const double xl = sx[l];
+            const double rad_y0 = im_pos_y - sy[l];
+            // the element's constants (scalar registers) or table, loaded once per atom
+            float pz[TAB ? 1 : kPot3dStride];
+            unsigned long long tz = 0;
+            int loff = 0;
+            if constexpr (LDS) {
+                loff = __builtin_amdgcn_readfirstlane(sZ[l]) * a.vb_n - a.vb_lo;
+            } else if constexpr (TAB) {
+                tz = vtab_base(a.vtab + (size_t)(__builtin_amdgcn_readfirstlane(sZ[l]) - 1) * mse_tab::kBuckets);
+            } else {
+                const float* row = tab3d + (size_t)(sZ[l] - 1) * kPot3dStride;
+#pragma unroll
+                for (int q = 0; q < kPot3dStride; ++q) pz[q] = row[q];
+            }
+            if (no_shift) {
+                // the xy distance of each pixel is the same at every integration point;
+                // a pixel beyond 8 A takes no part (no sum, no carried p2)
+                bool in[NX];
+                float xy2f[NX];
+                bool any = false, low = false;
+#pragma unroll
+                for (int i = 0; i < NX; ++i) {
+                    const double rad_x = im_pos_x[i] - xl;
+                    const double xyrad2 = rad_x * rad_x + rad_y0 * rad_y0;
+                    in[i] = xyrad2 <= 64.0;
+                    // a pixel that takes no part is evaluated at the smallest table
+                    // distance, so its discarded lookups stay within the tables
+                    xy2f[i] = in[i] ? (float)xyrad2 : r_lo2;
+                    any = any || in[i];
+                    low = low || xy2f[i] < r_lo2;
+                }
+                if (!any) continue;
+                // from there the depth steps in float and the atom's trapezoid terms are
+                // summed in float, then added to the pixel's double sum once
+                const float stepf = (float)(a.dz * int_r);
+                const float rz0 = srz[l];
+                if constexpr (TAB) {
+                    // points h0..integrals at depths rz0 - (h - h0) step; the trapezoid
+                    //   sum_{h > 0} (p(h) + p(h - 1)) = 2 sum_h p(h) - p(integrals) + c,
+                    // c = p2 (carried from the row's previous atom) when h0 > 0 and
+                    // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
+                    // with s the sum over h0 + 1..integrals.  The pixels share the depths;
+                    // one beyond 8 A is evaluated (within the table) and discarded.
+                    //
+                    // With the tables in LDS the clamp to the staged range is left out
+                    // when no lane of the wave needs it: every r^2 = fma(rz, rz, xy2) is
+                    // at least xy2 >= r_lo2, and at most 64 + zmax^2, zmax bounding |rz|
+                    // over the atom's points (|rz0| + its depth span; the 1e-3 margin
+                    // covers the float steps' rounding, < 4096 * 2^-24 relative).
+                    bool clamp = true;
+                    if constexpr (LDS) {
+                        const float zmax = (fabsf(rz0) + (float)(a.integrals - h0) * stepf) * 1.001f;
+                        clamp = __builtin_amdgcn_ballot_w64(low || !(fmaf(zmax, zmax, 64.f) <= r_hi2)) != 0;
                     }
-                    if (no_shift) {
-                        // the xy distance of each pixel is the same at every integration point;
-                        // a pixel beyond 8 A takes no part (no sum, no carried p2)
-                        bool in[NX];
-                        float xy2f[NX];
-                        bool any = false, low = false;
+                    auto sweep = [&](auto clamped) {
+                        auto eval = [&](float r2) {
+                            if constexpr (LDS)
+                                return vtab_eval_lds(lt, loff, decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2);
+                            else
+                                return vtab_eval(tz, fmaxf(r2, r_lo2));
+                        };
+                        float rz = rz0;
+                        float pf[NX], s[NX], pl[NX];
 #pragma unroll
                         for (int i = 0; i < NX; ++i) {
-                            const double rad_x = im_pos_x[i] - xl;
-                            const double xyrad2 = rad_x * rad_x + rad_y0 * rad_y0;
-                            in[i] = xyrad2 <= 64.0;
-                            // a pixel that takes no part is evaluated at the smallest table
-                            // distance, so its discarded lookups stay within the tables
-                            xy2f[i] = in[i] ? (float)xyrad2 : r_lo2;
-                            any = any || in[i];
-                            low = low || xy2f[i] < r_lo2;
+                            pf[i] = eval(fmaf(rz, rz, xy2f[i]));
+                            s[i] = 0.f;
+                            pl[i] = pf[i];
                         }
-                        if (!any) continue;
-                        // from there the depth steps in float and the atom's trapezoid terms are
-                        // summed in float, then added to the pixel's double sum once
-                        const float stepf = (float)(a.dz * int_r);
-                        const float rz0 = srz[l];
-                        if constexpr (TAB) {
-                            // points h0..integrals at depths rz0 - (h - h0) step; the trapezoid
-                            //   sum_{h > 0} (p(h) + p(h - 1)) = 2 sum_h p(h) - p(integrals) + c,
-                            // c = p2 (carried from the row's previous atom) when h0 > 0 and
-                            // -p(0) when h0 = 0, summed as 2 s - p(last) + (2 p(h0) + c)
-                            // with s the sum over h0 + 1..integrals.  The pixels share the depths;
-                            // one beyond 8 A is evaluated (within the table) and discarded.
-                            //
-                            // With the tables in LDS the clamp to the staged range is left out
-                            // when no lane of the wave needs it: every r^2 = fma(rz, rz, xy2) is
-                            // at least xy2 >= r_lo2, and at most 64 + zmax^2, zmax bounding |rz|
-                            // over the atom's points (|rz0| + its depth span; the 1e-3 margin
-                            // covers the float steps' rounding, < 4096 * 2^-24 relative).
-                            bool clamp = true;
+                        // four points per trip (the lookups' asm keeps the compiler from
+                        // unrolling), so that their table reads are in flight together;
+                        // added in order
+                        int h = h0;
+                        for (; h + 3 < a.integrals; h += 4) {
+                            float rzk[4];
+                            rzk[0] = rz - stepf;
+                            rzk[1] = rzk[0] - stepf;
+                            rzk[2] = rzk[1] - stepf;
+                            rzk[3] = rzk[2] - stepf;
+                            rz = rzk[3];
                             if constexpr (LDS) {
-                                const float zmax = (fabsf(rz0) + (float)(a.integrals - h0) * stepf) * 1.001f;
-                                clamp = __builtin_amdgcn_ballot_w64(low || !(fmaf(zmax, zmax, 64.f) <= r_hi2)) != 0;
-                            }
-                            auto sweep = [&](auto clamped) {
-                                auto eval = [&](float r2) {
-                                    if constexpr (LDS)
-                                        return vtab_eval_lds(lt, loff, decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2);
-                                    else
-                                        return vtab_eval(tz, fmaxf(r2, r_lo2));
-                                };
-                                float rz = rz0;
-                                float pf[NX], s[NX], pl[NX];
+                                // the same lookups as eval, in phases: every bucket index,
+                                // then every table read, then the cubics
+                                float r2k[4][NX];
+                                unsigned qk[4][NX];
+                                float4 ck[4][NX];
 #pragma unroll
-                                for (int i = 0; i < NX; ++i) {
-                                    pf[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                    s[i] = 0.f;
-                                    pl[i] = pf[i];
-                                }
-                                // four points per trip (the lookups' asm keeps the compiler from
-                                // unrolling), so that their table reads are in flight together;
-                                // added in order
-                                int h = h0;
-                                for (; h + 3 < a.integrals; h += 4) {
-                                    float rzk[4];
-                                    rzk[0] = rz - stepf;
-                                    rzk[1] = rzk[0] - stepf;
-                                    rzk[2] = rzk[1] - stepf;
-                                    rzk[3] = rzk[2] - stepf;
-                                    rz = rzk[3];
-                                    if constexpr (LDS) {
-                                        // the same lookups as eval, in phases: every bucket index,
-                                        // then every table read, then the cubics
-                                        float r2k[4][NX];
-                                        unsigned qk[4][NX];
-                                        float4 ck[4][NX];
-#pragma unroll
-                                        for (int k = 0; k < 4; ++k)
-#pragma unroll
-                                            for (int i = 0; i < NX; ++i) {
-                                                const float r2 = fmaf(rzk[k], rzk[k], xy2f[i]);
-                                                r2k[k][i] = decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2;
-                                                asm("v_lshrrev_b32 %0, %1, %2" : "=v"(qk[k][i]) : "i"(mse_tab::kMantBits), "v"(__float_as_uint(r2k[k][i])));
-                                            }
-#pragma unroll
-                                        for (int k = 0; k < 4; ++k)
-#pragma unroll
-                                            for (int i = 0; i < NX; ++i) ck[k][i] = lt[(int)qk[k][i] + loff];
-#pragma unroll
-                                        for (int i = 0; i < NX; ++i)
-#pragma unroll
-                                            for (int k = 0; k < 4; ++k) {
-                                                const float4 c = ck[k][i];
-                                                const float m = (float)(__float_as_uint(r2k[k][i]) & ((1u << mse_tab::kMantBits) - 1u));
-                                                const float p = fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
-                                                s[i] += p;
-                                                if (k == 3) pl[i] = p;
-                                            }
-                                    } else {
-#pragma unroll
-                                        for (int i = 0; i < NX; ++i) {
-                                            const float pa = eval(fmaf(rzk[0], rzk[0], xy2f[i]));
-                                            const float pb = eval(fmaf(rzk[1], rzk[1], xy2f[i]));
-                                            const float pc = eval(fmaf(rzk[2], rzk[2], xy2f[i]));
-                                            pl[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                            s[i] += pa;
-                                            s[i] += pb;
-                                            s[i] += pc;
-                                            s[i] += pl[i];
-                                        }
-                                    }
-                                }
-#pragma clang loop unroll(disable)
-                                for (; h < a.integrals; ++h) {
-                                    rz -= stepf;
+                                for (int k = 0; k < 4; ++k)
 #pragma unroll
                                     for (int i = 0; i < NX; ++i) {
-                                        pl[i] = eval(fmaf(rz, rz, xy2f[i]));
-                                        s[i] += pl[i];
+                                        const float r2 = fmaf(rzk[k], rzk[k], xy2f[i]);
+                                        r2k[k][i] = decltype(clamped)::value ? __builtin_amdgcn_fmed3f(r2, r_lo2, r_hi2) : r2;
+                                        asm("v_lshrrev_b32 %0, %1, %2" : "=v"(qk[k][i]) : "i"(mse_tab::kMantBits), "v"(__float_as_uint(r2k[k][i])));
                                     }
-                                }
+#pragma unroll
+                                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                                    for (int i = 0; i < NX; ++i) ck[k][i] = lt[(int)qk[k][i] + loff];
+#pragma unroll
+                                for (int i = 0; i < NX; ++i)
+#pragma unroll
+                                    for (int k = 0; k < 4; ++k) {
+                                        const float4 c = ck[k][i];
+                                        const float m = (float)(__float_as_uint(r2k[k][i]) & ((1u << mse_tab::kMantBits) - 1u));
+                                        const float p = fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
+                                        s[i] += p;
+                                        if (k == 3) pl[i] = p;
+                                    }
+                            } else {
 #pragma unroll
                                 for (int i = 0; i < NX; ++i) {
-                                    if (!in[i]) continue;
-                                    const float acc = (2.f * s[i] - pl[i]) + (h0 > 0 ? 2.f * pf[i] + (float)p2[i] : pf[i]);
-                                    sumz[i] += 0.5 * (double)acc;
-                                    p2[i] = (double)pl[i];
+                                    const float pa = eval(fmaf(rzk[0], rzk[0], xy2f[i]));
+                                    const float pb = eval(fmaf(rzk[1], rzk[1], xy2f[i]));
+                                    const float pc = eval(fmaf(rzk[2], rzk[2], xy2f[i]));
+                                    pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                    s[i] += pa;
+                                    s[i] += pb;
+                                    s[i] += pc;
+                                    s[i] += pl[i];
                                 }
-                            };
-                            if (!LDS || clamp)
-                                sweep(std::true_type{});
-                            else
-                                sweep(std::false_type{});
-                            continue;
+                            }
+                        }
+#pragma clang loop unroll(disable)
+                        for (; h < a.integrals; ++h) {
+                            rz -= stepf;
+#pragma unroll
+                            for (int i = 0; i < NX; ++i) {
+                                pl[i] = eval(fmaf(rz, rz, xy2f[i]));
+                                s[i] += pl[i];
+                            }
                         }
 #pragma unroll
                         for (int i = 0; i < NX; ++i) {
                             if (!in[i]) continue;
-                            float p2f = (float)p2[i], acc = 0.f;
-                            for (int h = h0; h <= a.integrals; ++h) {
-                                const float rz = fmaf(-(float)(h - h0), stepf, rz0);
-                                const float r2 = fmaf(rz, rz, xy2f[i]);
-                                const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
-                                if (h != 0) acc += p1 + p2f;
-                                p2f = p1;
-                            }
+                            const float acc = (2.f * s[i] - pl[i]) + (h0 > 0 ? 2.f * pf[i] + (float)p2[i] : pf[i]);
                             sumz[i] += 0.5 * (double)acc;
-                            p2[i] = (double)p2f;
+                            p2[i] = (double)pl[i];
                         }
-                        continue;
-                    }
+                    };
+                    if (!LDS || clamp)
+                        sweep(std::true_type{});
+                    else
+                        sweep(std::false_type{});
+                    continue;
+                }
 #pragma unroll
-                    for (int i = 0; i < NX; ++i) {
-                        double rad_x = im_pos_x[i] - xl;
-                        double rad_y = rad_y0;
-                        for (int h = 0; h <= a.integrals; ++h) {
-                            rad_x -= a.shift_x;
-                            rad_y -= a.shift_y;
-                            const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
-                            const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
-                            if (xyrad2 <= 64.0 && rad_z <= 3.0) {
-                                const double r2 = xyrad2 + rad_z * rad_z;
-                                float p1f;
-                                if constexpr (LDS)  // not launched with a tilt; kept within the copy
-                                    p1f = vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2 < r_min2 ? r_min2f : (float)r2, r_lo2, r_hi2));
-                                else if constexpr (TAB)
-                                    p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
-                                else
-                                    p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
-                                const double p1 = (double)p1f;
-                                sumz[i] += (h != 0) * (p1 + p2[i]) * 0.5;
-                                p2[i] = p1;
-                            }
-                        }
+                for (int i = 0; i < NX; ++i) {
+                    if (!in[i]) continue;
+                    float p2f = (float)p2[i], acc = 0.f;
+                    for (int h = h0; h <= a.integrals; ++h) {
+                        const float rz = fmaf(-(float)(h - h0), stepf, rz0);
+                        const float r2 = fmaf(rz, rz, xy2f[i]);
+                        const float p1 = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2f ? r_min2f : r2);
+                        if (h != 0) acc += p1 + p2f;
+                        p2f = p1;
+                    }
+                    sumz[i] += 0.5 * (double)acc;
+                    p2[i] = (double)p2f;
+                }
+                continue;
+            }
+#pragma unroll
+            for (int i = 0; i < NX; ++i) {
+                double rad_x = im_pos_x[i] - xl;
+                double rad_y = rad_y0;
+                for (int h = 0; h <= a.integrals; ++h) {
+                    rad_x -= a.shift_x;
+                    rad_y -= a.shift_y;
+                    const double xyrad2 = rad_x * rad_x + rad_y * rad_y;
+                    const double rad_z = (current_z - h * a.dz * int_r) - sz[l];
+                    if (xyrad2 <= 64.0 && rad_z <= 3.0) {
+                        const double r2 = xyrad2 + rad_z * rad_z;
+                        float p1f;
+                        if constexpr (LDS)  // not launched with a tilt; kept within the copy
+                            p1f = vtab_eval_lds(lt, loff, __builtin_amdgcn_fmed3f(r2 < r_min2 ? r_min2f : (float)r2, r_lo2, r_hi2));
+                        else if constexpr (TAB)
+                            p1f = vtab_eval(tz, fmaxf(r2 < r_min2 ? r_min2f : (float)r2, kVtabFloor));
+                        else
+                            p1f = v3d_tab<FORM>(pz, a.i_count, r2 < r_min2 ? r_min2f : (float)r2);
+                        const double p1 = (double)p1f;
+                        sumz[i] += (h != 0) * (p1 + p2[i]) * 0.5;
+                        p2[i] = p1;
                     }
                 }
             }
