@@ -217,9 +217,9 @@ __device__ __forceinline__ void add_grad(double* __restrict__ grad, const int* _
     unsafeAtomicAdd(o + 2, gz);
 }
 
-// Position gradients of the full-3D phase of slices first + blockIdx.z: the same 16x16 pixel tiles, block
-// rows, atoms and integration points as full3d_sumz with one pixel per thread (k_phase_full3d),
-// in the same order.  With p1_e the
+// Position gradients of the full-3D phase of slices first + blockIdx.z: per pixel the same
+// (slice, block row) sequence, atoms and integration points as full3d_sumz (k_phase_full3d), in
+// the same order, over 16x16 pixel tiles of its own.  With p1_e the
 // potential at evaluation e of a (k, j) row sequence, the trapezoid gives
 //   sumz = sum_e 0.5 [h_e != 0] (p1_e + p1_prev(e)),
 // so p1_e carries 0.5 [h_e != 0] from its own step and 0.5 from the next evaluation when

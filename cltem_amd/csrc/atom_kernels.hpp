@@ -997,8 +997,8 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
 
 // LDS = true (untilted beams only): the elements' tables are first copied into dynamic LDS
 // (v_nel * vb_n * 16 bytes), where the integration points read them, and each thread sums
-// kFull3dNX pixels 16 columns apart: grid (res / (16 kFull3dNX), res / 16, planes) then,
-// (res / 16, res / 16, planes) otherwise.
+// kFull3dNX pixels (tile_pixel_x<kFull3dNX, 1>): grid (res / (16 kFull3dNX), res / 16, planes)
+// then, (res / 16, res / 16, planes) otherwise.
 constexpr int kFull3dNX = 2;
 template <int FORM, bool TAB = false, bool LDS = false>
 __global__ __launch_bounds__(256) void k_potential_full3d(
