@@ -358,3 +358,40 @@ def test_full3d_tables_beyond_the_lds_budget(engine, oracle):
     e_t = max(rel_l2(a, b) for a, b in zip(tab, trans))
     print(f"7 elements, full 3D: tabulated vs direct {e_td:.2e}, vs FP64 {e_t:.2e}")
     assert e_td <= 2e-6 and e_t <= 1e-5
+
+
+@pytest.mark.parametrize("full3d", [0, 1])
+def test_dense_rows_span_staging_chunks(engine, oracle, full3d):
+    """The builders stage the atoms of a pixel tile's block rows (and, in full 3D, of every
+    slice of its depth window) as one sequence, 256 at a time.  A dense slab puts several
+    hundred atoms in each tile's sequence, so chunk boundaries fall inside block rows: the
+    full-3D trapezoid must carry p2 across them and restart it only at a row's first atom.
+    Transmission functions against the FP64 oracle."""
+    if full3d:
+        spec = sp.random_box({"C": 420}, (12.0, 12.0, 2.5), 0.6, seed=21)
+        extra = dict(slice_thickness=1.0, full3d=1, full3d_integrals=6)
+    else:
+        spec = sp.random_box({"C": 700}, (12.0, 12.0, 1.5), 0.5, seed=22)
+        extra = dict(slice_thickness=2.0)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=300.0, struct_min=tuple(lo), struct_max=tuple(hi),
+                area=(lo[0], hi[0], lo[1], hi[1]), **extra)
+    g, og, p, o = prepare(engine, oracle, scen, spec, "kirkland")
+    # atoms per slice (and per full-3D window of slices s - load_z .. s + load_z)
+    nb = g.blocks_x * g.blocks_y
+    bs = np.asarray(o["block_start"])
+    per_slice = np.array([bs[(s + 1) * nb] - bs[s * nb] for s in range(g.n_slices)])
+    if full3d:
+        lz = g.load_z
+        per_seq = [per_slice[max(0, s - lz): s + lz + 1].sum() for s in range(g.n_slices)]
+    else:
+        per_seq = list(per_slice)
+    print(f"full3d={full3d}: {g.n_slices} slices, atoms per slice {per_slice.tolist()}, "
+          f"largest staging sequence {max(per_seq)}")
+    assert max(per_seq) > 300
+    trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
+    errs = [rel_l2(engine.read_transmission(0, s), trans[s]) for s in range(g.n_slices)]
+    print(f"transmission rel-L2 max {max(errs):.2e}")
+    # the phases stay below 1 rad, so float rounding keeps the builders near 1e-7; one trapezoid
+    # term lost or doubled per row would cost about 1e-4 of a phase near the atom
+    assert max(errs) <= 1e-6
