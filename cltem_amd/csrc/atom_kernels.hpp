@@ -361,6 +361,17 @@ __device__ __forceinline__ float vtab_eval(unsigned long long base, float r2) {
     return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
 }
 
+// V(r2) as vtab_eval computes it, for kVtabFloor <= r2 < 2^kOctaveHi, where the offset needs
+// no clamp.
+__device__ __forceinline__ float vtab_eval_in(unsigned long long base, float r2) {
+    const unsigned bits = __float_as_uint(r2);
+    const unsigned off = (bits >> (mse_tab::kMantBits - 4)) & ~15u;
+    using gfloat4 = const __attribute__((address_space(1))) float4;
+    const float4 c = *(const float4*)reinterpret_cast<gfloat4*>(base + off);
+    const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
+    return fmaf(fmaf(fmaf(c.w, m, c.z), m, c.y), m, c.x);
+}
+
 // V(r2) as vtab_eval computes it, from the LDS copy of an element's buckets
 // [vb_lo, vb_lo + vb_n), which starts at lt[off + vb_lo]; r2 must lie in those buckets
 // (the caller clamps it to them).
@@ -587,9 +598,11 @@ __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, co
                     // r2 <= 64 + 2^-46.  The potential takes rad in float.
                     if (r2 <= 0x1.0000000000001p+6) {
                         if constexpr (TAB) {
-                            // r^2 >= +0, so the floor is the larger bit pattern
+                            // r_min^2 is above the tables' floor (the host enables them only
+                            // then) and r2 <= 64 + 2^-46 is 64 or less in float, far inside
+                            // their top: the lookup needs no clamp
                             const float f = r2 < r_min2 ? r_minf * r_minf : (float)r2;
-                            sumz[jy * PX + i] += (double)vtab_eval(tz, __uint_as_float(max(__float_as_uint(f), __float_as_uint(kVtabFloor))));
+                            sumz[jy * PX + i] += (double)vtab_eval_in(tz, f);
                         } else {
                             const float rf = r2 < r_min2 ? r_minf : __builtin_amdgcn_sqrtf((float)r2);
                             sumz[jy * PX + i] += (double)vproj_tab<FORM>(pt, a.i_count, rf);

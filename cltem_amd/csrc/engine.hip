@@ -1121,7 +1121,8 @@ const float4* potential_tables(mse_engine* e, const ConfigData& c, const mse_geo
     // the distances the kernels evaluate must lie inside the tables: r >= r_min = px / 4, and
     // r^2 <= 64 + (depth reach)^2 with the full-3D depth below (load_z + 1) slices
     const double r_min = 0.25 * g.real_scale;
-    if (r_min * r_min < std::ldexp(1.0, mse_tab::kOctaveLo)) return nullptr;
+    // (with a margin, so that r_min^2 formed in float is above the floor too)
+    if (r_min * r_min < std::ldexp(1.0, mse_tab::kOctaveLo) * (1.0 + 1e-6)) return nullptr;
     const double reach = (g.load_z + 2) * g.slice_thickness;
     if (g.full3d && 64.0 + reach * reach >= std::ldexp(1.0, mse_tab::kOctaveHi)) return nullptr;
     const int t = g.full3d ? 1 : 0;
