@@ -627,7 +627,7 @@ def main():
     for c in range(args.warmup):
         build_config(engine, 0, displaced[c])
         if tune is None:
-            tune = engine.autotune(0, args.batch, reps=3)
+            tune = engine.autotune(0, args.batch, reps=10)  # the column variants lie within a few percent
             log(f"autotune at B={args.batch}: {tune}")
         engine.stem_batch(0, pos, args.batch, probe, dets, use_graph=bool(args.graph))
         log(f"warmup configuration {c} done")
@@ -659,7 +659,7 @@ def main():
         npr = max(B, args.sweep_probes if B > 1 else min(args.sweep_probes, 64))
         sub = pos[:npr]
         if B > 1 and B != args.batch:  # kernel variants tuned for this batch size too
-            t = engine.autotune(0, B, reps=3)
+            t = engine.autotune(0, B, reps=10)
             sweep_tuning[str(B)] = {"best_row": t["best_row"], "best_col": t["best_col"]}
         engine.stem_batch(0, sub[:B], B, probe, dets)
         engine.synchronize()
