@@ -33,7 +33,6 @@ enum RowMode : int {
 enum ColMode : int {
     COL_PROP = 0,         // slice step, column half
     COL_PROP_DETECT = 1,  // last slice: propagate and reduce detectors, no store
-    COL_BAND = 2,         // transmission band limit (mask <= , scale 1/res^2)
     COL_PROBE = 3,        // generate the k-space probe and inverse y-DFT
     COL_FFT_DETECT = 4,   // detectors from a stored wave (F_y S / res)
     COL_FFT_DIFF = 5,     // |fftshift Psi|^2 of a stored wave into a double image
@@ -120,7 +119,7 @@ struct ColArgs {
     const float* filter;     // COL_FILTER: res*res real factors, row-major [ky][kx] (unshifted)
     const double* kx;        // k-grid (COL_PROBE)
     const double* ky;
-    float scale;             // COL_BAND / COL_FFT_*: multiply after the forward DFT
+    float scale;             // COL_FFT_*: multiply after the forward DFT
     double det_scale;        // intensity factor applied to |V|^2 in detector sums
     const unsigned* det_bits;  // ndet bitmasks over the fft-shifted image, res*res bits each
     int ndet;
@@ -398,24 +397,20 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
     }
 
     if constexpr (MODE == COL_PROP || MODE == COL_PROP_DETECT || MODE == COL_PROP_STORE_DETECT ||
-                  MODE == COL_BAND || MODE == COL_PROP_CONJ) {
+                  MODE == COL_PROP_CONJ) {
         const int xs = (x < N / 2) ? x : N - x;  // |signed kx index|
-        float2 pxv = make_float2(a.scale, 0.f);
+        const float2* pxt = a.px;
         const float2* pyt = a.py;
-        if constexpr (MODE != COL_BAND) {
-            const float2* pxt = a.px;
-            if (a.wave_prop) {
-                pxt = a.wave_prop[2 * wave];
-                pyt = a.wave_prop[2 * wave + 1];
-            }
-            pxv = pxt[x];
+        if (a.wave_prop) {
+            pxt = a.wave_prop[2 * wave];
+            pyt = a.wave_prop[2 * wave + 1];
         }
+        const float2 pxv = pxt[x];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + L * m;
             const bool keep = xs <= a.lim[y];
-            float2 f = pxv;
-            if constexpr (MODE != COL_BAND) f = cmul(pxv, pyt[y]);
+            float2 f = cmul(pxv, pyt[y]);
             if constexpr (MODE == COL_PROP_CONJ) f.y = -f.y;
             v[m] = keep ? cmul(v[m], f) : make_float2(0.f, 0.f);
         }
@@ -470,7 +465,7 @@ __global__ __launch_bounds__(TC * PL::L) __attribute__((amdgpu_waves_per_eu(WPE)
         }
     }
 
-    if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT || MODE == COL_BAND ||
+    if constexpr (MODE == COL_PROP || MODE == COL_PROP_STORE_DETECT ||
                   MODE == COL_PROBE || MODE == COL_PROP_CONJ || MODE == COL_CTEM_IMAGE || MODE == COL_FILTER) {
         if constexpr (MODE != COL_PROBE) sync();  // the forward pass's last reads precede new writes
         fft_line<PL, true>(v, ln, t, tws, sync);

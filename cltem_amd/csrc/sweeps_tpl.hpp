@@ -150,7 +150,6 @@ int col_pass_n(mse_engine* e, int mode, const float2* in, float2* out, int nsets
     case mse::M: mse::k_col<PL, TC, mse::M, SY, TR, WPE><<<grid, NT, 0, e->stream>>>(in, out, wave_stride, e->tw.p, args); break;
                     MSE_COL_CASE(COL_PROP)
                     MSE_COL_CASE(COL_PROP_DETECT)
-                    MSE_COL_CASE(COL_BAND)
                     MSE_COL_CASE(COL_PROBE)
                     MSE_COL_CASE(COL_FFT_DETECT)
                     MSE_COL_CASE(COL_FFT_DIFF)
