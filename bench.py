@@ -182,8 +182,8 @@ def cpu_baseline(w, g, spec0, sample, check_pos, gpu_det):
         "host_cpus_visible": visible,
         "kind": "port",
         "sample": f"{len(sample)} probes x {og.n_slices} slices of phonon configuration 0 (FP32 C++ "
-                  f"restatement, std::thread over probes on {cores} threads); potential build of "
-                  f"{og.n_slices} slices took {t_pot:.1f} s in FP64",
+                  f"restatement with an iterative radix-4 Stockham FFT, std::thread over probes on "
+                  f"{cores} threads); potential build of {og.n_slices} slices took {t_pot:.1f} s in FP64",
     }, err, len(check_pos)
 
 

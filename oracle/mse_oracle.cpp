@@ -491,6 +491,125 @@ struct Fft2 {
     }
 };
 
+// The CPU baseline's FFT for power-of-two sizes: an iterative Stockham transform (radix-4
+// stages and one radix-2 stage when log2 n is odd), twiddles per stage computed once, no
+// allocation per call; columns go through it eight at a time from a contiguous copy.  The
+// same sign convention and 1/sqrt(W*H) scale as Fft2; checked against Fft2 (and so against
+// numpy) by orc_fft2_fast.  The checker itself keeps Fft2.
+template <typename T>
+struct StockhamFft1 {
+    typedef std::complex<T> C;
+    int n = 0;
+    std::vector<int> radix;            // stage radices, 4s first, then one 2 if needed
+    std::vector<std::vector<C>> tw;    // stage of length L: w^(r p), w = exp(-2 pi i / L),
+                                       // p < L / R, r = 1..R-1 (at [(r - 1) (L / R) + p])
+
+    explicit StockhamFft1(int n_) : n(n_) {
+        int m = n;
+        while (m % 4 == 0) { radix.push_back(4); m /= 4; }
+        if (m == 2) radix.push_back(2);
+        int len = n;
+        for (int R : radix) {
+            const int q = len / R;
+            std::vector<C> w((size_t)(R - 1) * q);
+            for (int r = 1; r < R; ++r)
+                for (int p = 0; p < q; ++p) {
+                    long double a = -2.0L * (long double)M_PI * (long double)(r * p) / (long double)len;
+                    w[(size_t)(r - 1) * q + p] = C((T)std::cos((double)a), (T)std::sin((double)a));
+                }
+            tw.push_back(std::move(w));
+            len = q;
+        }
+    }
+    static bool fits(int n) {
+        return n >= 2 && (n & (n - 1)) == 0;
+    }
+    // the plain complex product (std::complex's operator* also handles infinities and NaNs
+    // out of line)
+    static C mul(const C& u, const C& v) {
+        return C(u.real() * v.real() - u.imag() * v.imag(), u.real() * v.imag() + u.imag() * v.real());
+    }
+    // Stockham decimation in frequency: the stage of length L and stride s (L s = n) takes
+    // a[t + s (p + r L/R)], r < R, through the R-point DFT, multiplies output r by w^(r p)
+    // and writes it to b[t + s (R p + r)].  x (n points) is transformed in place, y is scratch.
+    void run(C* x, C* y, bool inv) const {
+        int len = n, s = 1;
+        C* a = x;
+        C* b = y;
+        for (size_t st = 0; st < radix.size(); ++st) {
+            const int R = radix[st];
+            const int q = len / R;
+            const C* w = tw[st].data();
+            for (int p = 0; p < q; ++p) {
+                C w1 = w[p], w2(1, 0), w3(1, 0);
+                if (R == 4) {
+                    w2 = w[(size_t)q + p];
+                    w3 = w[(size_t)2 * q + p];
+                }
+                if (inv) {
+                    w1 = std::conj(w1);
+                    w2 = std::conj(w2);
+                    w3 = std::conj(w3);
+                }
+                const C* in = a + (size_t)s * p;
+                C* out = b + (size_t)s * R * p;
+                const size_t sq = (size_t)s * q;
+                if (R == 2) {
+                    for (int t = 0; t < s; ++t) {
+                        const C c0 = in[t], c1 = in[t + sq];
+                        out[t] = c0 + c1;
+                        out[t + s] = mul(c0 - c1, w1);
+                    }
+                } else {
+                    for (int t = 0; t < s; ++t) {
+                        const C c0 = in[t], c1 = in[t + sq], c2 = in[t + 2 * sq], c3 = in[t + 3 * sq];
+                        const C apc = c0 + c2, amc = c0 - c2, bpd = c1 + c3, bmd = c1 - c3;
+                        // (b - d) times -i (forward) or +i (inverse)
+                        const C jb = inv ? C(-bmd.imag(), bmd.real()) : C(bmd.imag(), -bmd.real());
+                        out[t] = apc + bpd;
+                        out[t + s] = mul(amc + jb, w1);
+                        out[t + 2 * s] = mul(apc - bpd, w2);
+                        out[t + 3 * s] = mul(amc - jb, w3);
+                    }
+                }
+            }
+            std::swap(a, b);
+            len = q;
+            s *= R;
+        }
+        if (a != x) std::memcpy((void*)x, (const void*)a, (size_t)n * sizeof(C));
+    }
+};
+
+template <typename T>
+struct FastFft2 {
+    typedef std::complex<T> C;
+    int w, h;
+    StockhamFft1<T> fx, fy;
+    T scale;
+    FastFft2(int w_, int h_) : w(w_), h(h_), fx(w_), fy(h_) {
+        if (sizeof(T) == 4)
+            scale = (T)(1.0f / std::sqrt((float)(w_ * h_)));  // clfftSetPlanScale(1.0f/sqrtf(W*H))
+        else
+            scale = (T)(1.0 / std::sqrt((double)w_ * (double)h_));
+    }
+    void run(C* d, bool inv) const {
+        constexpr int G = 8;  // columns per gather
+        thread_local std::vector<C> cols, scratch;
+        cols.resize((size_t)G * h);
+        scratch.resize((size_t)std::max(w, h));
+        for (int y = 0; y < h; ++y) fx.run(d + (size_t)y * w, scratch.data(), inv);
+        for (int x0 = 0; x0 < w; x0 += G) {
+            const int g = std::min(G, w - x0);
+            for (int y = 0; y < h; ++y)
+                for (int c = 0; c < g; ++c) cols[(size_t)c * h + y] = d[(size_t)y * w + x0 + c];
+            for (int c = 0; c < g; ++c) fy.run(cols.data() + (size_t)c * h, scratch.data(), inv);
+            for (int y = 0; y < h; ++y)
+                for (int c = 0; c < g; ++c) d[(size_t)y * w + x0 + c] = cols[(size_t)c * h + y] * scale;
+        }
+    }
+};
+
 // k-grid of SimulationGeneral::initialiseSimulation (microscope/simulationgeneral.cpp:427-455)
 template <typename T>
 void kgrid(const OrcGeometry* g, std::vector<T>& kx, std::vector<T>& ky, double* bandwidthkmax) {
@@ -639,8 +758,8 @@ void probe_kspace(const OrcGeometry* g, T pos_x, T pos_y, const double* ab, T co
 }
 
 // doMultiSliceStep for one wave (microscope/simulationgeneral.cpp:744-764)
-template <typename T>
-void slice_step(const Fft2<T>& fft, size_t N, std::complex<T>* psi, std::complex<T>* recip,
+template <typename T, typename FFT>
+void slice_step(const FFT& fft, size_t N, std::complex<T>* psi, std::complex<T>* recip,
                 const std::complex<T>* t, const std::complex<T>* P) {
     for (size_t i = 0; i < N; ++i) {  // complex_multiply_d.cl:19-24
         T ax = t[i].real(), ay = t[i].imag(), bx = psi[i].real(), by = psi[i].imag();
@@ -688,7 +807,7 @@ double stem_pixel(int res, const std::complex<T>* recip, double inner, double ou
     return sum;
 }
 
-template <typename T>
+template <typename T, typename FFT = Fft2<T>>
 int run_stem(const OrcGeometry* g, const double* trans, const double* Pd, const double* pos,
              int n_probes, const double* aberr, double cond_ap, double smooth, int taper,
              const double* det, int ndet, int sqrt2, double* out, int nthreads) {
@@ -699,7 +818,7 @@ int run_stem(const OrcGeometry* g, const double* trans, const double* Pd, const 
     std::vector<C> t((size_t)ns * N), P(N);
     for (size_t i = 0; i < (size_t)ns * N; ++i) t[i] = C((T)trans[2 * i], (T)trans[2 * i + 1]);
     for (size_t i = 0; i < N; ++i) P[i] = C((T)Pd[2 * i], (T)Pd[2 * i + 1]);
-    Fft2<T> fft(res, res);
+    FFT fft(res, res);
     parallel_for(n_probes, nthreads, [&](int p) {
         std::vector<C> psi(N), recip(N);
         // simulationstem.cpp:178-188 -> simulationcbed.cpp:54-59: position relative to padded start
@@ -708,7 +827,7 @@ int run_stem(const OrcGeometry* g, const double* trans, const double* Pd, const 
         probe_kspace<T>(g, px, py, aberr, (T)cond_ap, (T)smooth, taper, recip.data());
         std::memcpy((void*)psi.data(), (const void*)recip.data(), N * sizeof(C));
         fft.run(psi.data(), true);
-        for (int s = 0; s < ns; ++s) slice_step<T>(fft, N, psi.data(), recip.data(), &t[(size_t)s * N], P.data());
+        for (int s = 0; s < ns; ++s) slice_step<T, FFT>(fft, N, psi.data(), recip.data(), &t[(size_t)s * N], P.data());
         for (int d = 0; d < ndet; ++d)
             out[(size_t)p * ndet + d] = stem_pixel<T>(res, recip.data(), det[4 * d], det[4 * d + 1],
                                                       det[4 * d + 2], det[4 * d + 3],
@@ -1141,8 +1260,18 @@ int orc_run_stem(const OrcGeometry* g, const double* trans, const double* P, con
 int orc_run_stem_f32(const OrcGeometry* g, const double* trans, const double* P, const double* pos,
                      int n_probes, const double* aberr, double cond_ap_mrad, double smooth_mrad,
                      int taper, const double* det, int ndet, int sqrt2, double* out, int nthreads) {
+    if (StockhamFft1<float>::fits(g->resolution))
+        return run_stem<float, FastFft2<float>>(g, trans, P, pos, n_probes, aberr, cond_ap_mrad, smooth_mrad,
+                                                taper, det, ndet, sqrt2, out, nthreads);
     return run_stem<float>(g, trans, P, pos, n_probes, aberr, cond_ap_mrad, smooth_mrad, taper,
                            det, ndet, sqrt2, out, nthreads);
+}
+
+int orc_fft2_fast(double* data, int w, int h, int inverse) {
+    if (!StockhamFft1<double>::fits(w) || !StockhamFft1<double>::fits(h)) return -1;
+    FastFft2<double> fft(w, h);
+    fft.run(reinterpret_cast<std::complex<double>*>(data), inverse != 0);
+    return 0;
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@
  *                                     carried between atoms of a block row and the
  *                                     cumulative tilt shift by explicit point-value sums
  *   orc_fft2 ........................ numpy.fft.fft2(norm="ortho") golden vectors
+ *   orc_fft2_fast ................... orc_fft2 and numpy on power-of-two sizes
  *   orc_propagator / band limit ..... exact masks and exp(-i pi lambda k^2 dz)
  *   orc_probe ....................... numpy chi(), aperture-disc pixel count
  *   orc_run_* ....................... empty specimen identities, numpy multislice
@@ -123,6 +124,9 @@ int orc_transmission(const OrcGeometry* g, const double* sx, const double* sy, c
                      int i_count, int first, int count, int band_limit, double* out, int nthreads);
 
 int orc_fft2(double* data, int w, int h, int inverse);
+/* the CPU baseline's FFT (iterative Stockham, power-of-two w and h; -1 otherwise), the same
+ * transform as orc_fft2 */
+int orc_fft2_fast(double* data, int w, int h, int inverse);
 int orc_band_limit(const OrcGeometry* g, double* t);
 int orc_propagator(const OrcGeometry* g, double* P);
 /* k-space probe (before the IFFT) for position (px, py) in Angstrom relative to

@@ -116,6 +116,7 @@ def lib():
         L.orc_transmission.argtypes = [C.POINTER(OrcGeometry), dp, dp, dp, ip, ip, dp, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int]
         L.orc_fft2.argtypes = [dp, C.c_int, C.c_int, C.c_int]
+        L.orc_fft2_fast.argtypes = [dp, C.c_int, C.c_int, C.c_int]
         L.orc_band_limit.argtypes = [C.POINTER(OrcGeometry), dp]
         L.orc_propagator.argtypes = [C.POINTER(OrcGeometry), dp]
         L.orc_probe_kspace.argtypes = [C.POINTER(OrcGeometry), C.c_double, C.c_double, dp,
@@ -300,6 +301,14 @@ def fft2(a, inverse=False):
     a = np.ascontiguousarray(a, dtype=np.complex128).copy()
     h, w = a.shape
     lib().orc_fft2(_dp(a.view(np.float64)), w, h, int(inverse))
+    return a
+
+
+def fft2_fast(a, inverse=False):
+    """The CPU baseline's FFT (power-of-two sizes), in double."""
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    h, w = a.shape
+    assert lib().orc_fft2_fast(_dp(a.view(np.float64)), w, h, int(inverse)) == 0
     return a
 
 

@@ -283,6 +283,24 @@ def test_fft_golden_vectors(oracle):
         np.testing.assert_allclose(oracle.fft2(a, inverse=True), d["inv_" + shape], rtol=0, atol=1e-13)
 
 
+@pytest.mark.parametrize("shape", [(2, 4), (8, 8), (16, 64), (512, 512), (1024, 256)])
+def test_fast_fft_matches(oracle, shape):
+    """The CPU baseline's Stockham transform (power-of-two sizes) is the oracle's FFT: numpy's
+    orthonormal transform both ways, and the oracle's own recursive one."""
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f = oracle.fft2_fast(a)
+    np.testing.assert_allclose(f, np.fft.fft2(a, norm="ortho"), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(oracle.fft2_fast(a, inverse=True), np.fft.ifft2(a, norm="ortho"), rtol=0,
+                               atol=1e-13)
+    np.testing.assert_allclose(f, oracle.fft2(a), rtol=0, atol=1e-13)
+
+
+def test_fast_fft_refuses_other_sizes(oracle):
+    a = np.zeros((384, 384), complex)
+    assert oracle.lib().orc_fft2_fast(oracle._dp(a.view(np.float64)), 384, 384, 0) == -1
+
+
 @pytest.mark.parametrize("n", [256, 384, 768])
 def test_fft_identities(oracle, n):
     rng = np.random.default_rng(n)
