@@ -789,9 +789,10 @@ int mse_build_transmission(mse_engine* e, int cfg, int first, int count) {
         if (c.trans.p != before) upload_cfg_ptrs(e);
         const mse::PotArgs a = pot_args(e, c);
         float2* outp = c.trans.p + (size_t)first * e->N;
-        // potential kernels build up to 16 slices per launch (a few ms at most), which also
-        // keeps each launch's tail small against its work
-        const int chunk = 16;
+        // the full-3D kernels build up to 16 slices per launch (a few ms at most), the
+        // projected ones up to 64 (under a millisecond), which also keeps each launch's tail
+        // small against its work
+        const int chunk = g.full3d ? 16 : 64;
         for (int s0 = 0; s0 < count; s0 += chunk) {
             const int cs = std::min(chunk, count - s0);
             potential_planes(e, c, g, a, outp + (size_t)s0 * e->N, cs, first + s0);
