@@ -99,8 +99,10 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         // the phases from the same (tabulated or direct) potentials as the forward build
         pa.vtab = potential_tables(e, c, g);
         const bool tab = pa.vtab != nullptr;
-        for (int s0 = 0; s0 < n; s0 += 16) {
-            const int cs = std::min(16, n - s0);
+        // as mse_build_transmission: full 3D 32 slices per launch, projected up to 256
+        const int chunk = g.full3d ? 32 : 256;
+        for (int s0 = 0; s0 < n; s0 += chunk) {
+            const int cs = std::min(chunk, n - s0);
             dim3 grid(e->res / 16, e->res / 16, cs);
             float* o = e->adj_phase.p + (size_t)s0 * N;
             launch(e, PK_POT, [&]() {
