@@ -474,10 +474,10 @@ __device__ __forceinline__ int tile_pixel_y(int j) {
 }
 
 // Projected potential sums (sumz) of slice `slice` at PX x PY pixels per thread, sumz[j PX + i]
-// for pixel (tile_pixel_x(i), tile_pixel_y(j)).  The atoms of each block row are staged
-// through LDS in chunks of 256, with no limit on how many a row holds.  (The tables stay in
-// global memory here: one evaluation per atom and pixel is too little work to repay an LDS
-// copy per block.)
+// for pixel (tile_pixel_x(i), tile_pixel_y(j)).  The atoms of the tile's block rows are staged
+// through LDS as one sequence, in chunks of 256, with no limit on how many a row holds.  (The
+// tables stay in global memory here: one evaluation per atom and pixel is too little work to
+// repay an LDS copy per block.)
 template <int FORM, bool TAB, int PX = 1, int PY = 1>
 __device__ __forceinline__ void projected_sumz(const double* __restrict__ ax, const double* __restrict__ ay,
                                                const int* __restrict__ aZ, const int* __restrict__ bsp,
