@@ -724,8 +724,9 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
     // The atoms of the rows (k, j), k = topz..bottomz, j = startj..endj in that order (row
     // (k, j): sorted positions [bsp(k, j, starti), bsp(k, j, endi + 1))) are staged as one
     // sequence, 256 at a time, so that rows holding a few atoms each share a staging round.
-    // Bit 24 of sh0 marks the first atom of a row, where p2 starts again from 0.
-    constexpr int kRowStart = 1 << 24;
+    // Bit 24 of sh0 marks the first atom of a row, where p2 starts again from 0; bit 25 an
+    // atom whose points may reach past the LDS tables' top bucket (see the clamp below).
+    constexpr int kRowStart = 1 << 24, kDeep = 1 << 25;
     double p2[NX];
 #pragma unroll
     for (int i = 0; i < NX; ++i) p2[i] = 0.0;
@@ -800,7 +801,13 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                     }
                 }
                 info_w |= h0 | (int)(waves << 16);
-                srz[lid] = (float)((current_z - h0 * a.dz * int_r) - zl);
+                const float rz0 = (float)((current_z - h0 * a.dz * int_r) - zl);
+                srz[lid] = rz0;
+                if constexpr (LDS) {
+                    const float stepf = (float)(a.dz * int_r);
+                    const float zmax = (fabsf(rz0) + (float)(a.integrals - h0) * stepf) * 1.001f;
+                    if (!(fmaf(zmax, zmax, 64.f) <= r_hi2)) info_w |= kDeep;
+                }
             }
             sh0[lid] = info_w;
         }
@@ -867,12 +874,11 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
                     // when no lane of the wave needs it: every r^2 = fma(rz, rz, xy2) is
                     // at least xy2 >= r_lo2, and at most 64 + zmax^2, zmax bounding |rz|
                     // over the atom's points (|rz0| + its depth span; the 1e-3 margin
-                    // covers the float steps' rounding, < 4096 * 2^-24 relative).
+                    // covers the float steps' rounding, < 4096 * 2^-24 relative).  The
+                    // bound is the same in every lane, tested once per atom at staging
+                    // (kDeep).
                     bool clamp = true;
-                    if constexpr (LDS) {
-                        const float zmax = (fabsf(rz0) + (float)(a.integrals - h0) * stepf) * 1.001f;
-                        clamp = __builtin_amdgcn_ballot_w64(low || !(fmaf(zmax, zmax, 64.f) <= r_hi2)) != 0;
-                    }
+                    if constexpr (LDS) clamp = (info & kDeep) || __builtin_amdgcn_ballot_w64(low) != 0;
                     auto sweep = [&](auto clamped) {
                         auto eval = [&](float r2) {
                             if constexpr (LDS)
