@@ -730,6 +730,7 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
     double p2[NX];
 #pragma unroll
     for (int i = 0; i < NX; ++i) p2[i] = 0.0;
+    bool row_fresh = false;
     int k = topz, j = startj, p = 0, re = 0;
     bool more = topz <= bottomz && startj <= endj, fresh = true;
     if (more) {
@@ -817,13 +818,17 @@ __device__ __forceinline__ void full3d_sumz(const double* __restrict__ ax, const
             // wave's reach, in depth or laterally, are passed over by the whole wave
             int h0 = 0;
             const int info = __builtin_amdgcn_readfirstlane(sh0[l]);
-            if (info & kRowStart) {
-#pragma unroll
-                for (int i = 0; i < NX; ++i) p2[i] = 0.0;
-            }
+            // a new row restarts p2; atoms the wave passes over leave p2 as it is, so the
+            // restart is applied at the row's first atom the wave takes
+            row_fresh = row_fresh || (info & kRowStart);
             if (no_shift) {
                 if (!((info >> (16 + wave)) & 1)) continue;
                 h0 = info & 0xffff;
+            }
+            if (row_fresh) {
+#pragma unroll
+                for (int i = 0; i < NX; ++i) p2[i] = 0.0;
+                row_fresh = false;
             }
             const double xl = sx[l];
             const double rad_y0 = im_pos_y - sy[l];
