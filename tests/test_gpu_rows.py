@@ -360,16 +360,17 @@ def test_full3d_tables_beyond_the_lds_budget(engine, oracle):
     assert e_td <= 2e-6 and e_t <= 1e-5
 
 
-@pytest.mark.parametrize("full3d", [0, 1])
-def test_dense_rows_span_staging_chunks(engine, oracle, full3d):
+@pytest.mark.parametrize("full3d,tilt", [(0, 0.0), (1, 0.0), (1, 6.0)])
+def test_dense_rows_span_staging_chunks(engine, oracle, full3d, tilt):
     """The builders stage the atoms of a pixel tile's block rows (and, in full 3D, of every
     slice of its depth window) as one sequence, 256 at a time.  A dense slab puts several
     hundred atoms in each tile's sequence, so chunk boundaries fall inside block rows: the
-    full-3D trapezoid must carry p2 across them and restart it only at a row's first atom.
-    Transmission functions against the FP64 oracle."""
+    full-3D trapezoid must carry p2 across them and restart it only at a row's first atom
+    (also with a tilted beam, where no atom is passed over).  Transmission functions against
+    the FP64 oracle."""
     if full3d:
         spec = sp.random_box({"C": 420}, (12.0, 12.0, 2.5), 0.6, seed=21)
-        extra = dict(slice_thickness=1.0, full3d=1, full3d_integrals=6)
+        extra = dict(slice_thickness=1.0, full3d=1, full3d_integrals=6, beam_tilt_mrad=tilt, beam_azimuth=0.4)
     else:
         spec = sp.random_box({"C": 700}, (12.0, 12.0, 1.5), 0.5, seed=22)
         extra = dict(slice_thickness=2.0)
@@ -386,7 +387,7 @@ def test_dense_rows_span_staging_chunks(engine, oracle, full3d):
         per_seq = [per_slice[max(0, s - lz): s + lz + 1].sum() for s in range(g.n_slices)]
     else:
         per_seq = list(per_slice)
-    print(f"full3d={full3d}: {g.n_slices} slices, atoms per slice {per_slice.tolist()}, "
+    print(f"full3d={full3d} tilt={tilt}: {g.n_slices} slices, atoms per slice {per_slice.tolist()}, "
           f"largest staging sequence {max(per_seq)}")
     assert max(per_seq) > 300
     trans = oracle.transmission(og, o, p.values, p.form, p.i_per_atom)
