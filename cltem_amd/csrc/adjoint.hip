@@ -34,6 +34,15 @@ void copy_planes(mse_engine* e, float2* dst, const float2* src, size_t count) {
     HIPX(hipMemcpyAsync(dst, src, count * sizeof(float2), hipMemcpyDeviceToDevice, e->stream));
 }
 
+// One forward slice from `in` into `out` (in != out): the row pass reads one buffer and
+// writes the other, so recomputed states need no copy of their predecessor.
+void slice_step_into(mse_engine* e, const float2* in, float2* out, int nw, int slice) {
+    mse::RowArgs ra = base_row_args(e);
+    ra.slice = slice;
+    row_pass(e, mse::ROW_IFFT_T_FFT, in, out, nw, e->N, ra);
+    col_pass(e, mse::COL_PROP, out, out, nw, e->N, base_col_args(e));
+}
+
 }  // namespace
 }  // namespace mse_rt
 
@@ -152,19 +161,18 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         const int n_seg = (n + K - 1) / K;
         for (int j = n_seg - 1; j >= 0; --j) {
             const int start = j * K, len = std::min(K, n - start);
-            // recompute the forward states of the segment from its checkpoint
-            copy_planes(e, e->adj_seg.p, e->adj_ckpt.p + (size_t)j * plane, plane);
-            for (int i = 1; i < len; ++i) {
-                float2* cur = e->adj_seg.p + (size_t)i * plane;
-                copy_planes(e, cur, cur - plane, plane);
-                slice_step(e, cur, nw, start + i - 1);
-            }
+            // recompute the forward states of the segment from its checkpoint: state i of
+            // the segment is the checkpoint for i = 0, else slot i of adj_seg
+            const float2* ck = e->adj_ckpt.p + (size_t)j * plane;
+            auto state = [&](int i) -> const float2* { return i == 0 ? ck : e->adj_seg.p + (size_t)i * plane; };
+            for (int i = 1; i < len; ++i)
+                slice_step_into(e, state(i - 1), e->adj_seg.p + (size_t)i * plane, nw, start + i - 1);
             for (int s = start + len - 1; s >= start; --s) {
                 col_pass(e, mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
                 float2* gs = e->adj_gtau.p + (size_t)s * N;
                 mse::RowArgs rb = base_row_args(e);
                 rb.slice = s;
-                rb.aux_in = e->adj_seg.p + (size_t)(s - start) * plane;
+                rb.aux_in = state(s - start);
                 rb.aux_out = nw == 1 ? gs : e->adj_gt.p;  // one wave: its g_t is the sum
                 row_pass(e, mse::ROW_ADJ, e->adj_g.p, e->adj_g.p, nw, N, rb);
                 if (nw > 1)
