@@ -444,6 +444,11 @@ def bench_adjoint(args):
     ns, n_atoms = g.n_slices, len(spec)
     log(f"{dev}: {n_atoms} atoms, {g.resolution}^2, {ns} slices, adjoint position gradients")
     build_config(engine, 0, tgt)
+    # the sweep variants for one wave (at one wave the passes are latency-bound, and the
+    # variant that spreads a pass over the most blocks need not be the default)
+    tune = engine.autotune(0, 1, reps=10)
+    log(f"autotune (1 wave): row variant {tune['best_row']}, column variant {tune['best_col']}; "
+        f"column us {[round(x, 1) for x in tune['col_us']]}")
     engine.set_wave_count(1)
     engine.set_wave_configs([0])
     engine.init_plane_wave(1.0)
