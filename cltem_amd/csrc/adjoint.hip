@@ -157,6 +157,19 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         row_pass(e, mse::ROW_ADJ_SEED, e->adj_exit.p, e->adj_g.p, nw, N, ra);
 
         mse::ColArgs ca = base_col_args(e);
+        // Without per-wave beam tables the backward column pass is the forward propagator pass
+        // (with the variant tuned for nw waves) on the conjugated px, py tables; with them it
+        // conjugates the per-wave factors itself (COL_PROP_CONJ).
+        const bool conj_tables = ca.wave_prop == nullptr;
+        if (conj_tables) {
+            e->adj_pconj.ensure(2 * (size_t)e->res);
+            launch(e, PK_OTHER, [&]() {
+                mse::k_conj_tables<<<(unsigned)((2 * e->res + 255) / 256), 256, 0, e->stream>>>(e->px.p, e->py.p, e->res,
+                                                                                              e->adj_pconj.p);
+            });
+            ca.px = e->adj_pconj.p;
+            ca.py = e->adj_pconj.p + e->res;
+        }
         e->adj_gtau.ensure((size_t)std::max(n, 1) * N);  // g_t of every slice, summed over the waves
         const int n_seg = (n + K - 1) / K;
         for (int j = n_seg - 1; j >= 0; --j) {
@@ -168,7 +181,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
             for (int i = 1; i < len; ++i)
                 slice_step_into(e, state(i - 1), e->adj_seg.p + (size_t)i * plane, nw, start + i - 1);
             for (int s = start + len - 1; s >= start; --s) {
-                col_pass(e, mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
+                col_pass(e, conj_tables ? mse::COL_PROP : mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
                 float2* gs = e->adj_gtau.p + (size_t)s * N;
                 mse::RowArgs rb = base_row_args(e);
                 rb.slice = s;

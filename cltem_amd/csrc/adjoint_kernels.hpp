@@ -73,6 +73,17 @@ static __global__ void k_sum_planes(const float2* __restrict__ in, int nw, size_
     out[p] = s;
 }
 
+// out[0, n) = conj(px), out[n, 2n) = conj(py): conj(P') = conj(px) conj(py) is the
+// propagator of the backward column pass (negation is exact, so the products equal those of
+// the conjugated forward factor bit for bit).
+static __global__ void k_conj_tables(const float2* __restrict__ px, const float2* __restrict__ py, int n,
+                                     float2* __restrict__ out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= 2 * n) return;
+    const float2 v = i < n ? px[i] : py[i - n];
+    out[i] = make_float2(v.x, -v.y);
+}
+
 static __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
