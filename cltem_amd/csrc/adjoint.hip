@@ -217,11 +217,14 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
                 const int* bsp = c.block_start.p;
                 launch(e, PK_OTHER, [&]() {
                     if (e->form == 0)
-                        mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
+                        tab ? mse::k_atom_grad_projected<0, true><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p)
+                            : mse::k_atom_grad_projected<0><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
                     else if (e->form == 1)
-                        mse::k_atom_grad_projected<1><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
+                        tab ? mse::k_atom_grad_projected<1, true><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p)
+                            : mse::k_atom_grad_projected<1><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
                     else
-                        mse::k_atom_grad_projected<2><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
+                        tab ? mse::k_atom_grad_projected<2, true><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p)
+                            : mse::k_atom_grad_projected<2><<<grid, 256, 0, e->stream>>>(c.x.p, c.y.p, c.Z.p, c.orig.p, a_first, bsp, nblk, n, e->params.p, pa, gt, ph, e->adj_grad.p);
                 });
             }
         }

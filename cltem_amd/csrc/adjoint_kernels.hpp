@@ -89,14 +89,30 @@ static __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+// dV/d(r2) of the cubic that vtab_eval_in evaluates at r2, for kVtabFloor <= r2 < 2^kOctaveHi:
+// the derivative of what the tabulated forward sums.  With r2 = 2^e (1 + (i + m 2^-kMantBits)
+// 2^-kSubBits), m the low kMantBits mantissa bits, dm/d(r2) = 2^(23 - e), a float of biased
+// exponent 127 + 23 - e = 277 - bits(r2) >> 23 (142..170 over the tables' range).
+__device__ __forceinline__ float vtab_slope_in(unsigned long long base, float r2) {
+    const unsigned bits = __float_as_uint(r2);
+    const unsigned off = (bits >> (mse_tab::kMantBits - 4)) & ~15u;
+    using gfloat4 = const __attribute__((address_space(1))) float4;
+    const float4 c = *(const float4*)reinterpret_cast<gfloat4*>(base + off);
+    const float m = (float)(bits & ((1u << mse_tab::kMantBits) - 1u));
+    const float dv_dm = fmaf(fmaf(3.0f * c.w, m, 2.0f * c.z), m, c.y);
+    return dv_dm * __uint_as_float((277u - (bits >> 23)) << 23);
+}
+
 // One block of 256 threads per sorted atom of slices 0..n_slices-1 (sorted range
 // [first, first + gridDim.x)); its slice s is found from the slice starts bs[k * nblk].
 // The block visits the pixels of the atom's 8 A neighbourhood, keeps those whose tile block
 // range contains the atom's block (as the forward sum does), and reduces
 //   -sigma * dL/dphi(p) * v'(r) * dr/d(rad) in double, with dL/dphi from slice s's planes
-// g_tau + s N and phase + s N.  grad has 3 doubles (x, y, z) per input atom, indexed
-// through orig.
-template <int FORM>
+// g_tau + s N and phase + s N.  TAB: v'(r) / r = 2 dV/d(r2) of the element's table (the
+// derivative of the tabulated forward, with its clamp r2 < r_min^2 and its cutoff), else the
+// derivative of the formulas.  grad has 3 doubles (x, y, z) per input atom, indexed through
+// orig.
+template <int FORM, bool TAB = false>
 __global__ __launch_bounds__(256) void k_atom_grad_projected(
     const double* __restrict__ ax, const double* __restrict__ ay, const int* __restrict__ aZ,
     const int* __restrict__ orig, int first, const int* __restrict__ bs, int nblk, int n_slices,
@@ -133,6 +149,9 @@ __global__ __launch_bounds__(256) void k_atom_grad_projected(
     const int wx = x1 - x0 + 1, wy = y1 - y0 + 1;
     const int npix = (wx > 0 && wy > 0) ? wx * wy : 0;
     const double r_min = 0.25 * a.pixelscale;
+    const double r_min2 = r_min * r_min;
+    unsigned long long tz = 0;
+    if constexpr (TAB) tz = vtab_base(a.vtab + (size_t)(aZ[i] - 1) * mse_tab::kBuckets);
     // d(z', x', y')/d(rad_x, rad_y) of the tilted metric
     const double dzx = -0.5 * a.cos_phi * a.sin_2theta, dzy = -0.5 * a.sin_phi * a.sin_2theta;
     const double dxx = 1.0 + dzx / a.tan_theta * a.cos_phi, dxy = dzy / a.tan_theta * a.cos_phi;
@@ -141,9 +160,6 @@ __global__ __launch_bounds__(256) void k_atom_grad_projected(
     for (int q = threadIdx.x; q < npix; q += 256) {
         const int xid = x0 + q % wx;
         const int yid = y0 + q / wx;
-        int starti, endi, startj, endj;
-        tile_block_range(a, xid >> 4, yid >> 4, starti, endi, startj, endj);
-        if (bidx < starti || bidx > endi || bidy < startj || bidy > endj) continue;
         const double rad_x = (a.startx + xid * a.pixelscale) - xa;
         const double rad_y = (a.starty + yid * a.pixelscale) - ya;
         const double z_prime = -0.5 * (rad_x * a.cos_phi + rad_y * a.sin_phi) * a.sin_2theta;
@@ -151,17 +167,30 @@ __global__ __launch_bounds__(256) void k_atom_grad_projected(
         const double x_prime = rad_x + zt * a.cos_phi;
         const double y_prime = rad_y + zt * a.sin_phi;
         const double r2 = z_prime * z_prime + x_prime * x_prime + y_prime * y_prime;
-        if (!(r2 <= 64.0000000001)) continue;
-        const double rad = sqrt(r2);
-        if (rad < r_min || rad > 8.0) continue;  // clamped (constant) or outside the cutoff
+        double rad = 0.0;
+        if constexpr (TAB) {
+            // clamped to V(r_min) (constant) or outside the cutoff r2 <= 64 + 2^-46
+            if (!(r2 <= 0x1.0000000000001p+6) || r2 < r_min2) continue;
+        } else {
+            if (!(r2 <= 64.0000000001)) continue;
+            rad = sqrt(r2);
+            if (rad < r_min || rad > 8.0) continue;  // clamped (constant) or outside the cutoff
+        }
+        int starti, endi, startj, endj;
+        tile_block_range(a, xid >> 4, yid >> 4, starti, endi, startj, endj);
+        if (bidx < starti || bidx > endi || bidy < startj || bidy > endj) continue;
         const size_t p = (size_t)yid * a.res + xid;
         float s, c;
         sincosf(phase[p], &s, &c);
         const float2 g = g_tau[p];
         // dL/dphi = -Im(conj(g) tau)
         const double dl_dphi = -((double)g.x * s - (double)g.y * c);
-        const double dv = (double)dvproj<FORM>(pz, a.i_count, (float)rad);
-        const double f = a.sigma * dl_dphi * dv / rad;
+        double dv_r;  // v'(r) / r
+        if constexpr (TAB)
+            dv_r = 2.0 * (double)vtab_slope_in(tz, (float)r2);
+        else
+            dv_r = (double)dvproj<FORM>(pz, a.i_count, (float)rad) / rad;
+        const double f = a.sigma * dl_dphi * dv_r;
         const double dr_drx = z_prime * dzx + x_prime * dxx + y_prime * dyx;  // times rad
         const double dr_dry = z_prime * dzy + x_prime * dxy + y_prime * dyy;
         gx -= f * dr_drx;  // d rad_x / d x_a = -1
