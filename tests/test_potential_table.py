@@ -78,3 +78,38 @@ def test_projected_cutoff_threshold():
     for x in xs:
         ref = x <= 64.0000000001 and np.sqrt(x) <= 8.0
         assert (x <= t) == ref, x
+
+
+def _device_slope(tab, r2):
+    """dV/d(r2) as the tabulated position-gradient kernel computes it: the coefficients in the
+    low mantissa bits m (c_k 2^(-18 k), float), dV/dm = (3 c3 m + 2 c2) m + c1 in float, times
+    dm/d(r2) = 2^(23 - e), the float of biased exponent 277 - (bits >> 23)."""
+    r2 = np.asarray(r2, dtype=np.float32)
+    bits = r2.view(np.uint32)
+    c = tab[(bits >> 18).astype(np.int64) - ((127 - 20) << 5)].astype(np.float64)
+    d = np.stack([c[:, 1] * 2.0 ** -18, c[:, 2] * 2.0 ** -36, c[:, 3] * 2.0 ** -54], 1).astype(np.float32)
+    m = (bits & ((1 << 18) - 1)).astype(np.float32)
+    dv_dm = (np.float32(3) * d[:, 2] * m + np.float32(2) * d[:, 1]) * m + d[:, 0]
+    scale = ((277 - (bits >> 23)).astype(np.uint32) << 23).view(np.float32)
+    return dv_dm.astype(np.float64) * scale.astype(np.float64)
+
+
+@pytest.mark.parametrize("name,Z", [("kirkland", 79), ("kirkland", 6), ("peng", 14), ("lobato", 8)])
+def test_table_slope_matches_formula_derivative(oracle, name, Z):
+    """The projected gradient kernel takes v'(r) = 2 r dV/d(r2) from the slope of the table's
+    cubic.  Against a central difference of the FP64 formulas from 0.05 to 8 A: within 1e-5 of
+    max |v'| (a wrong exponent or a dropped term of the slope misses by orders of magnitude),
+    and on the cubic's own terms it is the exact derivative: the slope at a bucket start is c1
+    times 2^(23 - e) / 2^18."""
+    p = P.load(name)
+    tab = cltem_amd.potential_table(p, Z, False)
+    r = np.exp(np.linspace(np.log(0.05), np.log(7.99), 1500))
+    r2 = np.float32(r) ** 2
+    rr = np.sqrt(r2.astype(np.float64))
+    h = 1e-5 * rr
+    dv = (_oracle_v(oracle, p, Z, rr + h, False) - _oracle_v(oracle, p, Z, rr - h, False)) / (2 * h)
+    got = 2 * rr * _device_slope(tab, r2)
+    assert np.max(np.abs(got - dv)) <= 1e-5 * np.max(np.abs(dv))
+    # r2 = 1 starts bucket 640 (m = 0): the slope is c1 / width, width = 2^0 / 32
+    assert _device_slope(tab, [1.0])[0] == pytest.approx(float(np.float32(tab[640, 1] * 2.0 ** -18)) * 2.0 ** 18 * 32,
+                                                          rel=1e-7)
