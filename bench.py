@@ -445,10 +445,17 @@ def bench_adjoint(args):
     log(f"{dev}: {n_atoms} atoms, {g.resolution}^2, {ns} slices, adjoint position gradients")
     build_config(engine, 0, tgt)
     # the sweep variants for one wave (at one wave the passes are latency-bound, and the
-    # variant that spreads a pass over the most blocks need not be the default)
-    tune = engine.autotune(0, 1, reps=10)
-    log(f"autotune (1 wave): row variant {tune['best_row']}, column variant {tune['best_col']}; "
-        f"column us {[round(x, 1) for x in tune['col_us']]}")
+    # variant that spreads a pass over the most blocks need not be the default), or the ones
+    # given: under a kernel tracer every launch is slowed, and the tuner's choice with it
+    if args.variants:
+        rv, cv = (int(x) for x in args.variants.split(","))
+        engine.set_kernel_variant(0, rv)
+        engine.set_kernel_variant(1, cv)
+        log(f"sweep variants given: row {rv}, column {cv}")
+    else:
+        tune = engine.autotune(0, 1, reps=10)
+        log(f"autotune (1 wave): row variant {tune['best_row']}, column variant {tune['best_col']}; "
+            f"column us {[round(x, 1) for x in tune['col_us']]}")
     engine.set_wave_count(1)
     engine.set_wave_configs([0])
     engine.init_plane_wave(1.0)
@@ -577,6 +584,8 @@ def main():
                     help="probes of the first timed configuration checked against the FP64 oracle")
     ap.add_argument("--roof-batch", type=int, default=64, help="batch size of the roofline measurement")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variants", default="",
+                    help="adjoint: row,column sweep variants to use instead of autotuning them")
     ap.add_argument("--graph", type=int, default=1, help="replay each probe batch as a captured hipGraph")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: one batch of --batch probes through every slice, "
