@@ -148,7 +148,7 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
 
         e->adj_g.ensure(plane);
         e->adj_gt.ensure(plane);  // per-wave g_t
-        e->adj_seg.ensure((size_t)K * plane);
+        e->adj_seg.ensure((size_t)std::max(K - 1, 1) * plane);  // states 1..K-1 of a segment
         HIPX(hipStreamSynchronize(e->stream));
 
         // seed: res F_x (2 w psi_N)
@@ -175,11 +175,11 @@ int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz) 
         for (int j = n_seg - 1; j >= 0; --j) {
             const int start = j * K, len = std::min(K, n - start);
             // recompute the forward states of the segment from its checkpoint: state i of
-            // the segment is the checkpoint for i = 0, else slot i of adj_seg
+            // the segment is the checkpoint for i = 0, else slot i - 1 of adj_seg
             const float2* ck = e->adj_ckpt.p + (size_t)j * plane;
-            auto state = [&](int i) -> const float2* { return i == 0 ? ck : e->adj_seg.p + (size_t)i * plane; };
-            for (int i = 1; i < len; ++i)
-                slice_step_into(e, state(i - 1), e->adj_seg.p + (size_t)i * plane, nw, start + i - 1);
+            auto slot = [&](int i) { return e->adj_seg.p + (size_t)(i - 1) * plane; };
+            auto state = [&](int i) -> const float2* { return i == 0 ? ck : slot(i); };
+            for (int i = 1; i < len; ++i) slice_step_into(e, state(i - 1), slot(i), nw, start + i - 1);
             for (int s = start + len - 1; s >= start; --s) {
                 col_pass(e, conj_tables ? mse::COL_PROP : mse::COL_PROP_CONJ, e->adj_g.p, e->adj_g.p, nw, N, ca);
                 float2* gs = e->adj_gtau.p + (size_t)s * N;

@@ -362,7 +362,7 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
  * between atoms of one block row.  Gradients are undefined where a coordinate sits exactly
  * on a cutoff of the forward sum (slice and block edges, the 8 A lateral and 3 A depth
  * limits); MSE_ERR_UNSUPPORTED under mixed static potentials.  Device memory: the
- * checkpoints (ceil(n / K) * n_waves planes), one segment (K * n_waves planes), and the
+ * checkpoints (ceil(n / K) * n_waves planes), one segment ((K - 1) * n_waves planes), and the
  * wave-summed g_t and the phase of every slice (n complex and n float planes). */
 int mse_adjoint_positions(mse_engine* e, const double* dl_di, double* grad_xyz);
 /* ---- sweep timing: `reps` back-to-back row passes (slice `slice`, every wave with its own
