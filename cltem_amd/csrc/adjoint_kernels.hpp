@@ -91,8 +91,8 @@ static __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
 
 // dV/d(r2) of the cubic that vtab_eval_in evaluates at r2, for kVtabFloor <= r2 < 2^kOctaveHi:
 // the derivative of what the tabulated forward sums.  With r2 = 2^e (1 + (i + m 2^-kMantBits)
-// 2^-kSubBits), m the low kMantBits mantissa bits, dm/d(r2) = 2^(23 - e), a float of biased
-// exponent 127 + 23 - e = 277 - bits(r2) >> 23 (142..170 over the tables' range).
+// 2^-kSubBits), m the low kMantBits mantissa bits, dm/d(r2) = 2^(23 - e), the float of biased
+// exponent 127 + 23 - e = 277 - (bits(r2) >> 23) (142..170 over the tables' range).
 __device__ __forceinline__ float vtab_slope_in(unsigned long long base, float r2) {
     const unsigned bits = __float_as_uint(r2);
     const unsigned off = (bits >> (mse_tab::kMantBits - 4)) & ~15u;
