@@ -322,12 +322,18 @@ class Engine:
         return out
 
     def forward_checkpointed(self, config: int = 0) -> np.ndarray:
-        """Run every slice of `config` from the current entrance waves, keeping sqrt(n)
-        checkpoints for adjoint_positions; returns |psi_exit|^2, shape (n_waves, res, res)."""
+        """Run every slice of `config` from the current entrance waves, keeping every state
+        (within the checkpoint budget) or sqrt(n) checkpoints for adjoint_positions; returns
+        |psi_exit|^2, shape (n_waves, res, res)."""
         res = self.g.resolution
         out = np.zeros((self.n_waves, res, res))
         _check(self.lib.mse_forward_checkpointed(self.h, config, _dp(out)))
         return out
+
+    def set_checkpoint_budget(self, nbytes: int):
+        """Device bytes forward_checkpointed may spend on keeping every forward state (default
+        1 GiB); 0 always keeps sqrt(n) checkpoints and recomputes the states in between."""
+        _check(self.lib.mse_set_checkpoint_budget(self.h, int(nbytes)))
 
     def adjoint_positions(self, dl_di, n_atoms: int) -> np.ndarray:
         """dL/d(x, y, z) of every atom of the last sort of the checkpointed configuration

@@ -176,6 +176,7 @@ SIGNATURES = [
     ("mse_set_reference_offset", C.c_int, [_eng, C.c_int, C.c_double, C.c_double]),
     ("mse_forward_checkpointed", C.c_int, [_eng, C.c_int, _dp]),
     ("mse_adjoint_positions", C.c_int, [_eng, _dp, _dp]),
+    ("mse_set_checkpoint_budget", C.c_int, [_eng, C.c_ulonglong]),
     ("mse_stem_batch", C.c_int, [_eng, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),
                                  C.POINTER(MseDetector), C.c_int, C.c_int, C.c_int, _dp]),
     ("mse_stem_batch_frames", C.c_int, [_eng, C.c_int, _dp, _dp, C.c_int, C.c_int, C.POINTER(MseProbe),

@@ -2,7 +2,8 @@
 
 `exit_intensity(positions, engine, Z)` is the CTEM plane-wave exit intensity |psi_exit|^2
 as a differentiable function of the atom positions: the forward runs the engine's slice
-loop with sqrt(N) checkpoints (mse_forward_checkpointed) and the backward runs the adjoint
+loop keeping every state, or sqrt(N) checkpoints beyond the engine's checkpoint budget
+(mse_forward_checkpointed), and the backward runs the adjoint
 slice loop (mse_adjoint_positions) with the incoming dL/dI.  Positions are float64 (n, 3)
 tensors on the CPU: they are host data that the engine bins on the GPU.
 """

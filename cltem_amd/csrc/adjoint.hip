@@ -58,14 +58,24 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity) {
         const int n = cfg_geom(e, c).n_slices;
         const int nw = e->n_waves;
         const size_t plane = (size_t)nw * e->N;
-        const int K = ceil_sqrt(std::max(n, 1));
+        // every state when all of them fit the budget (no recompute), else sqrt(n) checkpoints
+        const bool keep_all = (double)n * (double)plane * sizeof(float2) <= (double)e->adj_budget;
+        const int K = keep_all ? 1 : ceil_sqrt(std::max(n, 1));
         const int n_ck = (n + K - 1) / K;
         e->adj_ready = false;
         e->adj_ckpt.ensure(std::max(n_ck, 1) * plane);
         e->adj_exit.ensure(plane);
+        // The state before slice s is checkpoint s / K when s % K == 0: the slice before writes
+        // it there directly and slice s reads it there, so checkpoints cost no copies (other
+        // slices step the engine's waves in place).
+        if (n > 0) copy_planes(e, e->adj_ckpt.p, e->waves.p, plane);
         for (int s = 0; s < n; ++s) {
-            if (s % K == 0) copy_planes(e, e->adj_ckpt.p + (size_t)(s / K) * plane, e->waves.p, plane);
-            slice_step(e, e->waves.p, nw, s);
+            const float2* in = s % K == 0 ? e->adj_ckpt.p + (size_t)(s / K) * plane : e->waves.p;
+            float2* out = (s + 1 < n && (s + 1) % K == 0) ? e->adj_ckpt.p + (size_t)((s + 1) / K) * plane : e->waves.p;
+            if (in == out)
+                slice_step(e, out, nw, s);
+            else
+                slice_step_into(e, in, out, nw, s);
         }
         copy_planes(e, e->adj_exit.p, e->waves.p, plane);
         if (intensity) {
@@ -83,6 +93,13 @@ int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity) {
         e->adj_nslices = n;
         e->adj_version = c.version;
         e->adj_ready = true;
+    });
+}
+
+int mse_set_checkpoint_budget(mse_engine* e, unsigned long long bytes) {
+    return run_guarded("mse_set_checkpoint_budget", e, [&]() {
+        REQUIRE(e, MSE_ERR_ARG, "engine is NULL");
+        e->adj_budget = (size_t)bytes;
     });
 }
 

@@ -203,6 +203,8 @@ struct mse_engine {
     // adjoint slice loop: checkpoints of the last checkpointed forward run
     DevBuf<float2> adj_ckpt, adj_exit, adj_seg, adj_g, adj_gt, adj_gtau;
     DevBuf<float2> adj_pconj;  // conj(px), conj(py): the backward column pass's tables
+    // forward states of all slices are kept (no recompute) when they take at most this many bytes
+    size_t adj_budget = (size_t)1 << 30;
     DevBuf<float> adj_phase;
     DevBuf<double> adj_w, adj_grad;
     int adj_cfg = -1, adj_k = 0, adj_nw = 0, adj_nslices = 0;

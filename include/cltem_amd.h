@@ -346,14 +346,20 @@ int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_u
  *      projected or full-3D transmission functions of initialiseSimulation
  *      (simulationgeneral.cpp:345-646).  ---- */
 /* From the current entrance waves (all referring to configuration cfg), run every slice of
- * cfg, keeping ceil(n/K) checkpoints of the waves, K = ceil(sqrt(n_slices)).  Leaves the
+ * cfg, keeping ceil(n/K) checkpoints of the waves: K = 1 (every state, so that the adjoint
+ * recomputes nothing) when the n_slices states of all waves take at most the checkpoint
+ * budget, else K = ceil(sqrt(n_slices)).  Leaves the
  * exit waves in the engine like mse_multislice; when intensity is not NULL writes
  * |psi_exit|^2 as n_waves * N doubles (wave-major, rows of res). */
 int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
+/* Device bytes that mse_forward_checkpointed may spend on keeping every forward state
+ * (n_slices * n_waves * res^2 * 8); default 1 GiB, 0 always keeps sqrt(n) checkpoints. */
+int mse_set_checkpoint_budget(mse_engine* e, unsigned long long bytes);
 /* Gradient of a loss L(I) of the exit intensities of the last mse_forward_checkpointed run:
  * dl_di = dL/dI (n_waves * N doubles, same layout as the intensities; for
  * L = sum (I - I_target)^2 it is 2 (I - I_target)).  Recomputes each segment of K slices
- * from its checkpoint and runs the adjoint slice loop backwards.  grad_xyz receives
+ * from its checkpoint (nothing when every state was kept) and runs the adjoint slice loop
+ * backwards.  grad_xyz receives
  * dL/dx, dL/dy, dL/dz for every atom of the last mse_sort_atoms call on cfg, in input order
  * (3 doubles per atom; atoms outside the simulated volume get 0).  With projected
  * potentials dL/dz = 0 because an atom's potential does not depend on its depth within its

@@ -150,6 +150,43 @@ def test_checkpointed_forward_equals_multislice_and_waves_add(engine, oracle):
     assert np.allclose(g3, g1, rtol=1e-5, atol=1e-9 * np.abs(g1).max())
 
 
+def test_all_states_and_sqrt_checkpoints_agree(engine):
+    """forward_checkpointed keeps every forward state when they fit the checkpoint budget,
+    else sqrt(n) checkpoints whose segments the adjoint recomputes.  Both routes give the
+    intensities of multislice and the same gradients, to float rounding (a slice can run out of
+    place on one route and in place on the other)."""
+    spec = sp.random_box({"Au": 10, "C": 30}, (12.0, 12.0, 9.0), 1.2, seed=2)
+    lo, hi = spec.limits
+    scen = dict(mode=1, resolution=256, voltage_kv=200.0, slice_thickness=1.0, struct_min=tuple(lo),
+                struct_max=tuple(hi), area=(lo[0], hi[0], lo[1], hi[1]))
+    g, p = setup(engine, scen, "kirkland")
+    engine.sort_atoms(0, spec.x, spec.y, spec.z, spec.Z)
+    engine.build_transmission(0)
+    engine.set_wave_count(2)
+    engine.set_wave_configs([0, 0])
+    engine.init_plane_wave(1.0)
+    engine.multislice()
+    ref = np.abs(engine.exit_wave(0)) ** 2
+    w = np.random.default_rng(5).standard_normal((2, 256, 256))
+    out = {}
+    try:
+        for budget in (1 << 30, 0):
+            engine.set_checkpoint_budget(budget)
+            engine.init_plane_wave(1.0)
+            I = engine.forward_checkpointed(0)
+            out[budget] = (I, engine.adjoint_positions(w, len(spec.Z)))
+    finally:
+        engine.set_checkpoint_budget(1 << 30)
+    print(f"{g.n_slices} slices")
+    for budget, (I, _) in out.items():
+        for wv in (0, 1):
+            assert np.abs(I[wv] - ref).max() <= 1e-6 * ref.max(), (budget, wv)
+    g_all, g_ck = out[1 << 30][1], out[0][1]
+    rel = np.linalg.norm(g_all - g_ck) / np.linalg.norm(g_ck)
+    print(f"every state kept vs sqrt(n) checkpoints: gradient rel-L2 {rel:.2e}")
+    assert np.abs(g_ck).max() > 0 and rel <= 1e-5
+
+
 def test_adjoint_with_wave_beam_tables(engine):
     """The backward column pass is the forward propagator pass on conjugated px, py tables, or,
     where waves carry beam tables of their own, a pass that conjugates each wave's factors.  A
