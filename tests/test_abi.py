@@ -55,6 +55,8 @@ def test_engine_calls_fail_loudly_on_null_handle():
     lib = _abi.load()
     assert lib.mse_synchronize(None) != 0
     assert b"NULL" in lib.mse_last_error() or b"engine" in lib.mse_last_error()
+    assert lib.mse_set_checkpoint_budget(None, 0) != 0
+    assert b"NULL" in lib.mse_last_error()
 
 
 def test_engine_creation_without_gpu_is_an_error():
