@@ -348,9 +348,9 @@ int mse_autotune(mse_engine* e, int config, int n_waves, int reps, double* row_u
 /* From the current entrance waves (all referring to configuration cfg), run every slice of
  * cfg, keeping ceil(n/K) checkpoints of the waves: K = 1 (every state, so that the adjoint
  * recomputes nothing) when the n_slices states of all waves take at most the checkpoint
- * budget, else K = ceil(sqrt(n_slices)).  Leaves the
- * exit waves in the engine like mse_multislice; when intensity is not NULL writes
- * |psi_exit|^2 as n_waves * N doubles (wave-major, rows of res). */
+ * budget, else K = ceil(sqrt(n_slices)).  Leaves the exit waves in the engine like
+ * mse_multislice; when intensity is not NULL writes |psi_exit|^2 as n_waves * N doubles
+ * (wave-major, rows of res). */
 int mse_forward_checkpointed(mse_engine* e, int cfg, double* intensity);
 /* Device bytes that mse_forward_checkpointed may spend on keeping every forward state
  * (n_slices * n_waves * res^2 * 8); default 1 GiB, 0 always keeps sqrt(n) checkpoints. */
